@@ -1,0 +1,199 @@
+"""eff_pot_amd: MI355X (gfx950) Orsay-Trento imaginary-time solver.
+
+Python host binding of libeffpot.so (HIP kernels + C ABI, include/effpot.h).
+The operators mirror the reference's zero-argument Fortran routines on module
+state (poten.f90 `poten()`, energy.f90 `energy()`, steprk.f90 `steprk(dt)`,
+steppc.f90 `steppc(dt, errHe, ...)`, init_deriv_parallel.f90 `derivnd`), as
+methods of a context that owns the device state of `batch` independent solves.
+There is no CPU fallback: without the built library every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (Params, Impurity, Energies, SolveOpts, Result, OK, FIELDS,  # noqa: F401
+                  make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
+                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, H4OT)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libeffpot.so")
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class EffpotError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libeffpot.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m eff_pot_amd.build` "
+                              "(the HIP path is the only implementation)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.effpot_ctx_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(vp)]
+        L.effpot_ctx_destroy.argtypes = [vp]
+        L.effpot_last_error.argtypes = [vp]
+        L.effpot_last_error.restype = C.c_char_p
+        L.effpot_footprint_bytes.argtypes = [C.POINTER(Params)]
+        L.effpot_footprint_bytes.restype = C.c_int64
+        L.effpot_upload_psi.argtypes = [vp, C.c_int, _dp]
+        L.effpot_download_psi.argtypes = [vp, C.c_int, _dp]
+        L.effpot_download_den.argtypes = [vp, C.c_int, _dp]
+        L.effpot_download_field.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.effpot_init_fermi.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, _dp]
+        L.effpot_set_impurity.argtypes = [vp, C.c_int, C.POINTER(Impurity)]
+        L.effpot_poten.argtypes = [vp]
+        L.effpot_energy.argtypes = [vp, C.POINTER(Energies)]
+        L.effpot_step_rk.argtypes = [vp, C.c_double]
+        L.effpot_step_pc.argtypes = [vp, C.c_double, C.POINTER(C.c_double)]
+        L.effpot_renormalize.argtypes = [vp]
+        L.effpot_derivnd.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.effpot_fft_r2c.argtypes = [vp, _dp, _dp]
+        L.effpot_fft_c2r.argtypes = [vp, _dp, _dp]
+        L.effpot_set_active.argtypes = [vp, C.c_int, C.c_int]
+        L.effpot_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Result)]
+        L.effpot_sync.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def footprint_bytes(params: Params) -> int:
+    return int(lib().effpot_footprint_bytes(C.byref(params)))
+
+
+class Context:
+    """Device state of `params.batch` solves on one GPU (opaque effpot_ctx)."""
+
+    def __init__(self, params: Params, device: int = 0):
+        self.p = params
+        self.h = C.c_void_p()
+        rc = lib().effpot_ctx_create(C.byref(params), device, C.byref(self.h))
+        if rc != OK:
+            raise EffpotError(rc, "effpot_ctx_create failed")
+        self.B = params.batch
+        self.shape = (params.nz, params.ny, params.nx)
+        self.hshape = (params.nz, params.ny, params.nx // 2 + 1)
+
+    def _chk(self, rc):
+        if rc != OK:
+            msg = lib().effpot_last_error(self.h)
+            raise EffpotError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self.h:
+            lib().effpot_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state -----------------------------------------------------------
+    def upload_psi(self, psi, slot=0):
+        psi = np.asarray(psi).reshape(self.shape)
+        buf = np.stack([psi.real, psi.imag], axis=-1).astype(np.float64).copy()
+        self._chk(lib().effpot_upload_psi(self.h, slot, buf.reshape(-1)))
+
+    def psi(self, slot=0):
+        out = np.zeros(self.shape + (2,))
+        self._chk(lib().effpot_download_psi(self.h, slot, out.reshape(-1)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def psi_raw(self, slot=0):
+        out = np.zeros(self.shape + (2,))
+        self._chk(lib().effpot_download_psi(self.h, slot, out.reshape(-1)))
+        return out
+
+    def den(self, slot=0):
+        out = np.zeros(self.shape)
+        self._chk(lib().effpot_download_den(self.h, slot, out.reshape(-1)))
+        return out
+
+    def field(self, which, slot=0):
+        out = np.zeros(self.shape)
+        self._chk(lib().effpot_download_field(self.h, slot, FIELDS.get(which, which), out.reshape(-1)))
+        return out
+
+    def init_fermi(self, rho0, R, afermi, noise, seed, centre=(0.0, 0.0, 0.0), slot=0):
+        self._chk(lib().effpot_init_fermi(self.h, slot, rho0, R, afermi, noise, seed,
+                                          np.asarray(centre, dtype=np.float64)))
+
+    def set_impurity(self, imp: Impurity, slot=0):
+        self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))
+
+    def set_active(self, slot, active):
+        self._chk(lib().effpot_set_active(self.h, slot, 1 if active else 0))
+
+    # ---- operators (reference routine names) -------------------------------
+    def poten(self):
+        self._chk(lib().effpot_poten(self.h))
+
+    def energy(self):
+        e = (Energies * self.B)()
+        self._chk(lib().effpot_energy(self.h, e))
+        return [x.as_dict() for x in e]
+
+    def step_rk(self, dt):
+        self._chk(lib().effpot_step_rk(self.h, dt))
+
+    def step_pc(self, dt):
+        err = (C.c_double * self.B)()
+        self._chk(lib().effpot_step_pc(self.h, dt, err))
+        return list(err)
+
+    def renormalize(self):
+        self._chk(lib().effpot_renormalize(self.h))
+
+    def derivnd(self, id_, iv, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.shape)
+        out = np.zeros_like(f)
+        self._chk(lib().effpot_derivnd(self.h, id_, iv, f.reshape(-1), out.reshape(-1)))
+        return out
+
+    def fft_r2c(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.shape)
+        out = np.zeros(self.hshape + (2,))
+        self._chk(lib().effpot_fft_r2c(self.h, f.reshape(-1), out.reshape(-1)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def fft_c2r(self, F):
+        F = np.asarray(F).reshape(self.hshape)
+        buf = np.stack([F.real, F.imag], axis=-1).astype(np.float64).copy()
+        out = np.zeros(self.shape)
+        self._chk(lib().effpot_fft_c2r(self.h, buf.reshape(-1), out.reshape(-1)))
+        return out
+
+    def solve(self, opts: SolveOpts):
+        r = (Result * self.B)()
+        self._chk(lib().effpot_solve(self.h, C.byref(opts), r))
+        return list(r)
+
+    def sync(self):
+        self._chk(lib().effpot_sync(self.h))
+
+
+def default_dt(params: Params, cap=1000.0):
+    """Largest stable step of the Hamming PC integrator with margin:
+    0.25 / (h2o2m4 * 7.07 * sum 1/h^2 + cap + 50 K); 7.07 = max |symbol| of
+    the 13-point second derivative."""
+    hx = 2 * params.xmax / params.nx
+    hy = 2 * params.ymax / params.ny
+    hz = 2 * params.zmax / params.nz
+    lam = params.h2o2m4 * 7.0729 * (1 / hx ** 2 + 1 / hy ** 2 + 1 / hz ** 2) + cap + 50.0
+    return 0.25 / lam

@@ -1,0 +1,1090 @@
+// C-ABI of the gfx950 Orsay-Trento solver (declared in include/effpot.h).
+//
+// Host orchestration only: every array operation below is a HIP kernel from
+// fft.hpp / stencil.hpp / misc.hpp launched on the context's stream.  The
+// psi / H psi history of the integrators is a per-slot ring of buffer indices
+// kept in device memory and rotated by a one-thread-per-slot kernel, so that
+// slots can run different integrator phases (RK starter vs predictor-
+// corrector) and no host round trip is needed between steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/effpot.h"
+#include "common.hpp"
+#include "fft.hpp"
+#include "misc.hpp"
+#include "stencil.hpp"
+
+using namespace ep;
+
+namespace {
+
+enum Role { R_CUR = 0, R_O1, R_O2, R_O3, R_SPARE, R_H1, R_H2, R_OUT };
+enum RingOp { RING_RESET = 0, RING_BEFORE_RK0, RING_AFTER_RK0, RING_BEFORE_RKJ, RING_AFTER_RKJ, RING_AFTER_PRED,
+              RING_AFTER_CORR };
+
+__global__ void k_ring(int* ring, const int* active, int B, int op) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B || !active[s]) return;
+  int* r = ring + 8 * s;
+  switch (op) {
+    case RING_RESET:
+      r[R_CUR] = 0; r[R_O1] = 1; r[R_O2] = 2; r[R_O3] = 3; r[R_SPARE] = 4; r[R_H1] = 0; r[R_H2] = 1; r[R_OUT] = 0;
+      break;
+    case RING_BEFORE_RK0: {
+      r[R_OUT] = r[R_O3]; r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = r[R_CUR];
+      int t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
+    } break;
+    case RING_AFTER_RK0: r[R_CUR] = r[R_OUT]; break;
+    case RING_BEFORE_RKJ: r[R_OUT] = r[R_SPARE]; break;
+    case RING_AFTER_RKJ: r[R_SPARE] = r[R_CUR]; r[R_CUR] = r[R_OUT]; break;
+    case RING_AFTER_PRED: {
+      int t = r[R_CUR]; r[R_CUR] = r[R_O3]; r[R_O3] = t;
+      t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
+    } break;
+    case RING_AFTER_CORR: {
+      int fin = r[R_SPARE], mod = r[R_CUR], n = r[R_O3];
+      r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = n; r[R_SPARE] = mod; r[R_CUR] = fin;
+    } break;
+  }
+}
+
+// Pointer of ring role r of slot s: all + (s*nbuf + ring[s][r]) * N.
+template <typename T> struct Ring {
+  cx<T>* psi;  // [B][5][N]
+  cx<T>* hp;   // [B][2][N]
+  const int* ring;
+  long N;
+  EP_DEV cx<T>* p(int s, int r) const { return psi + ((long)s * 5 + ring[8 * s + r]) * N; }
+  EP_DEV cx<T>* h(int s, int r) const { return hp + ((long)s * 2 + ring[8 * s + r]) * N; }
+};
+
+// ---- kernels that need the ring --------------------------------------------
+template <typename T, int TX, int MODE>  // MODE 0 predict, 1 correct, 2 rk stage
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs<T> a, Ring<T> rg) {
+  const int s = blockIdx.z;
+  if (!a.active[s]) return;
+  // resolve the roles of this slot into the argument block, then run the
+  // shared kernel body (the StepArgs pointers are slot-0 based; offsets are
+  // re-applied by the body, so pass slot-relative pointers minus the offset).
+  const long so = (long)s * a.slot;
+  StepArgs<T> b = a;
+  b.cur = rg.p(s, R_CUR) - so;
+  b.out = (MODE == 1 ? rg.p(s, R_SPARE) : rg.p(s, R_OUT)) - so;
+  b.o1 = rg.p(s, R_O1) - so; b.o2 = rg.p(s, R_O2) - so; b.o3 = rg.p(s, R_O3) - so;
+  b.h1 = rg.h(s, R_H1) - so; b.h2 = rg.h(s, R_H2) - so;
+  if constexpr (MODE == 0) k_pc_predict_body<T, TX>(b);
+  else if constexpr (MODE == 1) k_pc_correct_body<T, TX>(b);
+  else k_rk_stage_body<T, TX>(b);
+}
+
+template <typename T>
+__global__ void k_renorm_ring(Ring<T> rg, T* den, const double* scale, const int* active, long N) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const T f = (T)scale[s];
+  cx<T>* psi = rg.p(s, R_CUR);
+  T* d = den + (long)s * N;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    cx<T> p = f * psi[i];
+    psi[i] = p;
+    d[i] = p.x * p.x + p.y * p.y;
+  }
+}
+
+// partial sums of |psi_cur|^2 in the same [slot][block][2] layout as the steps
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_norm(Ring<T> rg, const int* active, long N, double* partial) {
+  __shared__ double red[8];
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const cx<T>* psi = rg.p(s, R_CUR);
+  double v[2] = {0.0, 0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    cx<T> p = psi[i];
+    v[1] += (double)(p.x * p.x + p.y * p.y);
+  }
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    double* q = partial + ((long)s * gridDim.x + blockIdx.x) * 2;
+    q[0] = 0.0; q[1] = v[1];
+  }
+}
+
+template <typename T>
+__global__ void k_den_from_cur(Ring<T> rg, T* den, const int* active, long N) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const cx<T>* psi = rg.p(s, R_CUR);
+  T* d = den + (long)s * N;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    cx<T> p = psi[i];
+    d[i] = p.x * p.x + p.y * p.y;
+  }
+}
+
+template <typename T, int TX, int ID>
+__global__ __launch_bounds__(kThreads) void k_deriv1(Grid g, StenCoef<T> sc, int zc, const T* f, T* df, int axis) {
+  __shared__ T lds[Tile<TX>::LX * Tile<TX>::LY];
+  const int z0 = blockIdx.y * zc, z1 = min(z0 + zc, g.nz);
+  zmarch<T, T, TX, ID == 1, ID == 2>(f, g, sc, z0, z1, lds, [&](int x, int y, int z, bool in, T, T(&d1)[3], T(&d2)[3]) {
+    if (!in) return;
+    df[g.idx(x, y, z)] = ID == 1 ? d1[axis] : d2[axis];
+  });
+}
+
+// ---------------------------------------------------------------------------
+template <typename T> struct State {
+  cx<T>* psi = nullptr;  // [B][5][N]
+  cx<T>* hp = nullptr;   // [B][2][N]
+  cx<T>* pc = nullptr;   // [B][N]
+  cx<T>* q = nullptr;
+  T *den = nullptr, *pot4 = nullptr, *uimp = nullptr, *dencg = nullptr, *delj4 = nullptr, *denalf = nullptr;
+  T *ua = nullptr, *ualphas = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr, *sto1 = nullptr;
+  T *intx = nullptr, *inty = nullptr, *intz = nullptr, *div = nullptr, *src[4] = {nullptr, nullptr, nullptr, nullptr};
+  cx<T>* spec[8] = {};  // each [B][M]
+  T* kern[3] = {};      // wcgk, fvlj4, kalfs: [M]
+  cx<T>* tw[3] = {};    // twiddles per axis
+  StenCoef<T> sc;
+  int* ring = nullptr;  // [B][8]
+};
+
+}  // namespace
+
+struct effpot_ctx {
+  effpot_params p;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int nx, ny, nz, nxh, B;
+  long N, M;
+  double hx, hy, hz, dxyz;
+  int npot;
+  double delinter;
+  State<double> s64;
+  State<float> s32;
+  bool mixed = false;
+  double* imptab = nullptr;  // [B][npot+2]
+  int* d_active = nullptr;   // current mask
+  int* d_ones = nullptr;
+  int* d_mask_tmp = nullptr;
+  std::vector<int> h_active;
+  std::vector<int> have_imp;
+  double* d_partial = nullptr;
+  int nblk_max = 0;
+  double* d_red = nullptr;    // [B][16]
+  double* d_scale = nullptr;  // [B]
+  int* d_bad = nullptr;
+  double* h_pin = nullptr;    // pinned host buffer
+  int tx = 64, zc = 16;
+  std::vector<void*> allocs;
+  std::string err;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      return ctx->fail(EFFPOT_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define LAUNCHCHK()                                                                                \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+int tstgrid(int n) {
+  if (n < 16 || n > 256) return 1;
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  while (m % 5 == 0) m /= 5;
+  return m != 1;
+}
+
+void fd_coeffs(int id, double* c) {  // same closed forms as the Bickley table rows (mates.f:1117-1125, 1202-1210)
+  const double p2 = 518400.0;
+  static const double fact[13] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880, 3628800, 39916800, 479001600};
+  for (int m = -6; m <= 6; m++) {
+    int am = m < 0 ? -m : m;
+    if (m == 0) { c[6] = id == 1 ? 0.0 : -5369.0 / 1800.0; continue; }
+    double den = fact[6 - am] * fact[6 + am];
+    double sgn = (am % 2 == 1) ? 1.0 : -1.0;
+    if (id == 1) c[m + 6] = (m > 0 ? sgn : -sgn) * (p2 / ((double)am * den));
+    else c[m + 6] = sgn * (2.0 * p2 / ((double)(am * am) * den));
+  }
+}
+
+template <typename T> int dalloc(effpot_ctx* ctx, T** p, size_t count) {
+  void* v = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&v, count * sizeof(T));
+  if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  ctx->allocs.push_back(v);
+  *p = (T*)v;
+  return EFFPOT_OK;
+}
+
+template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s) {
+  const long N = ctx->N, M = ctx->M, B = ctx->B;
+  int rc;
+#define A_(ptr, cnt) if ((rc = dalloc(ctx, &(ptr), (size_t)(cnt))) != EFFPOT_OK) return rc
+  A_(s.psi, B * 5 * N); A_(s.hp, B * 2 * N); A_(s.pc, B * N); A_(s.q, B * N);
+  T** reals[] = {&s.den, &s.pot4, &s.uimp, &s.dencg, &s.delj4, &s.denalf, &s.ua, &s.ualphas, &s.dx, &s.dy,
+                 &s.dz, &s.sto1, &s.intx, &s.inty, &s.intz, &s.div, &s.src[0], &s.src[1], &s.src[2], &s.src[3]};
+  for (T** r : reals) A_(*r, B * N);
+  for (int j = 0; j < 8; j++) A_(s.spec[j], B * M);
+  for (int j = 0; j < 3; j++) A_(s.kern[j], M);
+  A_(s.tw[0], ctx->nx); A_(s.tw[1], ctx->ny); A_(s.tw[2], ctx->nz);
+  A_(s.ring, B * 8);
+#undef A_
+  // zero the whole state (history, pc, q, fields)
+  HIPCHK(hipMemsetAsync(s.psi, 0, sizeof(cx<T>) * B * 5 * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.hp, 0, sizeof(cx<T>) * B * 2 * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.pc, 0, sizeof(cx<T>) * B * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.q, 0, sizeof(cx<T>) * B * N, ctx->st));
+  for (T** r : reals) HIPCHK(hipMemsetAsync(*r, 0, sizeof(T) * B * N, ctx->st));
+  // twiddles
+  int ns[3] = {ctx->nx, ctx->ny, ctx->nz};
+  for (int a = 0; a < 3; a++) {
+    std::vector<cx<T>> tw(ns[a]);
+    for (int k = 0; k < ns[a]; k++) {
+      double ang = -2.0 * M_PI * (double)k / (double)ns[a];
+      tw[k] = cx<T>{(T)std::cos(ang), (T)std::sin(ang)};
+    }
+    HIPCHK(hipMemcpy(s.tw[a], tw.data(), sizeof(cx<T>) * ns[a], hipMemcpyHostToDevice));
+  }
+  // stencil coefficients: c_m / h^id in FP64, then rounded to T
+  double c1[13], c2[13];
+  fd_coeffs(1, c1);
+  fd_coeffs(2, c2);
+  double hs[3] = {ctx->hx, ctx->hy, ctx->hz};
+  for (int a = 0; a < 3; a++) {
+    double s1 = 1.0 / hs[a], s2 = 1.0 / (hs[a] * hs[a]);
+    for (int m = 0; m < 13; m++) {
+      s.sc.d1[a][m] = (T)(c1[m] * s1);
+      s.sc.d2[a][m] = (T)(c2[m] * s2);
+    }
+  }
+  std::vector<int> ring(B * 8);
+  for (int b = 0; b < B; b++) {
+    int init[8] = {0, 1, 2, 3, 4, 0, 1, 0};
+    for (int k = 0; k < 8; k++) ring[b * 8 + k] = init[k];
+  }
+  HIPCHK(hipMemcpy(s.ring, ring.data(), sizeof(int) * B * 8, hipMemcpyHostToDevice));
+  return EFFPOT_OK;
+}
+
+dim3 pw_grid(long n, int nslots) {
+  long b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  return dim3((unsigned)b, nslots);
+}
+
+// ---- FFT launchers -----------------------------------------------------------
+#define FFT_SWITCH(n, M_)                                              \
+  switch (n) {                                                         \
+    case 16: M_(16); break; case 24: M_(24); break; case 32: M_(32); break; \
+    case 48: M_(48); break; case 64: M_(64); break; case 96: M_(96); break; \
+    case 128: M_(128); break; case 192: M_(192); break; case 256: M_(256); break; \
+    default: return ctx->fail(EFFPOT_ERR_GRID, "unsupported FFT length"); \
+  }
+
+int n1_of(int n) {
+  switch (n) { case 16: return 4; case 24: case 32: case 48: case 64: return 8; default: return 16; }
+}
+
+template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf, const int* act) {
+  XArgs<T> a{};
+  for (int f = 0; f < nf; f++) { a.r[f] = in[f]; a.c[f] = out[f]; }
+  a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = T(1); a.twN = s.tw[0]; a.active = act;
+  const int C = kThreads / n1_of(ctx->nx);
+  dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_x_r2c<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
+  FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* in, T* const* out, int nf, T scale, const int* act) {
+  XArgs<T> a{};
+  for (int f = 0; f < nf; f++) { a.r[f] = out[f]; a.c[f] = in[f]; }
+  a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = scale; a.twN = s.tw[0]; a.active = act;
+  const int C = kThreads / n1_of(ctx->nx);
+  dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_x_c2r<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
+  FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+LineMap ymap(effpot_ctx* ctx) { return LineMap{ctx->nxh, (long)ctx->nxh * ctx->ny, (long)ctx->nxh, ctx->nxh * ctx->nz}; }
+LineMap zmap(effpot_ctx* ctx) {
+  return LineMap{ctx->nxh * ctx->ny, 0, (long)ctx->nxh * ctx->ny, ctx->nxh * ctx->ny};
+}
+template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx<T>* const* f, int nf, int sign, const int* act) {
+  StridedArgs<T> a{};
+  for (int j = 0; j < nf; j++) a.f[j] = f[j];
+  a.sstride = ctx->M; a.lm = axis == 1 ? ymap(ctx) : zmap(ctx); a.twN = s.tw[axis]; a.sign = sign; a.active = act;
+  const int n = axis == 1 ? ctx->ny : ctx->nz;
+  const int C = kThreads / n1_of(n);
+  dim3 g((a.lm.nlines + C - 1) / C, nf, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_strided<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
+  FFT_SWITCH(n, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act) {
+  a.sstride = ctx->M; a.lm = zmap(ctx); a.twN = s.tw[2]; a.active = act;
+  a.scale = (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz));
+  const int C = kThreads / n1_of(ctx->nz);
+  dim3 g((a.lm.nlines + C - 1) / C, ngroups, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
+  FFT_SWITCH(ctx->nz, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+// ---- stencil launchers -----------------------------------------------------------
+#define TX_SWITCH(tx, M_)                         \
+  switch (tx) {                                   \
+    case 64: M_(64); break; case 32: M_(32); break; \
+    case 16: M_(16); break; default: M_(8); break;  \
+  }
+
+template <int TX> int nblk_of(effpot_ctx* ctx) {
+  dim3 g = zmarch_grid<TX>(ctx->nx, ctx->ny, ctx->nz, ctx->zc, 1);
+  return (int)(g.x * g.y);
+}
+int nblk(effpot_ctx* ctx) {
+  int r = 0;
+#define M_(TXX) r = nblk_of<TXX>(ctx)
+  TX_SWITCH(ctx->tx, M_)
+#undef M_
+  return r;
+}
+dim3 sgrid(effpot_ctx* ctx) {
+  dim3 g;
+#define M_(TXX) g = zmarch_grid<TXX>(ctx->nx, ctx->ny, ctx->nz, ctx->zc, ctx->B)
+  TX_SWITCH(ctx->tx, M_)
+#undef M_
+  return g;
+}
+Grid grid_of(effpot_ctx* ctx) {
+  return Grid{ctx->nx, ctx->ny, ctx->nz, ctx->p.icon == EFFPOT_ICON_PERIODIC ? 1 : 0, ctx->N};
+}
+
+template <typename T> Ring<T> ring_of(effpot_ctx* ctx, State<T>& s) { return Ring<T>{s.psi, s.hp, s.ring, ctx->N}; }
+
+template <typename T> int ring_op(effpot_ctx* ctx, State<T>& s, int op, const int* act) {
+  hipLaunchKernelGGL(k_ring, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, s.ring, act, ctx->B, op);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+template <typename T> StepArgs<T> step_args(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
+  StepArgs<T> a{};
+  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dt = (T)dt;
+  a.active = act; a.slot = ctx->N; a.pc = s.pc; a.q = s.q; a.pot4 = s.pot4; a.uimp = s.uimp; a.den = s.den;
+  a.partial = ctx->d_partial; a.nblk = nblk(ctx);
+  return a;
+}
+
+template <typename T, int MODE> int launch_step(effpot_ctx* ctx, State<T>& s, StepArgs<T>& a) {
+  Ring<T> rg = ring_of(ctx, s);
+  dim3 g = sgrid(ctx);
+#define M_(TXX) hipLaunchKernelGGL((k_step<T, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
+  TX_SWITCH(ctx->tx, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act) {
+  hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
+                     act, mode, ctx->p.n4, ctx->dxyz, ctx->d_scale);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+#define RC(x) do { int rc_ = (x); if (rc_ != EFFPOT_OK) return rc_; } while (0)
+
+// ---- poten (poten.f90:5-131 + term_alfa.f90:5-109) ----------------------------
+template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) {
+  const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
+  // stage A: FFT(den) -> dencg, delj4 [, denalf]
+  {
+    T* in[1] = {s.den};
+    cx<T>* sp[1] = {s.spec[0]};
+    RC(fft_x_r2c(ctx, s, in, sp, 1, act));
+    RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    ConvArgs<T> ca{};
+    ca.in[0] = s.spec[0];
+    ca.nout[0] = otc ? 3 : 2;
+    ca.out[0][0] = s.spec[1]; ca.kern[0][0] = s.kern[0];
+    ca.out[0][1] = s.spec[2]; ca.kern[0][1] = s.kern[1];
+    ca.out[0][2] = s.spec[3]; ca.kern[0][2] = s.kern[2];
+    RC(fft_zconv(ctx, s, ca, 1, act));
+    cx<T>* o[3] = {s.spec[1], s.spec[2], s.spec[3]};
+    RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
+    T* r[3] = {s.dencg, s.delj4, s.denalf};
+    RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act));
+  }
+  // derden + FFT sources
+  {
+    PotenArgs<T> a{};
+    a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.otc = otc;
+    a.cp4 = (T)ctx->p.cp4; a.cpp4 = (T)ctx->p.cpp4; a.den0s = (T)ctx->p.den0s;
+    a.den = s.den; a.dencg = s.dencg; a.denalf = s.denalf; a.dx = s.dx; a.dy = s.dy; a.dz = s.dz;
+    a.s0 = s.src[0]; a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3];
+    dim3 g = sgrid(ctx);
+#define M_(TXX) hipLaunchKernelGGL((k_grad<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
+    TX_SWITCH(ctx->tx, M_)
+#undef M_
+    LAUNCHCHK();
+  }
+  // stage B: correlation convolution [+ alpha_s integrals]
+  {
+    const int nf = otc ? 4 : 1;
+    T* in[4] = {s.src[0], s.src[1], s.src[2], s.src[3]};
+    cx<T>* sp[4] = {s.spec[4], s.spec[5], s.spec[6], s.spec[7]};
+    RC(fft_x_r2c(ctx, s, in, sp, nf, act));
+    RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
+    ConvArgs<T> ca{};
+    for (int g = 0; g < nf; g++) {
+      ca.in[g] = sp[g];
+      ca.nout[g] = 1;
+      ca.out[g][0] = s.spec[g];
+      ca.kern[g][0] = g == 0 ? s.kern[0] : s.kern[2];
+    }
+    RC(fft_zconv(ctx, s, ca, nf, act));
+    cx<T>* o[4] = {s.spec[0], s.spec[1], s.spec[2], s.spec[3]};
+    RC(fft_strided(ctx, s, 1, o, nf, +1, act));
+    T* r[4] = {s.sto1, s.intx, s.inty, s.intz};
+    RC(fft_x_c2r(ctx, s, o, r, nf, T(1), act));
+  }
+  if (otc) {
+    DivArgs<T> a{};
+    a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.den0s = (T)ctx->p.den0s;
+    a.denalf = s.denalf; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz; a.dx = s.dx; a.dy = s.dy; a.dz = s.dz;
+    a.src = s.src[0]; a.div = s.div;
+    dim3 g = sgrid(ctx);
+#define M_(TXX) hipLaunchKernelGGL((k_div<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
+    TX_SWITCH(ctx->tx, M_)
+#undef M_
+    LAUNCHCHK();
+    // stage C: FFT(sum_i d_i den int_i) * kalfs
+    T* in[1] = {s.src[0]};
+    cx<T>* sp[1] = {s.spec[4]};
+    RC(fft_x_r2c(ctx, s, in, sp, 1, act));
+    RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    ConvArgs<T> ca{};
+    ca.in[0] = s.spec[4]; ca.nout[0] = 1; ca.out[0][0] = s.spec[0]; ca.kern[0][0] = s.kern[2];
+    RC(fft_zconv(ctx, s, ca, 1, act));
+    cx<T>* o[1] = {s.spec[0]};
+    RC(fft_strided(ctx, s, 1, o, 1, +1, act));
+    T* r[1] = {s.ua};
+    RC(fft_x_c2r(ctx, s, o, r, 1, T(1), act));
+  }
+  {
+    Pot4Args<T> a{};
+    a.N = ctx->N; a.slot = ctx->N; a.active = act; a.otc = otc;
+    a.a0 = (T)(ctx->p.cp4 / 2.0); a.a1 = (T)(ctx->p.cpp4 / 3.0); a.den0s = (T)ctx->p.den0s;
+    a.h2o2m4 = (T)ctx->p.h2o2m4; a.alphas = (T)ctx->p.alphas; a.C = (T)ctx->p.C; a.beta = (T)ctx->p.beta;
+    a.den_m = (T)ctx->p.den_m;
+    a.delj4 = s.delj4; a.dencg = s.dencg; a.sto1 = s.sto1; a.ua = s.ua; a.div = s.div; a.den = s.den;
+    a.pot4 = s.pot4; a.ualphas = s.ualphas;
+    hipLaunchKernelGGL((k_pot4<T>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, a);
+    LAUNCHCHK();
+  }
+  return EFFPOT_OK;
+}
+
+// ---- integrators ----------------------------------------------------------------
+template <typename T> int renorm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
+  hipLaunchKernelGGL((k_renorm_ring<T>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, ring_of(ctx, s), s.den,
+                     ctx->d_scale, act, ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+// scale[s] = sqrt(n4 / (sum |psi|^2 dxyz)) for the live psi
+template <typename T> int norm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
+  hipLaunchKernelGGL((k_norm<T>), dim3(nblk(ctx), ctx->B), dim3(kThreads), 0, ctx->st, ring_of(ctx, s), act, ctx->N,
+                     ctx->d_partial);
+  LAUNCHCHK();
+  return reduce<2>(ctx, 1, act);
+}
+
+// RK-Gill step (steprk.f90:286-405).  Leaves the norm reduction in d_red/d_scale.
+template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
+  const double r2 = std::sqrt(2.0);
+  const double arun[4] = {0.5, 1.0 - 1.0 / r2, 1.0 + 1.0 / r2, 1.0 / 6.0};
+  const double brun[4] = {2.0, 1.0, 1.0, 2.0};
+  const double crun[4] = {0.5, 1.0 - 1.0 / r2, 1.0 + 1.0 / r2, 0.5};
+  for (int j = 0; j < 4; j++) {
+    StepArgs<T> a = step_args(ctx, s, dt, act);
+    a.arun = (T)arun[j]; a.brun = (T)brun[j]; a.crun = (T)crun[j]; a.first_stage = j == 0;
+    RC(ring_op(ctx, s, j == 0 ? RING_BEFORE_RK0 : RING_BEFORE_RKJ, act));
+    RC((launch_step<T, 2>(ctx, s, a)));
+    RC(ring_op(ctx, s, j == 0 ? RING_AFTER_RK0 : RING_AFTER_RKJ, act));
+    if (j <= 2) RC(poten_t(ctx, s, act));
+  }
+  RC(reduce<2>(ctx, 1, act));
+  return EFFPOT_OK;
+}
+
+// Hamming predictor-modifier-corrector (steppc.f90:35-159).
+template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
+  StepArgs<T> a = step_args(ctx, s, dt, act);
+  RC((launch_step<T, 0>(ctx, s, a)));
+  RC(ring_op(ctx, s, RING_AFTER_PRED, act));
+  RC(poten_t(ctx, s, act));
+  RC((launch_step<T, 1>(ctx, s, a)));
+  RC(ring_op(ctx, s, RING_AFTER_CORR, act));
+  RC(reduce<2>(ctx, 1, act));
+  return EFFPOT_OK;
+}
+
+// energy reductions -> host energies
+template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act, effpot_energies* out) {
+  EnergyArgs<T> a{};
+  const effpot_params& p = ctx->p;
+  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N;
+  a.otc = p.core == EFFPOT_CORE_OTC;
+  a.h2o2m4 = (T)p.h2o2m4; a.aux1 = (T)(0.5 * p.cp4); a.aux2 = (T)((1.0 / 3.0) * p.cpp4);
+  a.beta = (T)p.beta; a.den_m = (T)p.den_m; a.den0s = (T)p.den0s;
+  a.x0 = -p.xmax; a.y0 = -p.ymax; a.z0 = -p.zmax; a.hx = ctx->hx; a.hy = ctx->hy; a.hz = ctx->hz;
+  a.den = s.den; a.dencg = s.dencg; a.delj4 = s.delj4; a.denalf = s.denalf;
+  a.dx = s.dx; a.dy = s.dy; a.dz = s.dz; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz;
+  a.uimp = s.uimp; a.pot4 = s.pot4; a.partial = ctx->d_partial; a.nblk = nblk(ctx);
+  a.psi_all = s.psi; a.ring = s.ring;
+  dim3 g = sgrid(ctx);
+#define M_(TXX) hipLaunchKernelGGL((k_energy<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
+  TX_SWITCH(ctx->tx, M_)
+#undef M_
+  LAUNCHCHK();
+  RC(reduce<kEnergySums>(ctx, 0, act));
+  HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * ctx->B * kEnergySums, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const double dv = ctx->dxyz;
+  for (int b = 0; b < ctx->B; b++) {
+    const double* S = ctx->h_pin + b * kEnergySums;
+    effpot_energies e;
+    std::memset(&e, 0, sizeof(e));
+    e.natoms = S[6] * dv;
+    e.ekin4 = p.h2o2m4 * S[0] * dv;
+    e.elj4 = 0.5 * S[1] * dv;
+    e.ecor4 = S[2] * dv;
+    e.esolid = p.C * S[4] * dv;
+    e.etot4 = e.ekin4 + e.elj4 + e.ecor4;
+    if (p.core == EFFPOT_CORE_OTC) {
+      e.ealphas = -p.h2o2m4 * 0.5 * p.alphas * S[3] * dv;
+      e.etot4 += e.ealphas;
+    }
+    e.eHeX = S[5] * dv;
+    e.eimpu = e.ekinx + e.eHeX + e.eso;
+    e.etot = e.etot4 + e.eimpu;
+    e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
+    e.xcm = S[7] * dv / p.n4; e.ycm = S[8] * dv / p.n4; e.zcm = S[9] * dv / p.n4;
+    out[b] = e;
+  }
+  return EFFPOT_OK;
+}
+
+template <typename T> State<T>& st_of(effpot_ctx* ctx);
+template <> State<double>& st_of<double>(effpot_ctx* ctx) { return ctx->s64; }
+template <> State<float>& st_of<float>(effpot_ctx* ctx) { return ctx->s32; }
+
+int check_ctx(effpot_ctx* ctx) { return ctx ? EFFPOT_OK : EFFPOT_ERR_ARG; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI (linkage comes from the extern "C" declarations in effpot.h)
+// ===========================================================================
+
+int64_t effpot_footprint_bytes(const effpot_params* p) {
+  if (!p) return -1;
+  const int64_t N = (int64_t)p->nx * p->ny * p->nz, M = (int64_t)(p->nx / 2 + 1) * p->ny * p->nz;
+  const int64_t per64 = (2 * 9 + 20) * N * 8 + 8 * M * 16;
+  int64_t per = per64;
+  if (p->precision == EFFPOT_PREC_MIXED) per += per64 / 2;
+  return per * (int64_t)p->batch + 3 * M * 12 + (int64_t)(100 * 256 + 3) * 8 * p->batch;
+}
+
+int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
+  if (!p || !out) return EFFPOT_ERR_ARG;
+  *out = nullptr;
+  if (tstgrid(p->nx) || tstgrid(p->ny) || tstgrid(p->nz)) return EFFPOT_ERR_GRID;
+  if (!fft_size_supported(p->nx) || !fft_size_supported(p->ny) || !fft_size_supported(p->nz)) return EFFPOT_ERR_GRID;
+  if (p->core != EFFPOT_CORE_OT && p->core != EFFPOT_CORE_OTC) return EFFPOT_ERR_CORE;
+  if (p->npd != 13 || (p->icon != 13 && p->icon != 8) || p->batch < 1 || p->batch > 4096) return EFFPOT_ERR_ARG;
+  if (p->precision != EFFPOT_PREC_FP64 && p->precision != EFFPOT_PREC_MIXED) return EFFPOT_ERR_ARG;
+  effpot_ctx* ctx = new effpot_ctx();
+  ctx->p = *p;
+  ctx->dev = device;
+  if (hipSetDevice(device) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
+  ctx->nx = p->nx; ctx->ny = p->ny; ctx->nz = p->nz; ctx->nxh = p->nx / 2 + 1; ctx->B = p->batch;
+  ctx->N = (long)p->nx * p->ny * p->nz; ctx->M = (long)ctx->nxh * p->ny * p->nz;
+  ctx->hx = 2.0 * std::fabs(p->xmax) / p->nx;
+  ctx->hy = 2.0 * std::fabs(p->ymax) / p->ny;
+  ctx->hz = 2.0 * std::fabs(p->zmax) / p->nz;
+  ctx->dxyz = ctx->hx * ctx->hy * ctx->hz;
+  int nmax = std::max(p->nx, std::max(p->ny, p->nz));
+  double xm = std::max(p->xmax, std::max(p->ymax, p->zmax));
+  ctx->npot = 100 * nmax + 1;
+  ctx->delinter = 4.0 * xm / (double)(ctx->npot - 1);
+  ctx->tx = p->nx % 64 == 0 ? 64 : (p->nx % 32 == 0 ? 32 : (p->nx % 16 == 0 ? 16 : 8));
+  ctx->zc = std::min(16, p->nz);
+  ctx->mixed = p->precision == EFFPOT_PREC_MIXED;
+  ctx->h_active.assign(ctx->B, 1);
+  ctx->have_imp.assign(ctx->B, 0);
+  int rc;
+  auto bail = [&](int r) { effpot_ctx_destroy(ctx); return r; };
+  if ((rc = alloc_state(ctx, ctx->s64)) != EFFPOT_OK) return bail(rc);
+  if (ctx->mixed && (rc = alloc_state(ctx, ctx->s32)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->imptab, (size_t)ctx->B * (ctx->npot + 2))) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_active, ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_mask_tmp, ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->nblk_max = nblk(ctx);
+  if ((rc = dalloc(ctx, &ctx->d_partial, (size_t)ctx->B * ctx->nblk_max * 16)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_red, (size_t)ctx->B * 16)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_scale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_bad, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if (hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * ctx->B * 16 + 64, hipHostMallocDefault) != hipSuccess)
+    return bail(EFFPOT_ERR_NOMEM);
+  std::vector<int> ones(ctx->B, 1);
+  if (hipMemcpy(ctx->d_active, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ctx->d_ones, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(EFFPOT_ERR_DEVICE);
+  // one-time k-space kernels in FP64 (fforma / initcg / kalfs)
+  KernParams kp{p->nx, p->ny, p->nz, ctx->nxh, ctx->hx, ctx->hy, ctx->hz, p->eps4, p->sigma4, p->h_core, p->l};
+  hipLaunchKernelGGL(k_kspace, dim3(512), dim3(256), 0, ctx->st, kp, ctx->s64.kern[0], ctx->s64.kern[1],
+                     ctx->s64.kern[2]);
+  if (hipGetLastError() != hipSuccess) return bail(EFFPOT_ERR_DEVICE);
+  if (ctx->mixed) {
+    for (int j = 0; j < 3; j++)
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(512), dim3(256), 0, ctx->st, ctx->s64.kern[j],
+                         ctx->s32.kern[j], ctx->M);
+  }
+  if (hipStreamSynchronize(ctx->st) != hipSuccess) return bail(EFFPOT_ERR_DEVICE);
+  *out = ctx;
+  return EFFPOT_OK;
+}
+
+int effpot_ctx_destroy(effpot_ctx* ctx) {
+  if (!ctx) return EFFPOT_ERR_ARG;
+  hipSetDevice(ctx->dev);
+  if (ctx->st) hipStreamSynchronize(ctx->st);
+  for (void* v : ctx->allocs) hipFree(v);
+  if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  if (ctx->st) hipStreamDestroy(ctx->st);
+  delete ctx;
+  return EFFPOT_OK;
+}
+
+const char* effpot_last_error(const effpot_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int effpot_sync(effpot_ctx* ctx) {
+  RC(check_ctx(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_set_active(effpot_ctx* ctx, int slot, int active) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "slot out of range");
+  ctx->h_active[slot] = active ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(ctx->d_active, ctx->h_active.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+// Reset one slot's integrator state (ring, pc, q, history) in the given precision.
+template <typename T> static int reset_slot(effpot_ctx* ctx, State<T>& s, int slot) {
+  const long N = ctx->N;
+  HIPCHK(hipMemsetAsync(s.psi + (long)slot * 5 * N + N, 0, sizeof(cx<T>) * 4 * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.hp + (long)slot * 2 * N, 0, sizeof(cx<T>) * 2 * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.pc + (long)slot * N, 0, sizeof(cx<T>) * N, ctx->st));
+  HIPCHK(hipMemsetAsync(s.q + (long)slot * N, 0, sizeof(cx<T>) * N, ctx->st));
+  int init[8] = {0, 1, 2, 3, 4, 0, 1, 0};
+  HIPCHK(hipMemcpyAsync(s.ring + 8 * slot, init, sizeof(init), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+static int den_of_slot(effpot_ctx* ctx, int slot) {
+  std::vector<int> m(ctx->B, 0);
+  m[slot] = 1;
+  HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st,
+                     ring_of(ctx, ctx->s64), ctx->s64.den, ctx->d_mask_tmp, ctx->N);
+  LAUNCHCHK();
+  if (ctx->mixed) {
+    hipLaunchKernelGGL((k_den_from_cur<float>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st,
+                       ring_of(ctx, ctx->s32), ctx->s32.den, ctx->d_mask_tmp, ctx->N);
+    LAUNCHCHK();
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi) {
+  RC(check_ctx(ctx));
+  if (!psi || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  hipSetDevice(ctx->dev);
+  RC(reset_slot(ctx, ctx->s64, slot));
+  cx<double>* dst = ctx->s64.psi + (long)slot * 5 * ctx->N;  // buffer 0 = CUR after reset
+  HIPCHK(hipMemcpyAsync(dst, psi, sizeof(double) * 2 * ctx->N, hipMemcpyHostToDevice, ctx->st));
+  if (ctx->mixed) {
+    RC(reset_slot(ctx, ctx->s32, slot));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, ctx->st, (const double*)dst,
+                       (float*)(ctx->s32.psi + (long)slot * 5 * ctx->N), 2 * ctx->N);
+    LAUNCHCHK();
+  }
+  return den_of_slot(ctx, slot);
+}
+
+template <typename T> static int cur_index(effpot_ctx* ctx, State<T>& s, int slot, int* idx) {
+  int r[8];
+  HIPCHK(hipMemcpyAsync(r, s.ring + 8 * slot, sizeof(r), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  *idx = r[R_CUR];
+  return EFFPOT_OK;
+}
+
+// Which precision holds the live state of a slot: FP64 unless this is a mixed
+// context that has not been polished (the FP32 phase is the live one).
+static bool live32(effpot_ctx* ctx) { return ctx->mixed; }
+
+int effpot_download_psi(effpot_ctx* ctx, int slot, double* psi) {
+  RC(check_ctx(ctx));
+  if (!psi || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  hipSetDevice(ctx->dev);
+  int idx;
+  RC(cur_index(ctx, ctx->s64, slot, &idx));
+  HIPCHK(hipMemcpyAsync(psi, ctx->s64.psi + ((long)slot * 5 + idx) * ctx->N, sizeof(double) * 2 * ctx->N,
+                        hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_download_den(effpot_ctx* ctx, int slot, double* den) {
+  RC(check_ctx(ctx));
+  if (!den || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  HIPCHK(hipMemcpyAsync(den, ctx->s64.den + (long)slot * ctx->N, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out) {
+  RC(check_ctx(ctx));
+  if (!out || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  State<double>& s = ctx->s64;
+  double* src[13] = {s.pot4, s.dencg, s.delj4, s.denalf, s.ualphas, s.uimp, s.dx, s.dy, s.dz, s.sto1, s.intx, s.inty, s.intz};
+  if (which < 0 || which > 12) return ctx->fail(EFFPOT_ERR_ARG, "unknown field");
+  HIPCHK(hipMemcpyAsync(out, src[which] + (long)slot * ctx->N, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+// Counter-based uniform in [0,1): splitmix64 finaliser of the seed and index.
+static uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R, double afermi, double noise, uint64_t seed,
+                      const double centre[3]) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B || !centre || !(afermi > 0)) return ctx->fail(EFFPOT_ERR_ARG, "bad Fermi arguments");
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  std::vector<double> rho(ctx->N), psi(2 * ctx->N);
+  double sum = 0.0;
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int ix = 0; ix < nx; ix++) {
+        long i = (long)ix + (long)nx * ((long)iy + (long)ny * iz);
+        double dx = (-ctx->p.xmax + ctx->hx * ix) - centre[0];
+        double dy = (-ctx->p.ymax + ctx->hy * iy) - centre[1];
+        double dz = (-ctx->p.zmax + ctx->hz * iz) - centre[2];
+        double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        double u = (double)(mix64(seed ^ mix64((uint64_t)i)) >> 11) * 0x1.0p-53;
+        double v = rho0 / (1.0 + std::exp((r - R) / afermi)) + noise * u;
+        rho[i] = v;
+        sum += v;
+      }
+  double f = std::sqrt(ctx->p.n4 / (sum * ctx->dxyz));
+  for (long i = 0; i < ctx->N; i++) { psi[2 * i] = std::sqrt(rho[i]) * f; psi[2 * i + 1] = 0.0; }
+  return effpot_upload_psi(ctx, slot, psi.data());
+}
+
+int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
+  RC(check_ctx(ctx));
+  if (!imp || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  if (imp->kind != EFFPOT_POT_MORSE && imp->kind != EFFPOT_POT_NULL)
+    return ctx->fail(EFFPOT_ERR_POT, "unknown impurity potential");
+  hipSetDevice(ctx->dev);
+  ImpParams ip{imp->kind, imp->D, imp->re, imp->a, imp->r_cutoff, imp->umax, imp->cap, imp->rimp[0], imp->rimp[1], imp->rimp[2]};
+  double* tab = ctx->imptab + (long)slot * (ctx->npot + 2);
+  hipLaunchKernelGGL(k_imp_table, dim3((ctx->npot + 257) / 256), dim3(256), 0, ctx->st, ip, ctx->npot, ctx->delinter, tab);
+  LAUNCHCHK();
+  const double rmax = (double)(ctx->npot - 1) * ctx->delinter;
+  hipLaunchKernelGGL((k_imp_field<double>), dim3(1024), dim3(256), 0, ctx->st, ip, ctx->nx, ctx->ny, ctx->nz,
+                     -ctx->p.xmax, -ctx->p.ymax, -ctx->p.zmax, ctx->hx, ctx->hy, ctx->hz, ctx->npot, rmax, (const double*)tab,
+                     ctx->s64.uimp + (long)slot * ctx->N);
+  LAUNCHCHK();
+  if (ctx->mixed) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, ctx->st,
+                       (const double*)(ctx->s64.uimp + (long)slot * ctx->N), ctx->s32.uimp + (long)slot * ctx->N, ctx->N);
+    LAUNCHCHK();
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->have_imp[slot] = 1;
+  return EFFPOT_OK;
+}
+
+int effpot_poten(effpot_ctx* ctx) {
+  RC(check_ctx(ctx));
+  hipSetDevice(ctx->dev);
+  RC(poten_t(ctx, ctx->s64, ctx->d_active));
+  if (ctx->mixed) RC(poten_t(ctx, ctx->s32, ctx->d_active));
+  return EFFPOT_OK;
+}
+
+int effpot_energy(effpot_ctx* ctx, effpot_energies* out) {
+  RC(check_ctx(ctx));
+  if (!out) return ctx->fail(EFFPOT_ERR_ARG, "null output");
+  hipSetDevice(ctx->dev);
+  return energy_t(ctx, ctx->s64, ctx->d_active, out);
+}
+
+int effpot_step_rk(effpot_ctx* ctx, double dt) {
+  RC(check_ctx(ctx));
+  if (!(dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
+  hipSetDevice(ctx->dev);
+  return step_rk_t(ctx, ctx->s64, dt, ctx->d_active);
+}
+
+int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he) {
+  RC(check_ctx(ctx));
+  if (!(dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
+  hipSetDevice(ctx->dev);
+  RC(step_pc_t(ctx, ctx->s64, dt, ctx->d_active));
+  if (err_he) {
+    HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * ctx->B * 2, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int b = 0; b < ctx->B; b++) err_he[b] = ctx->h_pin[2 * b] / (double)ctx->N;
+  }
+  return EFFPOT_OK;
+}
+
+int effpot_renormalize(effpot_ctx* ctx) {
+  RC(check_ctx(ctx));
+  hipSetDevice(ctx->dev);
+  RC(norm_t(ctx, ctx->s64, ctx->d_active));
+  RC(renorm_t(ctx, ctx->s64, ctx->d_active));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df) {
+  RC(check_ctx(ctx));
+  if (!f || !df || id < 1 || id > 2 || iv < 1 || iv > 3) return ctx->fail(EFFPOT_ERR_ARG, "bad derivnd arguments");
+  hipSetDevice(ctx->dev);
+  State<double>& s = ctx->s64;
+  HIPCHK(hipMemcpyAsync(s.src[0], f, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->st));
+  Grid g = grid_of(ctx);
+  dim3 gr = sgrid(ctx);
+  gr.z = 1;
+#define M_(TXX)                                                                                                   \
+  if (id == 1) hipLaunchKernelGGL((k_deriv1<double, TXX, 1>), gr, dim3(kThreads), 0, ctx->st, g, s.sc, ctx->zc, \
+                                  (const double*)s.src[0], s.src[1], iv - 1);                                     \
+  else hipLaunchKernelGGL((k_deriv1<double, TXX, 2>), gr, dim3(kThreads), 0, ctx->st, g, s.sc, ctx->zc,         \
+                          (const double*)s.src[0], s.src[1], iv - 1)
+  TX_SWITCH(ctx->tx, M_)
+#undef M_
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(df, s.src[1], sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out) {
+  RC(check_ctx(ctx));
+  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  hipSetDevice(ctx->dev);
+  State<double>& s = ctx->s64;
+  std::vector<int> m(ctx->B, 0);
+  m[0] = 1;
+  HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(s.src[0], in, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->st));
+  double* r[1] = {s.src[0]};
+  cx<double>* c[1] = {s.spec[0]};
+  RC(fft_x_r2c(ctx, s, r, c, 1, ctx->d_mask_tmp));
+  RC(fft_strided(ctx, s, 1, c, 1, -1, ctx->d_mask_tmp));
+  RC(fft_strided(ctx, s, 2, c, 1, -1, ctx->d_mask_tmp));
+  HIPCHK(hipMemcpyAsync(out, s.spec[0], sizeof(double) * 2 * ctx->M, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out) {
+  RC(check_ctx(ctx));
+  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  hipSetDevice(ctx->dev);
+  State<double>& s = ctx->s64;
+  std::vector<int> m(ctx->B, 0);
+  m[0] = 1;
+  HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(s.spec[0], in, sizeof(double) * 2 * ctx->M, hipMemcpyHostToDevice, ctx->st));
+  cx<double>* c[1] = {s.spec[0]};
+  double* r[1] = {s.src[0]};
+  RC(fft_strided(ctx, s, 2, c, 1, +1, ctx->d_mask_tmp));
+  RC(fft_strided(ctx, s, 1, c, 1, +1, ctx->d_mask_tmp));
+  RC(fft_x_c2r(ctx, s, c, r, 1, 1.0 / ((double)ctx->nx * ctx->ny * ctx->nz), ctx->d_mask_tmp));
+  HIPCHK(hipMemcpyAsync(out, s.src[0], sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+// One precision phase of the relaxation on the slots in `mask` (host copy):
+// [RK starter] + predictor-corrector with renormalisation after every step,
+// energy every pener steps; a slot leaves the mask when its |dE/E| per step
+// drops below tol.  Returns per-slot results in res (steps accumulate).
+template <typename T>
+static int relax_phase(effpot_ctx* ctx, State<T>& s, double dt, double tol, int n_rk, int pener, int max_steps,
+                       int check_finite, std::vector<int>& mask, std::vector<effpot_result>& res) {
+  const int B = ctx->B;
+  auto push_mask = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(ctx->d_active, mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
+    return EFFPOT_OK;
+  };
+  RC(push_mask());
+  std::vector<effpot_energies> e(B);
+  std::vector<double> eprev(B, 0.0);
+  RC(poten_t(ctx, s, ctx->d_active));
+  int steps = 0;
+  for (int k = 0; k < n_rk; k++) {
+    RC(step_rk_t(ctx, s, dt, ctx->d_active));
+    RC(renorm_t(ctx, s, ctx->d_active));
+    steps++;
+  }
+  RC(poten_t(ctx, s, ctx->d_active));
+  RC(energy_t(ctx, s, ctx->d_active, e.data()));
+  for (int b = 0; b < B; b++) if (mask[b]) eprev[b] = e[b].etot;
+  int since = 0;
+  auto any = [&]() { for (int b = 0; b < B; b++) if (mask[b]) return true; return false; };
+  while (any() && steps < max_steps) {
+    RC(step_pc_t(ctx, s, dt, ctx->d_active));
+    RC(renorm_t(ctx, s, ctx->d_active));
+    steps++; since++;
+    if (since == pener || steps == max_steps) {
+      if (check_finite) {
+        HIPCHK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int) * B, ctx->st));
+        hipLaunchKernelGGL((k_check_den<T>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const T*)s.den, ctx->N,
+                           ctx->N, ctx->d_active, ctx->d_bad);
+        LAUNCHCHK();
+        std::vector<int> bad(B);
+        HIPCHK(hipMemcpyAsync(bad.data(), ctx->d_bad, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        for (int b = 0; b < B; b++)
+          if (mask[b] && bad[b]) return ctx->fail(EFFPOT_ERR_NONFINITE, "non-positive or non-finite density (slot " + std::to_string(b) + ")");
+      }
+      RC(poten_t(ctx, s, ctx->d_active));
+      RC(energy_t(ctx, s, ctx->d_active, e.data()));
+      HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * B * 2, hipMemcpyDeviceToHost, ctx->st));
+      bool changed = false;
+      for (int b = 0; b < B; b++) {
+        if (!mask[b]) continue;
+        double rel = std::fabs(e[b].etot - eprev[b]) / std::fabs(e[b].etot) / (double)since;
+        if (!std::isfinite(e[b].etot)) return ctx->fail(EFFPOT_ERR_NONFINITE, "non-finite energy (slot " + std::to_string(b) + ")");
+        eprev[b] = e[b].etot;
+        res[b].e = e[b];
+        res[b].rel_de = rel;
+        res[b].steps += since + (res[b].steps == 0 && steps == since + n_rk ? n_rk : 0);
+        if (rel < tol) { res[b].converged = 1; mask[b] = 0; changed = true; }
+      }
+      since = 0;
+      if (changed) RC(push_mask());
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {
+  RC(check_ctx(ctx));
+  if (!opt || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  if (!(opt->dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
+  if (opt->pener < 1 || opt->max_steps < 1 || opt->n_rk < 0) return ctx->fail(EFFPOT_ERR_ARG, "bad solve options");
+  hipSetDevice(ctx->dev);
+  const int B = ctx->B;
+  for (int b = 0; b < B; b++)
+    if (ctx->h_active[b] && !ctx->have_imp[b]) return ctx->fail(EFFPOT_ERR_STATE, "impurity not set");
+  std::vector<effpot_result> res(B);
+  std::memset(res.data(), 0, sizeof(effpot_result) * B);
+  std::vector<int> mask = ctx->h_active;
+  const std::vector<int> user_mask = ctx->h_active;
+  int rc;
+  if (!ctx->mixed) {
+    rc = relax_phase(ctx, ctx->s64, opt->dt, opt->tol, opt->n_rk, opt->pener, opt->max_steps, opt->check_finite, mask, res);
+  } else {
+    rc = relax_phase(ctx, ctx->s32, opt->dt, opt->tol_fp32, opt->n_rk, opt->pener, opt->max_steps, opt->check_finite, mask,
+                     res);
+    if (rc == EFFPOT_OK) {
+      // FP64 polish: psi (FP32) -> FP64 live buffer, fresh integrator history
+      for (int b = 0; b < B; b++) {
+        if (!user_mask[b]) continue;
+        int idx;
+        if ((rc = cur_index(ctx, ctx->s32, b, &idx)) != EFFPOT_OK) break;
+        if ((rc = reset_slot(ctx, ctx->s64, b)) != EFFPOT_OK) break;
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, ctx->st,
+                           (const float*)(ctx->s32.psi + ((long)b * 5 + idx) * ctx->N),
+                           (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), 2 * ctx->N);
+        LAUNCHCHK();
+      }
+      if (rc == EFFPOT_OK) {
+        HIPCHK(hipMemcpyAsync(ctx->d_active, user_mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
+        hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, ring_of(ctx, ctx->s64),
+                           ctx->s64.den, ctx->d_active, ctx->N);
+        LAUNCHCHK();
+        std::vector<int> m2 = user_mask;
+        std::vector<effpot_result> r2(B);
+        std::memset(r2.data(), 0, sizeof(effpot_result) * B);
+        rc = relax_phase(ctx, ctx->s64, opt->dt, opt->tol, opt->n_rk, opt->pener,
+                         opt->polish_steps > 0 ? opt->polish_steps : opt->max_steps, opt->check_finite, m2, r2);
+        for (int b = 0; b < B; b++) {
+          if (!user_mask[b]) continue;
+          int st32 = res[b].steps;
+          res[b] = r2[b];
+          res[b].steps += st32;
+        }
+      }
+    }
+  }
+  // restore the caller's activity mask
+  HIPCHK(hipMemcpyAsync(ctx->d_active, user_mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (int b = 0; b < B; b++) out[b] = res[b];
+  return rc;
+}
+

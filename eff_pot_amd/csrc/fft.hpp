@@ -1,0 +1,466 @@
+// Batched 3-D real<->complex FFTs for gfx950, built from one device routine:
+// a length-N complex transform done as N = N1*N2 with a register FFT of
+// length N1 per thread, a twiddle, one LDS transpose and a register FFT of
+// length N2.  Three kernel families use it:
+//   x_r2c / x_c2r : rows along x (contiguous), two real rows packed into one
+//                   complex row; output is the half spectrum (nx/2+1,ny,nz);
+//   strided       : complex lines along y or z of the half spectrum, C lines
+//                   (consecutive kx) per work-group so every global access is
+//                   a contiguous run of C complex values;
+//   z_conv        : forward z transform, multiply by up to 3 real k-space
+//                   kernels (times 1/(nx ny nz)), inverse z transform, all
+//                   without leaving the work-group (the spectrum stays in
+//                   registers between the products).
+// Replaces fft.f90:6-144 (FFTW r2c/c2r plans) and the k-space products of
+// poten.f90:34-71,95-97 and term_alfa.f90:172-205.
+#pragma once
+#include "common.hpp"
+
+namespace ep {
+
+// ---- 48th roots of unity (all register-FFT twiddles are powers of them) ----
+struct Tw48 { double c[48], s[48]; };
+constexpr double ce_sin_(double x) {
+  double x2 = x * x, t = x, s = x;
+  for (int k = 1; k < 14; k++) { t *= -x2 / ((2.0 * k) * (2.0 * k + 1.0)); s += t; }
+  return s;
+}
+constexpr double ce_cos_(double x) {
+  double x2 = x * x, t = 1.0, s = 1.0;
+  for (int k = 1; k < 14; k++) { t *= -x2 / ((2.0 * k - 1.0) * (2.0 * k)); s += t; }
+  return s;
+}
+constexpr Tw48 make_tw48() {
+  Tw48 w{};
+  const double pi24 = 3.14159265358979323846 / 24.0;
+  for (int j = 0; j < 48; j++) {
+    int q = j / 6, r = j % 6;
+    double Cr = ce_cos_(r * pi24), Sr = ce_sin_(r * pi24);
+    double Cm = ce_cos_((6 - r) * pi24), Sm = ce_sin_((6 - r) * pi24);
+    double c = 0, s = 0;  // cos, sin of 2 pi j / 48
+    switch (q) {
+      case 0: c = Cr; s = Sr; break;
+      case 1: c = Sm; s = Cm; break;
+      case 2: c = -Sr; s = Cr; break;
+      case 3: c = -Cm; s = Sm; break;
+      case 4: c = -Cr; s = -Sr; break;
+      case 5: c = -Sm; s = -Cm; break;
+      case 6: c = Sr; s = -Cr; break;
+      default: c = Cm; s = -Sm; break;
+    }
+    w.c[j] = c;
+    w.s[j] = -s;  // exp(-i theta)
+  }
+  return w;
+}
+struct Tw48f { float c[48], s[48]; };
+constexpr Tw48f make_tw48f() {
+  Tw48 d = make_tw48();
+  Tw48f f{};
+  for (int j = 0; j < 48; j++) { f.c[j] = (float)d.c[j]; f.s[j] = (float)d.s[j]; }
+  return f;
+}
+__constant__ Tw48 c_tw48 = make_tw48();
+__constant__ Tw48f c_tw48f = make_tw48f();
+
+template <typename T> EP_DEV cx<T> tw48(int j);
+template <> EP_DEV cx<double> tw48<double>(int j) { return {c_tw48.c[j], c_tw48.s[j]}; }
+template <> EP_DEV cx<float> tw48<float>(int j) { return {c_tw48f.c[j], c_tw48f.s[j]}; }
+
+// ---- small DFT butterflies (natural order in and out) -----------------------
+template <typename T> EP_DEV void dft2(cx<T>* v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = a + b; v[1] = a - b;
+}
+template <typename T> EP_DEV void dft3(cx<T>* v, int sign) {
+  const T h = T(0.86602540378443864676);
+  cx<T> t = v[1] + v[2], u = v[1] - v[2];
+  cx<T> m = v[0] - T(0.5) * t;
+  cx<T> r = mul_si(h * u, sign);
+  v[0] = v[0] + t; v[1] = m + r; v[2] = m - r;
+}
+template <typename T> EP_DEV void dft4(cx<T>* v, int sign) {
+  cx<T> s02 = v[0] + v[2], d02 = v[0] - v[2], s13 = v[1] + v[3], d13 = mul_si(v[1] - v[3], sign);
+  v[0] = s02 + s13; v[2] = s02 - s13; v[1] = d02 + d13; v[3] = d02 - d13;
+}
+template <int R, typename T> EP_DEV void dft_r(cx<T>* v, int sign) {
+  if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 3) dft3(v, sign);
+  else dft4(v, sign);
+}
+
+// One Stockham pass on a register array (all indices compile-time after unrolling).
+template <typename T, int L, int P, int R>
+EP_DEV void reg_pass(const cx<T>* src, cx<T>* dst, int sign) {
+  constexpr int M = L / R;
+  static_assert(48 % (P * R) == 0, "twiddle must be a 48th root");
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    const int k = i % P;
+    cx<T> v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      v[j] = src[i + j * M];
+      if (j > 0 && k > 0) {
+        cx<T> w = tw48<T>((j * k * (48 / (P * R))) % 48);
+        if (sign > 0) w.y = -w.y;
+        v[j] = cmul(v[j], w);
+      }
+    }
+    dft_r<R>(v, sign);
+    const int base = (i - k) * R + k;
+#pragma unroll
+    for (int j = 0; j < R; j++) dst[base + j * P] = v[j];
+  }
+}
+
+template <typename T, int L> struct RegFFT;
+template <typename T> struct RegFFT<T, 3> { static EP_DEV void run(cx<T>* a, int s) { dft3(a, s); } };
+template <typename T> struct RegFFT<T, 4> { static EP_DEV void run(cx<T>* a, int s) { dft4(a, s); } };
+template <typename T> struct RegFFT<T, 6> {
+  static EP_DEV void run(cx<T>* a, int s) { cx<T> b[6]; reg_pass<T, 6, 1, 2>(a, b, s); reg_pass<T, 6, 2, 3>(b, a, s); }
+};
+template <typename T> struct RegFFT<T, 8> {
+  static EP_DEV void run(cx<T>* a, int s) { cx<T> b[8]; reg_pass<T, 8, 1, 4>(a, b, s); reg_pass<T, 8, 4, 2>(b, a, s); }
+};
+template <typename T> struct RegFFT<T, 12> {
+  static EP_DEV void run(cx<T>* a, int s) { cx<T> b[12]; reg_pass<T, 12, 1, 4>(a, b, s); reg_pass<T, 12, 4, 3>(b, a, s); }
+};
+template <typename T> struct RegFFT<T, 16> {
+  static EP_DEV void run(cx<T>* a, int s) { cx<T> b[16]; reg_pass<T, 16, 1, 4>(a, b, s); reg_pass<T, 16, 4, 4>(b, a, s); }
+};
+
+// N = N1*N2 (N1 >= N2, N1 a power of two dividing 256): C = 256/N1 lines per group.
+template <int N> struct Shape;
+template <> struct Shape<16> { static constexpr int N1 = 4, N2 = 4; };
+template <> struct Shape<24> { static constexpr int N1 = 8, N2 = 3; };
+template <> struct Shape<32> { static constexpr int N1 = 8, N2 = 4; };
+template <> struct Shape<48> { static constexpr int N1 = 8, N2 = 6; };
+template <> struct Shape<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct Shape<96> { static constexpr int N1 = 16, N2 = 6; };
+template <> struct Shape<128> { static constexpr int N1 = 16, N2 = 8; };
+template <> struct Shape<192> { static constexpr int N1 = 16, N2 = 12; };
+template <> struct Shape<256> { static constexpr int N1 = 16, N2 = 16; };
+
+inline bool fft_size_supported(int n) {
+  switch (n) { case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: return true; }
+  return false;
+}
+
+// Accessor describing where line c, element i lives in global memory.
+// Lines c = cx + ncx*co; address = base + cx*1 + co*s_outer + i*s_elem.
+struct LineMap {
+  int ncx;          // lines per contiguous run (nxh)
+  long s_outer;     // stride between runs
+  long s_elem;      // stride between elements of a line
+  int nlines;       // lines per field and slot
+  EP_DEV long addr(int c, int i) const {
+    int co = c / ncx, ci = c - co * ncx;
+    return (long)ci + (long)co * s_outer + (long)i * s_elem;
+  }
+};
+
+// ---- the two-level transform, step by step -------------------------------------
+// LDS layout between the steps: index (k1*N2 + n2)*C + c (c fastest: conflict free).
+
+// Step 1+2 for the lines of this group: a[n1] = x[N2 n1 + n2]; FFT_N1; twiddle; to LDS.
+template <typename T, int N, class Load>
+EP_DEV void fft_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  const int t = threadIdx.x;
+  if (t < C * N2) {
+    const int c = t % C, n2 = t / C;
+    cx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = load(c, N2 * n1 + n2);
+    RegFFT<T, N1>::run(a, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> v = a[k1];
+      if (k1 > 0 && n2 > 0) {
+        cx<T> w = twN[n2 * k1];
+        if (sign > 0) w.y = -w.y;
+        v = cmul(v, w);
+      }
+      lds[(k1 * N2 + n2) * C + c] = v;
+    }
+  }
+}
+// Step 3: thread (c,k1) gathers b[n2] from LDS, FFT_N2; X[k1 + N1 k2] = b[k2].
+template <typename T, int N>
+EP_DEV void fft_step3(const cx<T>* lds, cx<T> (&b)[Shape<N>::N2], int sign) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  const int t = threadIdx.x;
+  const int c = t % C, k1 = t / C;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; n2++) b[n2] = lds[(k1 * N2 + n2) * C + c];
+  RegFFT<T, N2>::run(b, sign);
+}
+// Inverse of the layout above starting from registers: thread (c,k1) holds
+// X[k1 + N1 j] (j < N2).  Decomposition N = N2*N1: FFT_N2 over j, twiddle
+// w^{k1 k1'}, LDS, FFT_N1 -> x[k1' + N2 k2'] delivered to store(c, i, v).
+template <typename T, int N, class Store>
+EP_DEV void fft_from_regs(cx<T> (&a)[Shape<N>::N2], cx<T>* lds, const cx<T>* twN, int sign, Store store) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  const int t = threadIdx.x;
+  {
+    const int c = t % C, k1 = t / C;
+    RegFFT<T, N2>::run(a, sign);
+#pragma unroll
+    for (int kk = 0; kk < N2; kk++) {
+      cx<T> v = a[kk];
+      if (kk > 0 && k1 > 0) {
+        cx<T> w = twN[k1 * kk];
+        if (sign > 0) w.y = -w.y;
+        v = cmul(v, w);
+      }
+      lds[(kk * N1 + k1) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  if (t < C * N2) {
+    const int c = t % C, kk = t / C;
+    cx<T> b[N1];
+#pragma unroll
+    for (int m = 0; m < N1; m++) b[m] = lds[(kk * N1 + m) * C + c];
+    RegFFT<T, N1>::run(b, sign);
+#pragma unroll
+    for (int m = 0; m < N1; m++) store(c, kk + N2 * m, b[m]);
+  }
+}
+
+// ---- strided complex lines (y or z axis of the half spectrum) ---------------
+// grid: (ceil(nlines/C), nfields, nslots); in place on f[blockIdx.y] + slot*sstride.
+template <typename T> struct StridedArgs {
+  cx<T>* f[4];
+  long sstride;
+  LineMap lm;
+  const cx<T>* twN;
+  int sign;
+  const int* active;
+};
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs<T> a) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  if (!a.active[blockIdx.z]) return;
+  __shared__ cx<T> lds[N * C];
+  const LineMap lm = a.lm;
+  cx<T>* d = a.f[blockIdx.y] + (long)blockIdx.z * a.sstride;
+  const int c0 = blockIdx.x * C;
+  auto load = [&](int c, int i) -> cx<T> {
+    int cc = c0 + c;
+    return cc < lm.nlines ? d[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
+  };
+  fft_step1<T, N>(lds, a.twN, a.sign, load);
+  __syncthreads();
+  cx<T> b[N2];
+  fft_step3<T, N>(lds, b, a.sign);
+  const int c = threadIdx.x % C, k1 = threadIdx.x / C;
+  const int cc = c0 + c;
+  if (cc < lm.nlines) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) d[lm.addr(cc, k1 + N1 * k2)] = b[k2];
+  }
+}
+
+// ---- z transform with the k-space products in between ----------------------
+// Group g (blockIdx.y): input in[g]; nout[g] outputs out[g][j] = IFFT_z(K * FFT_z(in))
+// with K = kern[g][j] (M reals shared by all slots) times `scale`.
+template <typename T> struct ConvArgs {
+  const cx<T>* in[4];
+  cx<T>* out[4][3];
+  const T* kern[4][3];
+  int nout[4];
+  long sstride;
+  LineMap lm;
+  T scale;
+  const cx<T>* twN;
+  const int* active;
+};
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  if (!a.active[blockIdx.z]) return;
+  __shared__ cx<T> lds[N * C];
+  const int g = blockIdx.y;
+  const LineMap lm = a.lm;
+  const long so = (long)blockIdx.z * a.sstride;
+  const cx<T>* in = a.in[g] + so;
+  const int c0 = blockIdx.x * C;
+  auto load = [&](int c, int i) -> cx<T> {
+    int cc = c0 + c;
+    return cc < lm.nlines ? in[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
+  };
+  fft_step1<T, N>(lds, a.twN, -1, load);
+  __syncthreads();
+  cx<T> spec[N2];
+  fft_step3<T, N>(lds, spec, -1);
+  const int c = threadIdx.x % C, kk1 = threadIdx.x / C;
+  const int cc = c0 + c;
+  const bool valid = cc < lm.nlines;
+  const int nout = a.nout[g];
+  for (int j = 0; j < nout; j++) {
+    const T* K = a.kern[g][j];
+    cx<T> v[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      T kv = valid ? K[lm.addr(cc, kk1 + N1 * k2)] * a.scale : T(0);
+      v[k2] = kv * spec[k2];
+    }
+    __syncthreads();  // LDS reuse
+    cx<T>* out = a.out[g][j] + so;
+    auto store = [&](int cl, int i, cx<T> val) {
+      int c2 = c0 + cl;
+      if (c2 < lm.nlines) out[lm.addr(c2, i)] = val;
+    };
+    fft_from_regs<T, N>(v, lds, a.twN, +1, store);
+  }
+}
+
+// ---- x axis: two real rows <-> one complex row ------------------------------
+// Row r (0 <= r < ny*nz) of field f, slot s: real at r[f] + s*rslot + r*nx,
+// half spectrum at c[f] + s*cslot + r*nxh.
+template <typename T> struct XArgs {
+  T* r[4];
+  cx<T>* c[4];
+  long rslot, cslot;
+  int nrows;  // ny*nz (even)
+  T scale;    // c2r only
+  const cx<T>* twN;
+  const int* active;
+};
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
+  const cx<T>* twN = rm.twN;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1, NH = N / 2 + 1;
+  constexpr int P = C + 1;  // padded line stride in the staging layout [i][c]
+  if (!rm.active[blockIdx.z]) return;
+  __shared__ cx<T> lds[N * P];
+  const int f = blockIdx.y;
+  const T* src = rm.r[f] + (long)blockIdx.z * rm.rslot;
+  cx<T>* dst = rm.c[f] + (long)blockIdx.z * rm.cslot;
+  const int pair0 = blockIdx.x * C;  // pairs of rows
+  const int npairs = rm.nrows / 2;
+  // stage: z_c[i] = a_c[i] + i b_c[i]
+  for (int e = threadIdx.x; e < C * N; e += kThreads) {
+    const int c = e / N, i = e - c * N;
+    const int pr = pair0 + c;
+    cx<T> v{T(0), T(0)};
+    if (pr < npairs) {
+      const T* ra = src + (long)(2 * pr) * N;
+      v = cx<T>{ra[i], ra[N + i]};
+    }
+    lds[i * P + c] = v;
+  }
+  __syncthreads();
+  cx<T> a[N1];
+  const int t = threadIdx.x;
+  const bool s1 = t < C * N2;
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[(N2 * n1 + n2) * P + c];
+  }
+  __syncthreads();
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+    RegFFT<T, N1>::run(a, -1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> v = a[k1];
+      if (k1 > 0 && n2 > 0) v = cmul(v, twN[n2 * k1]);
+      lds[(k1 * N2 + n2) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  cx<T> b[N2];
+  fft_step3<T, N>(lds, b, -1);
+  __syncthreads();
+  {
+    const int c = t % C, k1 = t / C;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) lds[(k1 + N1 * k2) * P + c] = b[k2];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < C * NH; e += kThreads) {
+    const int c = e / NH, k = e - c * NH;
+    const int pr = pair0 + c;
+    if (pr >= npairs) continue;
+    cx<T> z = lds[k * P + c], zm = lds[((N - k) % N) * P + c];
+    cx<T> s = cx<T>{z.x + zm.x, z.y - zm.y};  // z + conj(zm)
+    cx<T> d = cx<T>{z.x - zm.x, z.y + zm.y};  // z - conj(zm)
+    cx<T>* oa = dst + (long)(2 * pr) * NH;
+    oa[k] = cx<T>{T(0.5) * s.x, T(0.5) * s.y};
+    oa[NH + k] = cx<T>{T(0.5) * d.y, T(-0.5) * d.x};
+  }
+}
+
+// Half-spectrum rows -> real rows (scaled by `scale`); the imaginary parts of
+// the self-conjugate bins 0 and N/2 are ignored, as a c2r transform does.
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
+  const cx<T>* twN = rm.twN;
+  const T scale = rm.scale;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1, NH = N / 2 + 1;
+  constexpr int P = C + 1;
+  if (!rm.active[blockIdx.z]) return;
+  __shared__ cx<T> lds[N * P];
+  const int f = blockIdx.y;
+  const cx<T>* src = rm.c[f] + (long)blockIdx.z * rm.cslot;
+  T* dst = rm.r[f] + (long)blockIdx.z * rm.rslot;
+  const int pair0 = blockIdx.x * C;
+  const int npairs = rm.nrows / 2;
+  for (int e = threadIdx.x; e < C * NH; e += kThreads) {
+    const int c = e / NH, k = e - c * NH;
+    const int pr = pair0 + c;
+    cx<T> A{T(0), T(0)}, B{T(0), T(0)};
+    if (pr < npairs) {
+      const cx<T>* ra = src + (long)(2 * pr) * NH;
+      A = ra[k]; B = ra[NH + k];
+    }
+    if (k == 0 || k == N / 2) { A.y = T(0); B.y = T(0); }
+    lds[k * P + c] = cx<T>{A.x - B.y, A.y + B.x};
+    if (k > 0 && k < N / 2) lds[(N - k) * P + c] = cx<T>{A.x + B.y, B.x - A.y};
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  cx<T> a[N1];
+  const bool s1 = t < C * N2;
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[(N2 * n1 + n2) * P + c];
+  }
+  __syncthreads();
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+    RegFFT<T, N1>::run(a, +1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> v = a[k1];
+      if (k1 > 0 && n2 > 0) v = cmul(v, conjg(twN[n2 * k1]));
+      lds[(k1 * N2 + n2) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  cx<T> b[N2];
+  fft_step3<T, N>(lds, b, +1);
+  __syncthreads();
+  {
+    const int c = t % C, k1 = t / C;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) lds[(k1 + N1 * k2) * P + c] = b[k2];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < C * N; e += kThreads) {
+    const int c = e / N, i = e - c * N;
+    const int pr = pair0 + c;
+    if (pr >= npairs) continue;
+    cx<T> z = lds[i * P + c];
+    T* ra = dst + (long)(2 * pr) * N;
+    ra[i] = scale * z.x;
+    ra[N + i] = scale * z.y;
+  }
+}
+
+}  // namespace ep

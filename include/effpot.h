@@ -1,0 +1,189 @@
+/*
+ * effpot.h -- C ABI of the MI355X (gfx950) Orsay-Trento imaginary-time solver.
+ *
+ * The reference (bcntls2016/eff_pot, Fortran) keeps all state in module
+ * globals and exposes zero-argument subroutines.  This ABI replaces those
+ * subroutines one-for-one with calls on an opaque context that owns the
+ * device memory of `batch` independent solves (slots) sharing one grid and
+ * one functional.  Every entry point returns EFFPOT_OK (0) or a non-zero
+ * status; effpot_last_error() returns the message.  Where the reference
+ * would STOP, this ABI returns a status and the Fortran shim turns it into
+ * `error stop` with the same message.
+ *
+ * Arrays crossing the boundary are column-major with ix fastest, exactly as
+ * the Fortran module arrays (dimen.f90:29-145): real fields are
+ * double[nz][ny][nx]; the wavefunction is interleaved complex double
+ * (re, im) per point, i.e. Fortran complex(8) psi(nx,ny,nz).
+ * The library never keeps host pointers past the call that received them.
+ */
+#ifndef EFFPOT_H
+#define EFFPOT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define EFFPOT_OK 0
+#define EFFPOT_ERR_ARG 1       /* bad argument / NULL pointer                     */
+#define EFFPOT_ERR_GRID 2      /* n not 2^a 3^b 5^c <= 256 (tstgrid.f90:8-12)      */
+#define EFFPOT_ERR_CORE 3      /* unknown core4 (DFT4He3d.f90:355-363)            */
+#define EFFPOT_ERR_POT 4       /* unknown impurity potential (V_impur.f90:928-931)*/
+#define EFFPOT_ERR_DEVICE 5    /* HIP runtime failure                             */
+#define EFFPOT_ERR_NONFINITE 6 /* non-positive / NaN density (steprk.f90:94-110)  */
+#define EFFPOT_ERR_STATE 7     /* call out of order (e.g. step before impurity)   */
+#define EFFPOT_ERR_NOMEM 8     /* device allocation failed                        */
+#define EFFPOT_ERR_DT 9        /* inconsistent / non-positive time step           */
+
+/* ---- enumerations ------------------------------------------------------ */
+#define EFFPOT_CORE_OT 0  /* core4='OT '  (poten.f90:27-92, no alpha_s)      */
+#define EFFPOT_CORE_OTC 1 /* core4='OTC'  (poten.f90:94-105, term_alfa.f90) */
+
+#define EFFPOT_PREC_FP64 0  /* every array and operation in FP64                 */
+#define EFFPOT_PREC_MIXED 1 /* FP32 bulk fields/FFTs/stencils, FP64 reductions   */
+
+#define EFFPOT_POT_NULL 0  /* Select_Pot 'null' (V_impur.f90): U = 0            */
+#define EFFPOT_POT_MORSE 1 /* isotropic Morse D[e^{-2a(r-re)} - 2e^{-a(r-re)}] */
+
+#define EFFPOT_ICON_PERIODIC 8 /* derivnd icon=8  (init_deriv_parallel.f90:529-634) */
+#define EFFPOT_ICON_ZERO 13    /* derivnd icon=13 (redef.f:133-170): f=0 outside  */
+
+/* ---- parameters -------------------------------------------------------- */
+/* Grid, functional and run-shape parameters: the namelist subset of
+ * DFT4He3d.f90:104-134 that the hot path reads, plus h_core (the LJ core
+ * radius h4, chosen from core4 in DFT4He3d.f90:346-363, here explicit). */
+typedef struct effpot_params {
+  int32_t nx, ny, nz; /* grid points (tstgrid: 2^a 3^b 5^c, 16..256)       */
+  int32_t core;       /* EFFPOT_CORE_OT | EFFPOT_CORE_OTC                   */
+  int32_t npd;        /* stencil points; only 13 is supported (modules.f90:15) */
+  int32_t icon;       /* EFFPOT_ICON_ZERO (13, default) | EFFPOT_ICON_PERIODIC */
+  int32_t precision;  /* EFFPOT_PREC_FP64 | EFFPOT_PREC_MIXED               */
+  int32_t batch;      /* number of independent solves (slots), >= 1         */
+  double xmax, ymax, zmax; /* box half widths; h = 2*xmax/nx (DFT4He3d.f90:205-207) */
+  double n4;          /* number of He atoms the density is normalised to   */
+  double h_core;      /* LJ core radius h4 (A)                             */
+  double eps4, sigma4;          /* LJ parameters (modules.f90:193-194)     */
+  double cp4, cpp4;             /* correlation c2, c3 (modules.f90:122-123)*/
+  double alphas, l, den0s;      /* alpha_s term (modules.f90:125-127)      */
+  double h2o2m4;                /* hbar^2/2m_4 (modules.f90:128)           */
+  double C, beta, den_m;        /* solid penalty (modules.f90:129-131)     */
+} effpot_params;
+
+/* Impurity: the potential kind and its parameters, the Select_Pot cutoff
+ * (r < r_cutoff -> umax, V_impur.f90:87-932 pattern), the cap applied to the
+ * interpolated field (potenimp.f90:184-186, 1000 K) and the position. */
+typedef struct effpot_impurity {
+  int32_t kind;      /* EFFPOT_POT_MORSE | EFFPOT_POT_NULL               */
+  int32_t pad_;
+  double D, re, a;   /* Morse: depth (K), minimum (A), range (1/A)        */
+  double r_cutoff;   /* r < r_cutoff -> umax                              */
+  double umax;
+  double cap;        /* uimp = min(uimp, cap)                             */
+  double rimp[3];    /* impurity position (A)                             */
+} effpot_impurity;
+
+/* Energies as printed by DFT4He3d.f90:460-463 (formats 6050/6060) and
+ * computed by energy.f90:12-202, plus the chemical potential and the
+ * helium centre of mass (r_cm.f90:22-57). */
+typedef struct effpot_energies {
+  double natoms;   /* sum(den)*dxyz                                       */
+  double etot4;    /* helium total (energy.f90:101-109)                   */
+  double ekin4;    /* h2o2m4 sum |grad psi|^2 dV                          */
+  double elj4;     /* 1/2 sum delj4*den dV                                */
+  double ealphas;  /* alpha_s term (OTC)                                  */
+  double esolid;   /* solid penalty energy, reported, not added (:96)     */
+  double ecor4;    /* correlation                                         */
+  double eimpu;    /* impurity energy E_imp = ekinx + eHeX + eso (:195)   */
+  double ekinx;    /* classical impurity kinetic energy (0: fixed)        */
+  double eHeX;     /* sum den*uimp dV (:146-165)                          */
+  double uplus;    /* X+-He interaction (0: no selec_plus table)          */
+  double eso;      /* spin-orbit (0 for an isotropic impurity)            */
+  double etot;     /* etot4 + eimpu (:197)                                */
+  double mu;       /* chemical potential <psi|H|psi>/N                    */
+  double xcm, ycm, zcm; /* helium centre of mass, normalised by n4          */
+} effpot_energies;
+
+/* Options of the relaxation loop (effpot_solve). */
+typedef struct effpot_solve_opts {
+  double dt;        /* imaginary time step (1/K)                          */
+  double tol;       /* converged when |dE/E| per step < tol               */
+  int32_t n_rk;     /* Runge-Kutta-Gill starter steps (3)                  */
+  int32_t pener;    /* steps between energy evaluations (DFT4He3d pener)   */
+  int32_t max_steps;/* hard cap on steps (not counting polish)             */
+  int32_t polish_steps; /* MIXED only: FP64 steps after the FP32 phase      */
+  double tol_fp32;  /* MIXED only: per-step |dE/E| that ends the FP32 phase */
+  int32_t check_finite; /* 1: scan the density for NaN/<=0 at every energy */
+  int32_t pad_;
+} effpot_solve_opts;
+
+typedef struct effpot_result {
+  effpot_energies e; /* energies of the final state                       */
+  int32_t steps;     /* steps taken (RK + PC [+ polish])                   */
+  int32_t converged; /* 1 if the tol criterion was met                     */
+  double rel_de;     /* last |dE/E| per step                               */
+  double err_he;     /* last predictor-corrector error estimate errHe      */
+} effpot_result;
+
+typedef struct effpot_ctx effpot_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+/* Replaces dimen() (dimen.f90:8), Init_deriv_p (init_deriv_parallel.f90:51-82),
+ * fftini (fft.f90:6-55) and the one-time k-space kernels fforma/initcg/kalfs
+ * (fforma.f90:11-60, initcg.f90:5-51, DFT4He3d.f90:396-400).  Selects HIP
+ * device `device`. */
+int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out);
+int effpot_ctx_destroy(effpot_ctx* ctx);
+const char* effpot_last_error(const effpot_ctx* ctx);
+/* Device bytes held by a context with these parameters (admission control). */
+int64_t effpot_footprint_bytes(const effpot_params* p);
+
+/* ---- state I/O (readenc.f90:151-168 / printoutc.f90:69-114 payloads) --- */
+int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi);
+int effpot_download_psi(effpot_ctx* ctx, int slot, double* psi);
+int effpot_download_den(effpot_ctx* ctx, int slot, double* den);
+/* Download a field of the last poten(): 0 pot4, 1 dencg, 2 delj4, 3 denalf,
+ * 4 ualphas, 5 uimp, 6 dxden, 7 dyden, 8 dzden, 9 sto1 (correlation
+ * convolution), 10 intxalf, 11 intyalf, 12 intzalf. */
+int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out);
+/* Fermi droplet rho0/(1+exp((r-R)/afermi)) + noise*u(seed,i), psi=sqrt(rho)
+ * normalised to n4, centred at (xc,yc,zc).  The reference promises this
+ * start (readenc.f90:15-17) but never builds it.  Computed on the host so
+ * that the CPU and GPU start from bit-identical input. */
+int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R,
+                      double afermi, double noise, uint64_t seed,
+                      const double centre[3]);
+
+/* ---- impurity field (potenimpini.f90 / updatepoten+update+potenimp) ---- */
+int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp);
+
+/* ---- operators on all active slots ------------------------------------ */
+int effpot_poten(effpot_ctx* ctx);                         /* poten.f90:5   */
+int effpot_energy(effpot_ctx* ctx, effpot_energies* out);  /* energy.f90:12, out[batch] */
+int effpot_step_rk(effpot_ctx* ctx, double dt);            /* steprk.f90:1  */
+int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he); /* steppc.f90:1, err_he[batch] or NULL */
+int effpot_renormalize(effpot_ctx* ctx);                   /* psi *= sqrt(n4/N) */
+/* One 13-point derivative (derivnd, init_deriv_parallel.f90:1-46) of a real
+ * host field f along axis iv (1..3), order id (1..2), into df (host). */
+int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df);
+/* 3-D transforms of fft.f90 on host buffers: r2c unnormalised into the
+ * half spectrum (nx/2+1,ny,nz) interleaved; c2r scaled by 1/(nx ny nz). */
+int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out);
+int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out);
+/* Slots taking part in the operators above (1 active, 0 frozen). */
+int effpot_set_active(effpot_ctx* ctx, int slot, int active);
+
+/* ---- whole relaxation ---------------------------------------------------- */
+/* RK-Gill starter + Hamming predictor-corrector in imaginary time with
+ * renormalisation to n4 after every step, energy every `pener` steps, until
+ * |dE/E|/pener < tol for every active slot.  out[batch]. */
+int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out);
+
+/* Synchronise the context's stream. */
+int effpot_sync(effpot_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EFFPOT_H */

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/effpot.h (no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import eff_pot_amd as ep
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "effpot.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(effpot_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_all_declared_symbols():
+    names = declared()
+    assert "effpot_solve" in names and "effpot_poten" in names
+    lib = ctypes.CDLL(ep.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header():
+    # 8 int32 + 16 doubles
+    assert ctypes.sizeof(ep.Params) == 8 * 4 + 16 * 8
+    assert ctypes.sizeof(ep.Impurity) == 8 + 9 * 8
+    assert ctypes.sizeof(ep.Energies) == 17 * 8
+    assert ctypes.sizeof(ep.SolveOpts) == 2 * 8 + 4 * 4 + 8 + 8
+
+
+def test_footprint_is_host_only():
+    p = ep.make_params(128, xmax=32.0, batch=4)
+    b = ep.footprint_bytes(p)
+    assert b > 4 * 128 ** 3 * 8 * 30
+    p.precision = ep.PREC_MIXED
+    assert ep.footprint_bytes(p) > b
