@@ -1,0 +1,125 @@
+! iso_c_binding interface of the gfx950 solver library (include/effpot.h).
+! The derived types mirror the C structs field for field; every function
+! returns the C status (0 = EFFPOT_OK).
+module effpot_c
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: EFFPOT_OK = 0
+  integer(c_int), parameter :: EFFPOT_CORE_OT = 0, EFFPOT_CORE_OTC = 1
+  integer(c_int), parameter :: EFFPOT_PREC_FP64 = 0, EFFPOT_PREC_MIXED = 1
+  integer(c_int), parameter :: EFFPOT_POT_NULL = 0, EFFPOT_POT_MORSE = 1
+
+  type, bind(c) :: effpot_params
+    integer(c_int32_t) :: nx, ny, nz, core, npd, icon, precision, batch
+    real(c_double) :: xmax, ymax, zmax, n4, h_core, eps4, sigma4, cp4, cpp4
+    real(c_double) :: alphas, l, den0s, h2o2m4, cpen, beta, den_m
+  end type
+
+  type, bind(c) :: effpot_impurity
+    integer(c_int32_t) :: kind, pad_
+    real(c_double) :: d, re, a, r_cutoff, umax, cap
+    real(c_double) :: rimp(3)
+  end type
+
+  type, bind(c) :: effpot_energies
+    real(c_double) :: natoms, etot4, ekin4, elj4, ealphas, esolid, ecor4, eimpu
+    real(c_double) :: ekinx, ehex, uplus, eso, etot, mu, xcm, ycm, zcm
+  end type
+
+  type, bind(c) :: effpot_solve_opts
+    real(c_double) :: dt, tol
+    integer(c_int32_t) :: n_rk, pener, max_steps, polish_steps
+    real(c_double) :: tol_fp32
+    integer(c_int32_t) :: check_finite, pad_
+  end type
+
+  type, bind(c) :: effpot_result
+    type(effpot_energies) :: e
+    integer(c_int32_t) :: steps, converged
+    real(c_double) :: rel_de, err_he
+  end type
+
+  interface
+    integer(c_int) function effpot_ctx_create(p, device, ctx) bind(c, name="effpot_ctx_create")
+      import :: c_int, c_ptr, effpot_params
+      type(effpot_params), intent(in) :: p
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function effpot_ctx_destroy(ctx) bind(c, name="effpot_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function effpot_last_error(ctx) bind(c, name="effpot_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function effpot_upload_psi(ctx, slot, psi) bind(c, name="effpot_upload_psi")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      complex(c_double_complex), intent(in) :: psi(*)
+    end function
+    integer(c_int) function effpot_download_psi(ctx, slot, psi) bind(c, name="effpot_download_psi")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      complex(c_double_complex), intent(out) :: psi(*)
+    end function
+    integer(c_int) function effpot_init_fermi(ctx, slot, rho0, r, afermi, noise, seed, centre) &
+        bind(c, name="effpot_init_fermi")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), value :: rho0, r, afermi, noise
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(in) :: centre(3)
+    end function
+    integer(c_int) function effpot_set_impurity(ctx, slot, imp) bind(c, name="effpot_set_impurity")
+      import :: c_int, c_ptr, effpot_impurity
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      type(effpot_impurity), intent(in) :: imp
+    end function
+    integer(c_int) function effpot_poten(ctx) bind(c, name="effpot_poten")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function effpot_energy(ctx, e) bind(c, name="effpot_energy")
+      import :: c_int, c_ptr, effpot_energies
+      type(c_ptr), value :: ctx
+      type(effpot_energies), intent(out) :: e
+    end function
+    integer(c_int) function effpot_solve(ctx, opt, res) bind(c, name="effpot_solve")
+      import :: c_int, c_ptr, effpot_solve_opts, effpot_result
+      type(c_ptr), value :: ctx
+      type(effpot_solve_opts), intent(in) :: opt
+      type(effpot_result), intent(out) :: res
+    end function
+  end interface
+
+contains
+
+  ! Turns a non-zero status into `error stop` with the library's message
+  ! (the reference STOPs at the same points).
+  subroutine check(ctx, rc, where)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (rc == EFFPOT_OK) return
+    text = ''
+    if (c_associated(ctx)) then
+      call c_f_pointer(effpot_last_error(ctx), msg, [512])
+      do i = 1, 512
+        if (msg(i) == c_null_char) exit
+        text(i:i) = msg(i)
+      end do
+    end if
+    write(0, '(a,": status ",i0,": ",a)') where, rc, trim(text)
+    error stop 'effpot: severe error'
+  end subroutine check
+end module effpot_c

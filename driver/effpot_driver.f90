@@ -1,0 +1,180 @@
+! Driver of the GPU eff_pot path.  Same namelist (/input/, DFT4He3d.f90:104-134
+! subset plus the Morse/relaxation keys), the same density-file format
+! (readenc mode 3, readenc.f90:151-168) and the same stdout energy labels
+! (DFT4He3d.f90:460-463, formats 6050/6060) as the Fortran reference, so that
+! get_impurity_energy.sh and loop.sh keep working; the field operators run on
+! the GPU through effpot_c.
+program effpot_driver
+  use iso_c_binding
+  use effpot_c
+  implicit none
+
+  character(len=40) :: title = 'Helium4 - 3dim.  '
+  character(len=60) :: filedenin = 'he4.input.density'
+  character(len=3) :: core4 = 'OTC'
+  character(len=80) :: selec_pi = 'Morse'
+  integer :: n4 = 300, mode = 0, nx = 64, ny = 64, nz = 64, npd = 13, icon = 13
+  integer :: niter = 0, pener = 50, nthread = 1, precision = 0, ninvar = 10
+  integer(c_int64_t) :: seed = 20261203_c_int64_t
+  real(8) :: xmax = 24.d0, ymax = 24.d0, zmax = 24.d0, xc = 0.d0, yc = 0.d0, zc = 0.d0
+  real(8) :: eps4 = 10.22d0, sigma4 = 2.556d0, h_core = 2.377d0, l = 1.d0
+  real(8) :: cp4 = -2.41186d4, cpp4 = 1.85850d6, den4c = 0.062d0, alphas = 54.31d0, den0s = 0.04d0
+  real(8) :: h2o2m4 = 6.05969638298d0, deltat = 0.d0, deltatps = 0.d0
+  real(8) :: ximp = 0.d0, yimp = 0.d0, zimp = 0.d0
+  real(8) :: morse_d = 30.d0, morse_re = 4.d0, morse_a = 1.d0
+  real(8) :: r_cutoff_pi = 0.d0, umax_pi = 0.d0, afermi = 0.6d0, rho0 = 0.0218d0, noise = 1.d-6
+  real(8) :: tol = 1.d-9
+  real(8), parameter :: ktops = 7.63822291d0
+  namelist /input/ title, filedenin, core4, selec_pi, n4, mode, nx, ny, nz, npd, icon, niter, pener, &
+                   nthread, precision, seed, xmax, ymax, zmax, xc, yc, zc, eps4, sigma4, h_core, l, &
+                   cp4, cpp4, den4c, alphas, den0s, h2o2m4, deltat, deltatps, ximp, yimp, zimp, &
+                   morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar
+
+  type(effpot_params) :: p
+  type(effpot_impurity) :: imp
+  type(effpot_energies) :: e
+  type(effpot_solve_opts) :: opt
+  type(effpot_result) :: res
+  type(c_ptr) :: ctx
+  complex(8), allocatable :: psi(:,:,:)
+  real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, rimp(3), vimp(3), hx, hy, hz
+  integer :: nxp, nyp, nzp, nninvar0, isalto
+  complex(8), allocatable :: invar0(:)
+  real(8) :: dist
+
+  read(5, nml=input)
+  open(10, file="DFT4He3d.namelist.read")
+  write(10, nml=input)
+  close(10)
+  if (tstgrid(nx) /= 0) stop 'SEVERE ERROR: NX is not correct'
+  if (tstgrid(ny) /= 0) stop 'SEVERE ERROR: NY is not correct'
+  if (tstgrid(nz) /= 0) stop 'SEVERE ERROR: NZ is not correct'
+  if (deltat == 0.d0) then
+    if (deltatps == 0.d0) then
+      if (niter > 0) stop 'You must specify either Deltat (in kelvin) or Deltatps (in picosecond)'
+    else
+      deltat = deltatps / ktops
+    end if
+  else if (deltatps /= 0.d0 .and. deltatps /= deltat * ktops) then
+    stop 'Inconsistent deltat - deltatps'
+  end if
+
+  p%nx = nx; p%ny = ny; p%nz = nz; p%npd = npd; p%icon = icon; p%precision = precision; p%batch = 1
+  select case (core4)
+  case ('OT ', 'OT')
+    p%core = EFFPOT_CORE_OT
+  case ('OTC')
+    p%core = EFFPOT_CORE_OTC
+  case default
+    stop ' ... The program stops due to a severe error.'
+  end select
+  p%xmax = xmax; p%ymax = ymax; p%zmax = zmax; p%n4 = dble(n4); p%h_core = h_core
+  p%eps4 = eps4; p%sigma4 = sigma4; p%cp4 = cp4; p%cpp4 = cpp4; p%alphas = alphas; p%l = l
+  p%den0s = den0s; p%h2o2m4 = h2o2m4; p%cpen = 3.1577504d4; p%beta = 40.d0; p%den_m = 0.37d0
+  call check(c_null_ptr, effpot_ctx_create(p, 0_c_int, ctx), 'effpot_ctx_create')
+
+  hx = 2.d0 * xmax / nx; hy = 2.d0 * ymax / ny; hz = 2.d0 * zmax / nz
+  rimp = (/ximp, yimp, zimp/)
+  select case (mode)
+  case (0)   ! Fermi droplet (the start readenc.f90:15-17 describes)
+    call check(ctx, effpot_init_fermi(ctx, 0_c_int, rho0, 2.22d0 * dble(n4)**(1.d0/3.d0), afermi, noise, &
+                                      seed, (/xc, yc, zc/)), 'effpot_init_fermi')
+  case (3)   ! wavefunction snapshot (readenc.f90:151-168)
+    allocate(psi(nx, ny, nz))
+    open(unit=1, file=filedenin, status='old')
+    call titols(1, '#', isalto)
+    read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp
+    if (nxp /= nx .or. nyp /= ny .or. nzp /= nz) stop 'readenc: grid of the file differs from the namelist'
+    read(1, *) rimp
+    read(1, *) vimp
+    read(1, *) nninvar0
+    allocate(invar0(nninvar0))
+    read(1, *) invar0
+    read(1, *) psi
+    close(1)
+    call check(ctx, effpot_upload_psi(ctx, 0_c_int, psi), 'effpot_upload_psi')
+  case default
+    stop 'ERROR READEN: This mode is still not programmed'
+  end select
+
+  imp%pad_ = 0
+  if (trim(selec_pi) == 'Morse') then
+    imp%kind = EFFPOT_POT_MORSE
+  else if (trim(selec_pi) == 'null') then
+    imp%kind = EFFPOT_POT_NULL
+  else
+    write(6, '(A,": aquest potencial no est definit, de part de: Selec_Pot")') trim(selec_pi)
+    stop 'From V_impur: 0001'
+  end if
+  imp%d = morse_d; imp%re = morse_re; imp%a = morse_a
+  imp%r_cutoff = r_cutoff_pi; imp%umax = umax_pi; imp%cap = 1000.d0; imp%rimp = rimp
+  call check(ctx, effpot_set_impurity(ctx, 0_c_int, imp), 'effpot_set_impurity')
+
+  if (niter > 0) then
+    opt%dt = deltat; opt%tol = tol; opt%n_rk = 3; opt%pener = pener; opt%max_steps = niter
+    opt%polish_steps = niter; opt%tol_fp32 = 2.d-8; opt%check_finite = 1; opt%pad_ = 0
+    call check(ctx, effpot_solve(ctx, opt, res), 'effpot_solve')
+    e = res%e
+    write(6, '(" Relaxation: steps",i8,"  converged",i2,"  |dE/E| per step",1p,e12.4)') res%steps, &
+      res%converged, res%rel_de
+  else
+    call check(ctx, effpot_poten(ctx), 'effpot_poten')
+    call check(ctx, effpot_energy(ctx, e), 'effpot_energy')
+  end if
+
+  write(6, '("Number of He4 atoms",1P,E15.6)') e%natoms
+  write(6, 6050) e%natoms, e%etot4, e%etot4 / e%natoms, e%ekin4, e%elj4, e%ealphas, e%esolid, e%ecor4
+  write(6, 6060) e%eimpu, e%ekinx, e%ehex, e%uplus, e%eso, e%etot
+  dist = sqrt((rimp(1) - e%xcm)**2 + (rimp(2) - e%ycm)**2 + (rimp(3) - e%zcm)**2)
+  write(6, *) "X-HE CoM distance: ", dist
+  call check(ctx, effpot_ctx_destroy(ctx), 'effpot_ctx_destroy')
+
+6050 format(//,T5,'FIRST ENERGY BALANCE: ',                    //     &
+              ,T5,'Number of particles ..........: ',F18.6,'  ',/,    &
+               T5,'TOTAL   energy (He) ..........: ',F18.6,' K',/,    &
+               T5,'Energy per particle (He) .....: ',F18.6,' K',/,    &
+               T5,'Kinetic energy (He) ..........: ',F18.6,' K',/,    &
+               T5,'Lennard-Jones energy (He) ....: ',F18.6,' K',/,    &
+               T5,'Alpha_s term  energy (He) ....: ',F18.6,' K',/,    &
+               T5,'Solid energy (He)  ...........: ',F18.6,' K',/,    &
+               T5,'Correlation energy   (He) ....: ',F18.6,' K')
+6060 format(1x,T5,'Impurity energy (X) ..........: ',F18.6,' K',/,    &
+               T5,'Kinetic energy (X) ...........: ',F18.6,' K',/,    &
+               T5,'Interaction energy (X*-He) ...: ',F18.6,' K',/,    &
+               T5,'Interaction energy (X+-He) ...: ',F18.6,' K',/,    &
+               T5,'Spin-Orbit energy (X) ........: ',F18.6,' K',/,    &
+               T5,'TOTAL ENERGY (He+X) ..........: ',F18.6,' K',/)
+
+contains
+
+  integer function tstgrid(n)   ! n = 2^a 3^b 5^c <= 256 (tstgrid.f90:8-12)
+    integer, intent(in) :: n
+    integer :: m
+    tstgrid = 1
+    if (n < 16 .or. n > 256) return
+    m = n
+    do while (mod(m, 2) == 0); m = m / 2; end do
+    do while (mod(m, 3) == 0); m = m / 3; end do
+    do while (mod(m, 5) == 0); m = m / 5; end do
+    if (m == 1) tstgrid = 0
+  end function tstgrid
+
+  subroutine titols(ulog, cchar, isalto)   ! skip the leading comment lines (titols.f90)
+    integer, intent(in) :: ulog
+    character(len=1), intent(in) :: cchar
+    integer, intent(out) :: isalto
+    character(len=1) :: pchar
+    integer :: i, ios
+    isalto = 0
+    do
+      read(ulog, '(A1)', iostat=ios) pchar
+      if (ios /= 0) stop 'STOP due to severe errors in routine TITOLS'
+      if (pchar /= cchar) exit
+      isalto = isalto + 1
+    end do
+    rewind(ulog)
+    do i = 1, isalto
+      read(ulog, *)
+    end do
+  end subroutine titols
+end program effpot_driver

@@ -61,6 +61,7 @@ def lib():
         L.effpot_set_active.argtypes = [vp, C.c_int, C.c_int]
         L.effpot_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Result)]
         L.effpot_sync.argtypes = [vp]
+        L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -183,6 +184,11 @@ class Context:
         r = (Result * self.B)()
         self._chk(lib().effpot_solve(self.h, C.byref(opts), r))
         return list(r)
+
+    def kernel_time(self, which, reps=20):
+        ms = C.c_double()
+        self._chk(lib().effpot_kernel_time(self.h, which, reps, C.byref(ms)))
+        return ms.value
 
     def sync(self):
         self._chk(lib().effpot_sync(self.h))

@@ -1031,6 +1031,40 @@ static int relax_phase(effpot_ctx* ctx, State<T>& s, double dt, double tol, int 
   return EFFPOT_OK;
 }
 
+template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int which, int reps, double* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  StepArgs<T> a = step_args(ctx, s, 1e-4, ctx->d_active);
+  auto once = [&]() -> int {
+    switch (which) {
+      case 0: return launch_step<T, 0>(ctx, s, a);
+      case 1: return launch_step<T, 1>(ctx, s, a);
+      case 2: return poten_t(ctx, s, ctx->d_active);
+      default: return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
+    }
+  };
+  RC(once());
+  HIPCHK(hipEventRecord(e0, ctx->st));
+  for (int r = 0; r < reps; r++) RC(once());
+  HIPCHK(hipEventRecord(e1, ctx->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = (double)t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return EFFPOT_OK;
+}
+
+int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch) {
+  RC(check_ctx(ctx));
+  if (!ms_per_launch || reps < 1) return ctx->fail(EFFPOT_ERR_ARG, "bad arguments");
+  hipSetDevice(ctx->dev);
+  if (ctx->mixed) return time_kernel_t(ctx, ctx->s32, which, reps, ms_per_launch);
+  return time_kernel_t(ctx, ctx->s64, which, reps, ms_per_launch);
+}
+
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {
   RC(check_ctx(ctx));
   if (!opt || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");

@@ -180,6 +180,14 @@ int effpot_set_active(effpot_ctx* ctx, int slot, int active);
  * |dE/E|/pener < tol for every active slot.  out[batch]. */
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out);
 
+/* Benchmark hook (no reference counterpart): average duration in ms of one
+ * launch of a hot-path kernel on the context's stream, measured with HIP
+ * events over `reps` back-to-back launches on the current (live-precision)
+ * state: 0 = H psi + predictor/modifier, 1 = H psi + corrector, 2 = the whole
+ * poten() kernel sequence.  The launches overwrite the integrator state:
+ * call it after the solves that matter. */
+int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
+
 /* Synchronise the context's stream. */
 int effpot_sync(effpot_ctx* ctx);
 

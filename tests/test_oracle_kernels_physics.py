@@ -44,6 +44,20 @@ def test_fvlj_quadrature(p):
     assert abs(got - ref) < 1e-7 * abs(lib().oracle_bforce(eps, sig, hc)), (got, ref)
 
 
+def test_fvlj_exact_mpmath():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    eps, sig, hc = 10.22, 2.556, 2.377
+    for p, tol in ((0.3, 1e-10), (1.0, 5e-8), (1.7320508075688772, 5e-6)):
+        a = 2 * mp.pi * mp.mpf(p)
+
+        def I(n):
+            return a ** (n - 1) * mp.quadosc(lambda u: mp.sin(u) / u ** n, [a * hc, mp.inf], omega=1)
+        s6 = mp.mpf(sig) ** 6
+        exact = float(8 * eps * s6 * (s6 * I(11) - I(5)) / p)
+        assert abs(lib().oracle_fvlj(p, eps, sig, hc) - exact) < tol, p
+
+
 def test_bforce_limit():
     b = lib().oracle_bforce(10.22, 2.556, H4OT)
     assert abs(b - (-718.99)) < 5e-3
