@@ -12,7 +12,8 @@ import os
 
 import numpy as np
 
-from .abi import (Params, Impurity, Energies, SolveOpts, Result, OK, FIELDS,  # noqa: F401
+from .abi import (Params, Impurity, Energies, SolveOpts, Result, Slot, OK, FIELDS,  # noqa: F401
+                  SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
                   ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, H4OT)
 
@@ -50,6 +51,7 @@ def lib():
         L.effpot_init_fermi.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _dp]
         L.effpot_set_impurity.argtypes = [vp, C.c_int, C.POINTER(Impurity)]
+        L.effpot_set_natoms.argtypes = [vp, C.c_int, C.c_double]
         L.effpot_poten.argtypes = [vp]
         L.effpot_energy.argtypes = [vp, C.POINTER(Energies)]
         L.effpot_step_rk.argtypes = [vp, C.c_double]
@@ -62,6 +64,7 @@ def lib():
         L.effpot_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Result)]
         L.effpot_sync.argtypes = [vp]
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
         _lib = L
     return _lib
 
@@ -135,6 +138,9 @@ class Context:
         self._chk(lib().effpot_init_fermi(self.h, slot, rho0, R, afermi, noise, seed,
                                           np.asarray(centre, dtype=np.float64)))
 
+    def set_natoms(self, n4, slot=0):
+        self._chk(lib().effpot_set_natoms(self.h, slot, n4))
+
     def set_impurity(self, imp: Impurity, slot=0):
         self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))
 
@@ -184,6 +190,13 @@ class Context:
         r = (Result * self.B)()
         self._chk(lib().effpot_solve(self.h, C.byref(opts), r))
         return list(r)
+
+    def new_slots(self):
+        return (Slot * self.B)()
+
+    def tick(self, opts: SolveOpts, slots):
+        """Advance every START/RK/PC slot by one step of its own phase."""
+        self._chk(lib().effpot_tick(self.h, C.byref(opts), slots))
 
     def kernel_time(self, which, reps=20):
         ms = C.c_double()

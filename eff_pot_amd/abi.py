@@ -52,6 +52,15 @@ class SolveOpts(C.Structure):
                 ("tol_fp32", C.c_double), ("check_finite", C.c_int32), ("pad_", C.c_int32)]
 
 
+SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED = range(6)
+
+
+class Slot(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("fp32", C.c_int32), ("nrk", C.c_int32), ("steps", C.c_int32),
+                ("since", C.c_int32), ("converged", C.c_int32), ("eprev", C.c_double),
+                ("rel_de", C.c_double), ("e", Energies)]
+
+
 class Result(C.Structure):
     _fields_ = [("e", Energies), ("steps", C.c_int32), ("converged", C.c_int32),
                 ("rel_de", C.c_double), ("err_he", C.c_double)]

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -178,9 +179,16 @@ struct effpot_ctx {
   int nblk_max = 0;
   double* d_red = nullptr;    // [B][16]
   double* d_scale = nullptr;  // [B]
+  double* d_n4 = nullptr;     // [B] atoms each slot is normalised to
+  std::vector<double> h_n4;
   int* d_bad = nullptr;
+  int* d_mask = nullptr;      // [5][B] per-purpose activity masks of effpot_tick
+  int* h_mask = nullptr;      // pinned staging for d_mask
   double* h_pin = nullptr;    // pinned host buffer
   int tx = 64, zc = 16;
+  bool graphs = true;               // replay captured PC steps (EFFPOT_GRAPHS=0 disables)
+  hipGraphExec_t pc_exec[2] = {nullptr, nullptr};  // [0] FP64, [1] FP32
+  double pc_dt[2] = {0.0, 0.0};
   std::vector<void*> allocs;
   std::string err;
 
@@ -415,7 +423,7 @@ template <typename T, int MODE> int launch_step(effpot_ctx* ctx, State<T>& s, St
 
 template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act) {
   hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
-                     act, mode, ctx->p.n4, ctx->dxyz, ctx->d_scale);
+                     act, mode, (const double*)ctx->d_n4, ctx->dxyz, ctx->d_scale);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -599,7 +607,8 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
     e.eimpu = e.ekinx + e.eHeX + e.eso;
     e.etot = e.etot4 + e.eimpu;
     e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
-    e.xcm = S[7] * dv / p.n4; e.ycm = S[8] * dv / p.n4; e.zcm = S[9] * dv / p.n4;
+    const double n4 = ctx->h_n4[b];
+    e.xcm = S[7] * dv / n4; e.ycm = S[8] * dv / n4; e.zcm = S[9] * dv / n4;
     out[b] = e;
   }
   return EFFPOT_OK;
@@ -652,6 +661,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->tx = p->nx % 64 == 0 ? 64 : (p->nx % 32 == 0 ? 32 : (p->nx % 16 == 0 ? 16 : 8));
   ctx->zc = std::min(16, p->nz);
   ctx->mixed = p->precision == EFFPOT_PREC_MIXED;
+  if (const char* g = std::getenv("EFFPOT_GRAPHS")) ctx->graphs = std::atoi(g) != 0;
   ctx->h_active.assign(ctx->B, 1);
   ctx->have_imp.assign(ctx->B, 0);
   int rc;
@@ -666,7 +676,14 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_partial, (size_t)ctx->B * ctx->nblk_max * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_red, (size_t)ctx->B * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_scale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_n4, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->h_n4.assign(ctx->B, p->n4);
+  if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_bad, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_mask, (size_t)ctx->B * 5)) != EFFPOT_OK) return bail(rc);
+  if (hipHostMalloc((void**)&ctx->h_mask, sizeof(int) * ctx->B * 5, hipHostMallocDefault) != hipSuccess)
+    return bail(EFFPOT_ERR_NOMEM);
   if (hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * ctx->B * 16 + 64, hipHostMallocDefault) != hipSuccess)
     return bail(EFFPOT_ERR_NOMEM);
   std::vector<int> ones(ctx->B, 1);
@@ -692,8 +709,11 @@ int effpot_ctx_destroy(effpot_ctx* ctx) {
   if (!ctx) return EFFPOT_ERR_ARG;
   hipSetDevice(ctx->dev);
   if (ctx->st) hipStreamSynchronize(ctx->st);
+  for (int k = 0; k < 2; k++)
+    if (ctx->pc_exec[k]) hipGraphExecDestroy(ctx->pc_exec[k]);
   for (void* v : ctx->allocs) hipFree(v);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  if (ctx->h_mask) hipHostFree(ctx->h_mask);
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
   return EFFPOT_OK;
@@ -832,9 +852,17 @@ int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R, double a
         rho[i] = v;
         sum += v;
       }
-  double f = std::sqrt(ctx->p.n4 / (sum * ctx->dxyz));
+  double f = std::sqrt(ctx->h_n4[slot] / (sum * ctx->dxyz));
   for (long i = 0; i < ctx->N; i++) { psi[2 * i] = std::sqrt(rho[i]) * f; psi[2 * i + 1] = 0.0; }
   return effpot_upload_psi(ctx, slot, psi.data());
+}
+
+int effpot_set_natoms(effpot_ctx* ctx, int slot, double n4) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B || !(n4 > 0)) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or atom number");
+  ctx->h_n4[slot] = n4;
+  HIPCHK(hipMemcpy(ctx->d_n4 + slot, &n4, sizeof(double), hipMemcpyHostToDevice));
+  return EFFPOT_OK;
 }
 
 int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
@@ -966,67 +994,164 @@ int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out) {
   return EFFPOT_OK;
 }
 
-// One precision phase of the relaxation on the slots in `mask` (host copy):
-// [RK starter] + predictor-corrector with renormalisation after every step,
-// energy every pener steps; a slot leaves the mask when its |dE/E| per step
-// drops below tol.  Returns per-slot results in res (steps accumulate).
+// ---- continuous-batching engine ---------------------------------------------------
+// Device masks: 0 start, 1 rk, 2 pc, 3 energy check, 4 polish conversion.
+static int push_mask(effpot_ctx* ctx, int k, const std::vector<int>& m) {
+  int* h = ctx->h_mask + (long)k * ctx->B;
+  std::memcpy(h, m.data(), sizeof(int) * ctx->B);
+  HIPCHK(hipMemcpyAsync(ctx->d_mask + (long)k * ctx->B, h, sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  return EFFPOT_OK;
+}
+
+// FP32 state of slot b -> FP64 live buffer with a fresh integrator history.
+static int widen_slot(effpot_ctx* ctx, int b) {
+  int idx;
+  RC(cur_index(ctx, ctx->s32, b, &idx));
+  RC(reset_slot(ctx, ctx->s64, b));
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, ctx->st,
+                     (const float*)(ctx->s32.psi + ((long)b * 5 + idx) * ctx->N),
+                     (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), 2 * ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
+// One predictor-corrector step + renormalisation of the slots in mask dP.  The
+// launch sequence only references device memory (rings, masks, fields), so it
+// is captured once per precision and time step and replayed as a HIP graph.
 template <typename T>
-static int relax_phase(effpot_ctx* ctx, State<T>& s, double dt, double tol, int n_rk, int pener, int max_steps,
-                       int check_finite, std::vector<int>& mask, std::vector<effpot_result>& res) {
-  const int B = ctx->B;
-  auto push_mask = [&]() -> int {
-    HIPCHK(hipMemcpyAsync(ctx->d_active, mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
-    return EFFPOT_OK;
-  };
-  RC(push_mask());
-  std::vector<effpot_energies> e(B);
-  std::vector<double> eprev(B, 0.0);
-  RC(poten_t(ctx, s, ctx->d_active));
-  int steps = 0;
-  for (int k = 0; k < n_rk; k++) {
-    RC(step_rk_t(ctx, s, dt, ctx->d_active));
-    RC(renorm_t(ctx, s, ctx->d_active));
-    steps++;
+static int pc_step_graph(effpot_ctx* ctx, State<T>& s, int which, double dt, const int* dP) {
+  if (!ctx->graphs) {
+    RC(step_pc_t(ctx, s, dt, dP));
+    return renorm_t(ctx, s, dP);
   }
-  RC(poten_t(ctx, s, ctx->d_active));
-  RC(energy_t(ctx, s, ctx->d_active, e.data()));
-  for (int b = 0; b < B; b++) if (mask[b]) eprev[b] = e[b].etot;
-  int since = 0;
-  auto any = [&]() { for (int b = 0; b < B; b++) if (mask[b]) return true; return false; };
-  while (any() && steps < max_steps) {
-    RC(step_pc_t(ctx, s, dt, ctx->d_active));
-    RC(renorm_t(ctx, s, ctx->d_active));
-    steps++; since++;
-    if (since == pener || steps == max_steps) {
-      if (check_finite) {
-        HIPCHK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int) * B, ctx->st));
-        hipLaunchKernelGGL((k_check_den<T>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const T*)s.den, ctx->N,
-                           ctx->N, ctx->d_active, ctx->d_bad);
-        LAUNCHCHK();
-        std::vector<int> bad(B);
-        HIPCHK(hipMemcpyAsync(bad.data(), ctx->d_bad, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->st));
-        HIPCHK(hipStreamSynchronize(ctx->st));
-        for (int b = 0; b < B; b++)
-          if (mask[b] && bad[b]) return ctx->fail(EFFPOT_ERR_NONFINITE, "non-positive or non-finite density (slot " + std::to_string(b) + ")");
+  if (ctx->pc_exec[which] && ctx->pc_dt[which] != dt) {
+    HIPCHK(hipGraphExecDestroy(ctx->pc_exec[which]));
+    ctx->pc_exec[which] = nullptr;
+  }
+  if (!ctx->pc_exec[which]) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    int rc = step_pc_t(ctx, s, dt, dP);
+    if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, dP);
+    hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    if (rc != EFFPOT_OK) return rc;
+    if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->pc_exec[which], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    ctx->pc_dt[which] = dt;
+  }
+  HIPCHK(hipGraphLaunch(ctx->pc_exec[which], ctx->st));
+  return EFFPOT_OK;
+}
+
+template <typename T>
+static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve_opts* opt, effpot_slot* sl) {
+  const int B = ctx->B;
+  std::vector<int> ms(B, 0), mr(B, 0), mp(B, 0);
+  bool any_s = false, any_r = false, any_p = false;
+  for (int b = 0; b < B; b++) {
+    if (!ctx->h_active[b] || ((sl[b].fp32 != 0) != is32)) continue;
+    if (sl[b].phase == EFFPOT_SLOT_START) { ms[b] = 1; any_s = true; }
+    else if (sl[b].phase == EFFPOT_SLOT_RK) { mr[b] = 1; any_r = true; }
+    else if (sl[b].phase == EFFPOT_SLOT_PC) { mp[b] = 1; any_p = true; }
+  }
+  const int* dS = ctx->d_mask;
+  const int* dR = ctx->d_mask + B;
+  const int* dP = ctx->d_mask + 2 * B;
+  const int* dE = ctx->d_mask + 3 * B;
+  std::vector<effpot_energies> e(B);
+  const double tol = is32 ? opt->tol_fp32 : opt->tol;
+  const int cap = is32 || !ctx->mixed ? opt->max_steps : opt->max_steps + std::max(opt->polish_steps, 0);
+  // START: poten of the new states; they take their first RK step below.
+  if (any_s) {
+    RC(push_mask(ctx, 0, ms));
+    if (ctx->mixed && !is32) {
+      hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st,
+                         ring_of(ctx, ctx->s64), ctx->s64.den, dS, ctx->N);
+      LAUNCHCHK();
+    }
+    RC(poten_t(ctx, s, dS));
+    for (int b = 0; b < B; b++)
+      if (ms[b]) {
+        sl[b].nrk = 0; sl[b].since = 0;
+        if (opt->n_rk > 0) { sl[b].phase = EFFPOT_SLOT_RK; mr[b] = 1; any_r = true; }
+        else { sl[b].phase = EFFPOT_SLOT_PC; }
       }
-      RC(poten_t(ctx, s, ctx->d_active));
-      RC(energy_t(ctx, s, ctx->d_active, e.data()));
-      HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * B * 2, hipMemcpyDeviceToHost, ctx->st));
-      bool changed = false;
-      for (int b = 0; b < B; b++) {
-        if (!mask[b]) continue;
-        double rel = std::fabs(e[b].etot - eprev[b]) / std::fabs(e[b].etot) / (double)since;
-        if (!std::isfinite(e[b].etot)) return ctx->fail(EFFPOT_ERR_NONFINITE, "non-finite energy (slot " + std::to_string(b) + ")");
-        eprev[b] = e[b].etot;
-        res[b].e = e[b];
-        res[b].rel_de = rel;
-        res[b].steps += since + (res[b].steps == 0 && steps == since + n_rk ? n_rk : 0);
-        if (rel < tol) { res[b].converged = 1; mask[b] = 0; changed = true; }
+  }
+  std::vector<int> me(B, 0);
+  bool any_e = false;
+  if (any_r) {
+    RC(push_mask(ctx, 1, mr));
+    RC(step_rk_t(ctx, s, opt->dt, dR));
+    RC(renorm_t(ctx, s, dR));
+    for (int b = 0; b < B; b++)
+      if (mr[b]) {
+        sl[b].nrk++; sl[b].steps++;
+        if (sl[b].nrk >= opt->n_rk) { me[b] = 1; any_e = true; }  // baseline energy before PC
       }
-      since = 0;
-      if (changed) RC(push_mask());
+  }
+  if (any_p) {
+    RC(push_mask(ctx, 2, mp));
+    RC(pc_step_graph(ctx, s, is32 ? 1 : 0, opt->dt, dP));
+    for (int b = 0; b < B; b++)
+      if (mp[b]) {
+        sl[b].steps++; sl[b].since++;
+        if (sl[b].since >= opt->pener || sl[b].steps >= cap) { me[b] = 1; any_e = true; }
+      }
+  }
+  if (any_e) {
+    RC(push_mask(ctx, 3, me));
+    if (opt->check_finite) {
+      HIPCHK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int) * B, ctx->st));
+      hipLaunchKernelGGL((k_check_den<T>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const T*)s.den, ctx->N, ctx->N,
+                         dE, ctx->d_bad);
+      LAUNCHCHK();
+    }
+    RC(poten_t(ctx, s, dE));
+    RC(energy_t(ctx, s, dE, e.data()));
+    std::vector<int> bad(B, 0);
+    if (opt->check_finite) {
+      HIPCHK(hipMemcpyAsync(bad.data(), ctx->d_bad, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->st));
+      HIPCHK(hipStreamSynchronize(ctx->st));
+    }
+    for (int b = 0; b < B; b++) {
+      if (!me[b]) continue;
+      effpot_slot& q = sl[b];
+      if (bad[b] || !std::isfinite(e[b].etot)) { q.phase = EFFPOT_SLOT_FAILED; continue; }
+      if (q.phase == EFFPOT_SLOT_RK) {  // end of the RK starter
+        q.phase = EFFPOT_SLOT_PC; q.eprev = e[b].etot; q.e = e[b]; q.since = 0;
+        continue;
+      }
+      double rel = std::fabs(e[b].etot - q.eprev) / std::fabs(e[b].etot) / (double)std::max(q.since, 1);
+      q.rel_de = rel; q.eprev = e[b].etot; q.e = e[b]; q.since = 0;
+      if (rel < tol || q.steps >= cap) {
+        if (is32) {
+          RC(widen_slot(ctx, b));
+          q.fp32 = 0; q.phase = EFFPOT_SLOT_START;
+        } else {
+          q.phase = EFFPOT_SLOT_DONE; q.converged = rel < tol;
+        }
+      }
     }
   }
+  return EFFPOT_OK;
+}
+
+int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots) {
+  RC(check_ctx(ctx));
+  if (!opt || !slots) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  if (!(opt->dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
+  if (opt->pener < 1 || opt->max_steps < 1 || opt->n_rk < 0) return ctx->fail(EFFPOT_ERR_ARG, "bad solve options");
+  hipSetDevice(ctx->dev);
+  for (int b = 0; b < ctx->B; b++) {
+    int ph = slots[b].phase;
+    if (ctx->h_active[b] && ph >= EFFPOT_SLOT_START && ph <= EFFPOT_SLOT_PC && !ctx->have_imp[b])
+      return ctx->fail(EFFPOT_ERR_STATE, "impurity not set for slot " + std::to_string(b));
+    if (!ctx->mixed) slots[b].fp32 = 0;
+  }
+  if (ctx->mixed) RC(tick_prec(ctx, ctx->s32, true, opt, slots));
+  RC(tick_prec(ctx, ctx->s64, false, opt, slots));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }
@@ -1068,57 +1193,29 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {
   RC(check_ctx(ctx));
   if (!opt || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
-  if (!(opt->dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
-  if (opt->pener < 1 || opt->max_steps < 1 || opt->n_rk < 0) return ctx->fail(EFFPOT_ERR_ARG, "bad solve options");
-  hipSetDevice(ctx->dev);
   const int B = ctx->B;
-  for (int b = 0; b < B; b++)
-    if (ctx->h_active[b] && !ctx->have_imp[b]) return ctx->fail(EFFPOT_ERR_STATE, "impurity not set");
-  std::vector<effpot_result> res(B);
-  std::memset(res.data(), 0, sizeof(effpot_result) * B);
-  std::vector<int> mask = ctx->h_active;
-  const std::vector<int> user_mask = ctx->h_active;
-  int rc;
-  if (!ctx->mixed) {
-    rc = relax_phase(ctx, ctx->s64, opt->dt, opt->tol, opt->n_rk, opt->pener, opt->max_steps, opt->check_finite, mask, res);
-  } else {
-    rc = relax_phase(ctx, ctx->s32, opt->dt, opt->tol_fp32, opt->n_rk, opt->pener, opt->max_steps, opt->check_finite, mask,
-                     res);
-    if (rc == EFFPOT_OK) {
-      // FP64 polish: psi (FP32) -> FP64 live buffer, fresh integrator history
-      for (int b = 0; b < B; b++) {
-        if (!user_mask[b]) continue;
-        int idx;
-        if ((rc = cur_index(ctx, ctx->s32, b, &idx)) != EFFPOT_OK) break;
-        if ((rc = reset_slot(ctx, ctx->s64, b)) != EFFPOT_OK) break;
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, ctx->st,
-                           (const float*)(ctx->s32.psi + ((long)b * 5 + idx) * ctx->N),
-                           (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), 2 * ctx->N);
-        LAUNCHCHK();
-      }
-      if (rc == EFFPOT_OK) {
-        HIPCHK(hipMemcpyAsync(ctx->d_active, user_mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
-        hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, ring_of(ctx, ctx->s64),
-                           ctx->s64.den, ctx->d_active, ctx->N);
-        LAUNCHCHK();
-        std::vector<int> m2 = user_mask;
-        std::vector<effpot_result> r2(B);
-        std::memset(r2.data(), 0, sizeof(effpot_result) * B);
-        rc = relax_phase(ctx, ctx->s64, opt->dt, opt->tol, opt->n_rk, opt->pener,
-                         opt->polish_steps > 0 ? opt->polish_steps : opt->max_steps, opt->check_finite, m2, r2);
-        for (int b = 0; b < B; b++) {
-          if (!user_mask[b]) continue;
-          int st32 = res[b].steps;
-          res[b] = r2[b];
-          res[b].steps += st32;
-        }
-      }
-    }
+  std::vector<effpot_slot> sl(B);
+  std::memset(sl.data(), 0, sizeof(effpot_slot) * B);
+  for (int b = 0; b < B; b++) {
+    if (!ctx->h_active[b]) continue;
+    sl[b].phase = EFFPOT_SLOT_START;
+    sl[b].fp32 = ctx->mixed ? 1 : 0;
   }
-  // restore the caller's activity mask
-  HIPCHK(hipMemcpyAsync(ctx->d_active, user_mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
-  for (int b = 0; b < B; b++) out[b] = res[b];
+  for (;;) {
+    bool busy = false;
+    for (int b = 0; b < B; b++)
+      if (ctx->h_active[b] && sl[b].phase >= EFFPOT_SLOT_START && sl[b].phase <= EFFPOT_SLOT_PC) busy = true;
+    if (!busy) break;
+    RC(effpot_tick(ctx, opt, sl.data()));
+  }
+  int rc = EFFPOT_OK;
+  for (int b = 0; b < B; b++) {
+    std::memset(&out[b], 0, sizeof(effpot_result));
+    out[b].e = sl[b].e;
+    out[b].steps = sl[b].steps;
+    out[b].converged = sl[b].converged;
+    out[b].rel_de = sl[b].rel_de;
+    if (sl[b].phase == EFFPOT_SLOT_FAILED) rc = ctx->fail(EFFPOT_ERR_NONFINITE, "non-finite density or energy (slot " + std::to_string(b) + ")");
+  }
   return rc;
 }
-

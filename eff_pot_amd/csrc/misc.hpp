@@ -190,10 +190,10 @@ __global__ void k_renorm(cx<T>* psi, T* den, const double* scale, const int* act
 }
 
 // ---- deterministic reduction: partial[s][nblk][K] -> out[s][K] -------------
-// mode 1: also scale[s] = sqrt(n4 / (out[s][1] * dxyz)).
+// mode 1: also scale[s] = sqrt(n4[s] / (out[s][1] * dxyz)).
 template <int K>
 __global__ __launch_bounds__(kThreads) void k_reduce(const double* partial, int nblk, double* out,
-                                                      const int* active, int mode, double n4, double dxyz,
+                                                      const int* active, int mode, const double* n4, double dxyz,
                                                       double* scale) {
   __shared__ double red[4 * K];
   const int s = blockIdx.x;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* partial, int 
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[s * K + k] = v[k];
-    if (mode == 1) scale[s] = sqrt(n4 / (v[1] * dxyz));
+    if (mode == 1) scale[s] = sqrt(n4[s] / (v[1] * dxyz));
   }
 }
 

@@ -126,6 +126,25 @@ typedef struct effpot_result {
   double err_he;     /* last predictor-corrector error estimate errHe      */
 } effpot_result;
 
+/* Per-slot progress of the continuous-batching engine (effpot_tick). */
+#define EFFPOT_SLOT_IDLE 0    /* no solve in this slot                           */
+#define EFFPOT_SLOT_START 1   /* state uploaded: next tick runs poten + RK start  */
+#define EFFPOT_SLOT_RK 2      /* in the Runge-Kutta-Gill starter steps            */
+#define EFFPOT_SLOT_PC 3      /* in the predictor-corrector phase                 */
+#define EFFPOT_SLOT_DONE 4    /* converged (or out of steps): results valid        */
+#define EFFPOT_SLOT_FAILED 5  /* non-finite density/energy                        */
+typedef struct effpot_slot {
+  int32_t phase;     /* EFFPOT_SLOT_*                                         */
+  int32_t fp32;      /* 1 while in the FP32 phase of a MIXED context          */
+  int32_t nrk;       /* RK steps done in the current phase                    */
+  int32_t steps;     /* steps taken by this solve (all phases)                */
+  int32_t since;     /* PC steps since the last energy evaluation             */
+  int32_t converged; /* 1 if the FP64 tolerance was met                       */
+  double eprev;      /* energy at the last evaluation                         */
+  double rel_de;     /* last |dE/E| per step                                  */
+  effpot_energies e; /* energies at the last evaluation                       */
+} effpot_slot;
+
 typedef struct effpot_ctx effpot_ctx;
 
 /* ---- context ----------------------------------------------------------- */
@@ -154,6 +173,10 @@ int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out);
 int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R,
                       double afermi, double noise, uint64_t seed,
                       const double centre[3]);
+
+/* Atoms slot `slot` is normalised to (namelist n4, per slot; default
+ * params.n4).  Set it before effpot_init_fermi for that slot. */
+int effpot_set_natoms(effpot_ctx* ctx, int slot, double n4);
 
 /* ---- impurity field (potenimpini.f90 / updatepoten+update+potenimp) ---- */
 int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp);
@@ -187,6 +210,16 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
  * poten() kernel sequence.  The launches overwrite the integrator state:
  * call it after the solves that matter. */
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
+
+/* Continuous batching: advance every slot whose phase is START, RK or PC by
+ * one step of its own phase (a START slot gets poten + its first RK step;
+ * a MIXED slot switches to FP64 polish when its FP32 |dE/E| per step drops
+ * below tol_fp32; a slot is DONE when its FP64 |dE/E| per step < tol or it
+ * has taken max_steps).  `slots[batch]` is host memory owned by the caller;
+ * set a slot to START after effpot_init_fermi / effpot_upload_psi and
+ * effpot_set_impurity.  No reference counterpart (the reference has no
+ * driver loop); the solve of one slot is the same whatever the other slots do. */
+int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots);
 
 /* Synchronise the context's stream. */
 int effpot_sync(effpot_ctx* ctx);

@@ -1,0 +1,197 @@
+"""GPU relaxations against committed oracle fixtures, mixed precision against
+FP64, and continuous batching (a request solved alone and in a mix).
+
+Tolerances: FP64 trajectories 1e-8 relative after 2000 steps; converged
+E_imp within 1e-3 K of the FP64 oracle for both the FP64
+and the mixed FP32/FP64 path; a slot's result is independent of the others
+(1e-12 relative psi, 1e-9 K energies)."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import eff_pot_amd as ep
+from eff_pot_amd.abi import make_params, make_morse, CORE_OTC, PREC_MIXED, SolveOpts
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def load(name):
+    return json.load(open(os.path.join(GOLD, name)))
+
+
+def test_cfg0_trajectory_matches_oracle(gpu_available):
+    g = load("cfg0_oracle.json")
+    c = g["config"]
+    print("[cfg0] 64^3 N=300 2000 steps", flush=True)
+    p = make_params(c["n"], xmax=c["xmax"], n4=c["n4"], core=CORE_OTC, h_core=c["h_core"])
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(c["rho0"], 2.22 * c["n4"] ** (1 / 3), c["afermi"], c["noise"], c["seed"])
+        ctx.set_impurity(make_morse())
+        r = ctx.solve(SolveOpts(dt=g["dt"], tol=0.0, n_rk=3, pener=c["pener"], max_steps=c["steps"],
+                                polish_steps=0, tol_fp32=0.0, check_finite=1))[0]
+    assert r.steps == g["steps"]
+    for k in ("etot", "eimpu", "ekin4", "elj4", "ealphas", "ecor4", "natoms", "mu"):
+        ref = g["energies"][k]
+        assert abs(getattr(r.e, k) - ref) <= 1e-8 * abs(ref), (k, getattr(r.e, k), ref)
+
+
+def _conv32(precision, tol_fp32=1e-7):
+    g = load("conv32_oracle.json")
+    c = g["config"]
+    p = make_params(c["n"], xmax=c["xmax"], n4=c["n4"], core=CORE_OTC, h_core=c["h_core"], precision=precision)
+    ctx = ep.Context(p)
+    ctx.init_fermi(c["rho0"], 2.22 * c["n4"] ** (1 / 3), c["afermi"], c["noise"], c["seed"])
+    ctx.set_impurity(make_morse(rimp=tuple(c["rimp"])))
+    r = ctx.solve(SolveOpts(dt=g["dt"], tol=c["tol"], n_rk=3, pener=c["pener"], max_steps=c["max_steps"],
+                            polish_steps=c["max_steps"], tol_fp32=tol_fp32, check_finite=1))[0]
+    return g, ctx, r
+
+
+def test_converged_fp64_vs_oracle(gpu_available):
+    print("[conv32] fp64", flush=True)
+    g, ctx, r = _conv32(0)
+    ctx.close()
+    assert r.converged
+    assert abs(r.e.eimpu - g["energies"]["eimpu"]) <= 1e-3
+    assert abs(r.e.etot - g["energies"]["etot"]) <= 1e-3
+    assert abs(r.steps - g["steps"]) <= 2 * g["config"]["pener"]
+
+
+def test_converged_mixed_vs_oracle_and_fp64(gpu_available):
+    print("[conv32] mixed", flush=True)
+    g, c64, r64 = _conv32(0)
+    d64 = c64.den()
+    c64.close()
+    g, c32, r32 = _conv32(PREC_MIXED)
+    d32 = c32.den()
+    c32.close()
+    assert r32.converged
+    assert abs(r32.e.eimpu - g["energies"]["eimpu"]) <= 1e-3, (r32.e.eimpu, g["energies"]["eimpu"])
+    assert abs(r32.e.etot - g["energies"]["etot"]) <= 1e-3
+    l2 = np.linalg.norm(d32 - d64) / np.linalg.norm(d64)
+    print(f"mixed: steps={r32.steps} (fp64 {r64.steps}) dE_imp={r32.e.eimpu - r64.e.eimpu:.2e} K L2(rho)={l2:.2e}")
+    assert l2 < 1e-4
+
+
+def _req(ctx, slot, z, seed):
+    ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, seed, slot=slot)
+    ctx.set_impurity(make_morse(rimp=(0.0, 0.0, z), D=25.0 + 5 * slot), slot=slot)
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_same_result_alone_and_in_the_mix(precision, gpu_available):
+    print(f"[mix] precision={precision}", flush=True)
+    opts = SolveOpts(dt=1.9e-4, tol=1e-9, n_rk=3, pener=50, max_steps=6000, polish_steps=6000, tol_fp32=1e-7,
+                     check_finite=1)
+    zs, seeds = [0.5, 1.5, -2.0], [1, 2, 3]
+    alone = []
+    for k in range(3):
+        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision, batch=3)
+        with ep.Context(p) as ctx:
+            for s in range(3):
+                if s != k:
+                    ctx.set_active(s, False)
+            _req(ctx, k, zs[k], seeds[k])
+            slots = ctx.new_slots()
+            slots[k].phase = ep.SLOT_START
+            slots[k].fp32 = 1 if precision else 0
+            while slots[k].phase not in (ep.SLOT_DONE, ep.SLOT_FAILED):
+                ctx.tick(opts, slots)
+            alone.append((slots[k].e.eimpu, slots[k].e.etot, slots[k].steps, ctx.psi(k)))
+    # the mix: requests admitted at different times into one context
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision, batch=3)
+    with ep.Context(p) as ctx:
+        slots = ctx.new_slots()
+        admit = {0: 0, 1: 37, 2: 180}
+        t = 0
+        done = set()
+        psis = {}
+        while len(done) < 3:
+            for k, at in admit.items():
+                if at == t:
+                    _req(ctx, k, zs[k], seeds[k])
+                    slots[k].phase = ep.SLOT_START
+                    slots[k].fp32 = 1 if precision else 0
+            ctx.tick(opts, slots)
+            for k in range(3):
+                if k not in done and slots[k].phase in (ep.SLOT_DONE, ep.SLOT_FAILED):
+                    done.add(k)
+                    psis[k] = ctx.psi(k)
+            t += 1
+        for k in range(3):
+            ea, et, st, pa = alone[k]
+            assert slots[k].phase == ep.SLOT_DONE and slots[k].converged
+            assert slots[k].steps == st
+            assert abs(slots[k].e.eimpu - ea) <= 1e-9 and abs(slots[k].e.etot - et) <= 1e-9
+            assert np.linalg.norm(psis[k] - pa) <= 1e-12 * np.linalg.norm(pa)
+
+
+def test_nonfinite_density_is_reported(gpu_available):
+    p = make_params(32, xmax=12.0, n4=100.0)
+    with ep.Context(p) as ctx:
+        psi = np.full(ctx.shape, 0.1, dtype=complex)
+        psi[3, 4, 5] = np.nan
+        ctx.upload_psi(psi)
+        ctx.set_impurity(make_morse())
+        with pytest.raises(ep.EffpotError) as ei:
+            ctx.solve(SolveOpts(dt=1.9e-4, tol=1e-9, n_rk=3, pener=10, max_steps=100, polish_steps=0,
+                                tol_fp32=0.0, check_finite=1))
+        assert ei.value.code == ep.abi.ERR_NONFINITE
+
+
+def test_fortran_driver_labels(gpu_available, tmp_path):
+    exe = os.path.join(ROOT, "driver", "effpot_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    nml = ("&input\n nx=32, ny=32, nz=32, xmax=12.d0, ymax=12.d0, zmax=12.d0, n4=100, mode=0,\n"
+           " core4='OTC', h_core=2.377d0, selec_pi='Morse', zimp=1.5d0, niter=0, seed=99\n/\n")
+    out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert out.returncode == 0, out.stderr
+    txt = out.stdout
+    eint = float(re.search(r"Interaction energy \(X\*-He\) \.*: +(\S+)", txt).group(1))
+    rcm = float(re.search(r"X-HE CoM distance:\s+(\S+)", txt).group(1))
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 99)
+        ctx.set_impurity(make_morse(rimp=(0.0, 0.0, 1.5)))
+        ctx.poten()
+        e = ctx.energy()[0]
+    assert abs(eint - e["eHeX"]) < 1e-5
+    assert abs(rcm - np.sqrt(e["xcm"] ** 2 + e["ycm"] ** 2 + (1.5 - e["zcm"]) ** 2)) < 1e-5
+    # awk field positions used by get_impurity_energy.sh
+    line = [l for l in txt.splitlines() if "Interaction energy (X*" in l][0]
+    assert float(line.split()[4]) == eint
+
+
+def test_graph_replay_matches_eager(gpu_available, monkeypatch):
+    """Every replay of the captured predictor-corrector step, with eager energy
+    work and synchronisation in between, equals the eager launch sequence."""
+    print("[graph] replay vs eager", flush=True)
+    opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=1, max_steps=60, polish_steps=0, tol_fp32=0.0,
+                     check_finite=0)
+    traj = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("EFFPOT_GRAPHS", graphs)
+        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=2)
+        with ep.Context(p) as ctx:
+            for s in range(2):
+                _req(ctx, s, [0.5, -1.0][s], 11 + s)
+            slots = ctx.new_slots()
+            for s in range(2):
+                slots[s].phase = ep.SLOT_START
+            es = []
+            while any(slots[s].phase not in (ep.SLOT_DONE, ep.SLOT_FAILED) for s in range(2)):
+                ctx.tick(opts, slots)
+                es.append([(slots[s].e.etot, slots[s].e.eimpu) for s in range(2)])
+            traj[graphs] = (es, [ctx.psi(s) for s in range(2)])
+    assert len(traj["1"][0]) == len(traj["0"][0])
+    for a, b in zip(traj["1"][0], traj["0"][0]):
+        assert a == b
+    for a, b in zip(traj["1"][1], traj["0"][1]):
+        assert np.array_equal(a, b)
