@@ -65,6 +65,8 @@ def lib():
         L.effpot_sync.argtypes = [vp]
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
+        L.effpot_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -197,6 +199,14 @@ class Context:
     def tick(self, opts: SolveOpts, slots):
         """Advance every START/RK/PC slot by one step of its own phase."""
         self._chk(lib().effpot_tick(self.h, C.byref(opts), slots))
+
+    def autotune(self, reps=10):
+        tx, zc = C.c_int(), C.c_int()
+        self._chk(lib().effpot_autotune(self.h, reps, C.byref(tx), C.byref(zc)))
+        return tx.value, zc.value
+
+    def set_tiles(self, tx, zc):
+        self._chk(lib().effpot_set_tiles(self.h, tx, zc))
 
     def kernel_time(self, which, reps=20):
         ms = C.c_double()

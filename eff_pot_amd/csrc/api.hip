@@ -672,7 +672,17 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_active, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_mask_tmp, ctx->B)) != EFFPOT_OK) return bail(rc);
-  ctx->nblk_max = nblk(ctx);
+  {  // partial sums sized for the largest block count any tile choice gives
+    int tx0 = ctx->tx, zc0 = ctx->zc;
+    ctx->nblk_max = 0;
+    for (int tx : {64, 32, 16, 8})
+      for (int zc : {4, 8, 16, 32}) {
+        if (p->nx % tx) continue;
+        ctx->tx = tx; ctx->zc = std::min(zc, p->nz);
+        ctx->nblk_max = std::max(ctx->nblk_max, nblk(ctx));
+      }
+    ctx->tx = tx0; ctx->zc = zc0;
+  }
   if ((rc = dalloc(ctx, &ctx->d_partial, (size_t)ctx->B * ctx->nblk_max * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_red, (size_t)ctx->B * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_scale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -1179,6 +1189,42 @@ template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int
   *ms = (double)t / reps;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return EFFPOT_OK;
+}
+
+int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
+  RC(check_ctx(ctx));
+  hipSetDevice(ctx->dev);
+  int best_tx = ctx->tx, best_zc = ctx->zc;
+  double best = 1e30;
+  for (int tx : {64, 32, 16, 8}) {
+    if (ctx->nx % tx) continue;
+    for (int zc : {4, 8, 16, 32}) {
+      if (zc > ctx->nz) continue;
+      ctx->tx = tx; ctx->zc = zc;
+      double t0 = 0, t1 = 0;
+      if (ctx->mixed) { RC(time_kernel_t(ctx, ctx->s32, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s32, 1, reps, &t1)); }
+      else { RC(time_kernel_t(ctx, ctx->s64, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s64, 1, reps, &t1)); }
+      if (t0 + t1 < best) { best = t0 + t1; best_tx = tx; best_zc = zc; }
+    }
+  }
+  ctx->tx = best_tx; ctx->zc = best_zc;
+  for (int k = 0; k < 2; k++)  // captured graphs hold the old launch shapes
+    if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
+  if (tx_out) *tx_out = best_tx;
+  if (zc_out) *zc_out = best_zc;
+  return EFFPOT_OK;
+}
+
+int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc) {
+  RC(check_ctx(ctx));
+  if ((tx != 64 && tx != 32 && tx != 16 && tx != 8) || ctx->nx % tx || zc < 1 || zc > ctx->nz)
+    return ctx->fail(EFFPOT_ERR_ARG, "bad tile");
+  int tx0 = ctx->tx, zc0 = ctx->zc;
+  ctx->tx = tx; ctx->zc = zc;
+  if (nblk(ctx) > ctx->nblk_max) { ctx->tx = tx0; ctx->zc = zc0; return ctx->fail(EFFPOT_ERR_ARG, "tile exceeds partial buffer"); }
+  for (int k = 0; k < 2; k++)
+    if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
   return EFFPOT_OK;
 }
 

@@ -221,6 +221,14 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
  * driver loop); the solve of one slot is the same whatever the other slots do. */
 int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots);
 
+/* Autotuner (no reference counterpart): times the H psi + predictor and
+ * corrector kernels for every stencil tile (x width 64/32/16/8 dividing nx,
+ * z chunk 4/8/16/32) on the live state and keeps the fastest.  Overwrites the
+ * integrator state like effpot_kernel_time: call it before uploading states. */
+int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
+/* Sets the stencil tile explicitly (x width, z chunk). */
+int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc);
+
 /* Synchronise the context's stream. */
 int effpot_sync(effpot_ctx* ctx);
 

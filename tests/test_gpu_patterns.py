@@ -1,0 +1,67 @@
+"""Batched scan and continuous-batching server on the GPU (small grids).
+
+- scan: every slot of a batched context equals the same position solved
+  alone; the warm start (relaxed pure droplet) reaches the same E_imp within
+  1e-3 K in fewer steps;
+- server: requests of different droplet sizes arriving over time, routed to
+  grid classes; each result equals the request solved alone (1e-9 K) and the
+  latencies are recorded."""
+import numpy as np
+import pytest
+
+import eff_pot_amd as ep
+from eff_pot_amd import scan, serve
+from eff_pot_amd.abi import PREC_MIXED, SolveOpts, make_params, make_morse, CORE_OTC
+
+pytestmark = pytest.mark.gpu
+
+
+def test_scan_slots_equal_single_solves(gpu_available):
+    print("[scan] 32^3 x 4 positions", flush=True)
+    res, meta = scan.run_scan(n=32, spacing=0.75, natoms=100.0, npos=4, precision=0, tol=1e-8, max_steps=30000)
+    zs = [r["z0"] for r in res]
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    for k in (0, 3):
+        with ep.Context(p) as c:
+            c.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 20261203)
+            c.set_impurity(make_morse(rimp=(0.0, 0.0, zs[k])))
+            r = c.solve(SolveOpts(dt=meta["dt"], tol=1e-8, n_rk=3, pener=50, max_steps=30000,
+                                  polish_steps=30000, tol_fp32=1e-7, check_finite=0))[0]
+        assert r.steps == res[k]["steps"]
+        assert abs(r.e.eimpu - res[k]["eimp"]) < 1e-9
+
+
+def test_scan_warm_start(gpu_available):
+    print("[scan] warm start", flush=True)
+    cold, _ = scan.run_scan(n=32, spacing=0.75, natoms=100.0, npos=3, precision=0, tol=1e-8, max_steps=30000)
+    warm, _ = scan.run_scan(n=32, spacing=0.75, natoms=100.0, npos=3, precision=0, tol=1e-8, max_steps=30000,
+                            warm=True)
+    for c, w in zip(cold, warm):
+        assert c["converged"] and w["converged"]
+        assert abs(c["eimp"] - w["eimp"]) < 1e-3, (c, w)
+
+
+def test_server_results_match_alone(gpu_available):
+    print("[serve] 6 requests", flush=True)
+    reqs = [serve.Request(rid=i, natoms=n, D=d, re=4.0, a=1.0, pos=(0.0, 0.0, z), arrival=t, seed=50 + i)
+            for i, (n, d, z, t) in enumerate([(100, 30, 1.0, 0.0), (150, 20, -2.0, 0.05), (120, 45, 0.0, 0.1),
+                                              (110, 30, 2.0, 0.15), (180, 15, 0.5, 0.2), (100, 60, -1.0, 0.3)])]
+    srv = serve.Server({64: 3}, tol=1e-8, tol_fp32=1e-7, max_steps=30000, precision=PREC_MIXED)
+    try:
+        srv.run(reqs)
+    finally:
+        srv.close()
+    for r in reqs:
+        assert r.grid == 64 and r.converged and not r.failed
+        assert r.finish > r.start >= r.arrival
+    # the same requests alone, one context each
+    for r in reqs[:3]:
+        p = serve.class_params(64, 1, PREC_MIXED)
+        with ep.Context(p) as c:
+            c.set_natoms(r.natoms)
+            c.init_fermi(0.0218, serve.radius(r.natoms), 0.6, 1e-6, r.seed)
+            c.set_impurity(make_morse(rimp=r.pos, D=r.D, re=r.re, a=r.a))
+            x = c.solve(SolveOpts(dt=ep.default_dt(p), tol=1e-8, n_rk=3, pener=50, max_steps=30000,
+                                  polish_steps=30000, tol_fp32=1e-7, check_finite=0))[0]
+        assert x.steps == r.steps
+        assert abs(x.e.eimpu - r.eimp) < 1e-9 and abs(x.e.etot - r.etot) < 1e-9
