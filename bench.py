@@ -36,9 +36,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--max-steps", type=int, default=60000)
+    ap.add_argument("--max-steps", type=int, default=100000)
+    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--tol-fp32", type=float, default=1e-8)
     ap.add_argument("--cpu-steps", type=int, default=4, help="PC steps timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -83,8 +85,8 @@ def main():
     B = a.batch if a.mode == "mixed" else 1
     p, R = workload(a.n, a.mode, B)
     dt = ep.default_dt(p)
-    opts = ep.SolveOpts(dt=dt, tol=1e-9, n_rk=3, pener=50, max_steps=a.max_steps, polish_steps=a.max_steps,
-                        tol_fp32=2e-8, check_finite=0)
+    opts = ep.SolveOpts(dt=dt, tol=a.tol, n_rk=3, pener=50, max_steps=a.max_steps, polish_steps=a.max_steps,
+                        tol_fp32=a.tol_fp32, check_finite=0)
     zs = positions(R, B)
     ctx = ep.Context(p)
     for _ in range(a.warmup):
