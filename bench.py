@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4, help="PC steps timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
 
 
@@ -54,7 +55,7 @@ def workload(n, mode, batch):
     xmax = h * n / 2
     prec = ep.PREC_MIXED if mode == "mixed" else ep.PREC_FP64
     p = ep.make_params(n, xmax=xmax, n4=N4, core=ep.CORE_OTC, h_core=2.377, precision=prec,
-                       batch=batch if mode == "mixed" else 1)
+                       batch=batch if mode == "mixed" else 1, psi_complex=0)
     R = 2.22 * N4 ** (1.0 / 3.0)
     return p, R
 
@@ -89,6 +90,7 @@ def main():
                         tol_fp32=a.tol_fp32, check_finite=0)
     zs = positions(R, B)
     ctx = ep.Context(p)
+    tile = ctx.autotune(reps=5) if not a.no_tune else None   # setup, before any state is loaded
     for _ in range(a.warmup):
         solve_batch(ctx, p, R, zs, opts)
     ctx.sync()
@@ -119,7 +121,7 @@ def main():
         "data": "synthetic (seeded Fermi droplet + 1e-6 noise)",
         "config": {"workload": f"He4 N=1000 droplet, {a.n}^3 grid h={2 * p.xmax / a.n:.3f} A, OTC h_core=2.377, "
                                f"Morse D=30 re=4 a=1 at z0~0.5R, to |dE/E|<1e-9/step",
-                   "mode": a.mode, "batch": B, "dt": dt, "converged": conv,
+                   "mode": a.mode, "batch": B, "dt": dt, "converged": conv, "stencil_tile": tile,
                    "steps_per_solve": steps_per_solve, "E_imp_K": eimp,
                    "E_tot_K": [r.e.etot for r in results]},
     }
@@ -130,7 +132,9 @@ def main():
         ms = ctx.kernel_time(1, reps=20)
         bytes_alg = 19 * s * Npts * B     # 14 reads + 5 writes of s-byte reals per point and slot
         ach = bytes_alg / (ms * 1e-3) / 1e9
-        line["roofline"] = {"bound": "hbm", "kernel": "k_step<T,64,1> (H psi + Hamming corrector)",
+        tx = tile[0] if tile else 64
+        line["roofline"] = {"bound": "hbm", "kernel": f"k_step<{'float' if s == 4 else 'double'},{tx},1> "
+                                                      f"(H psi + Hamming corrector)",
                             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": ach / HBM_PEAK_GBS, "traffic": None, "ms_per_launch": ms,
                             "bytes_per_launch": bytes_alg}

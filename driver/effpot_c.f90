@@ -11,7 +11,7 @@ module effpot_c
   integer(c_int), parameter :: EFFPOT_POT_NULL = 0, EFFPOT_POT_MORSE = 1
 
   type, bind(c) :: effpot_params
-    integer(c_int32_t) :: nx, ny, nz, core, npd, icon, precision, batch
+    integer(c_int32_t) :: nx, ny, nz, core, npd, icon, precision, batch, psi_complex, reserved_
     real(c_double) :: xmax, ymax, zmax, n4, h_core, eps4, sigma4, cp4, cpp4
     real(c_double) :: alphas, l, den0s, h2o2m4, cpen, beta, den_m
   end type

@@ -60,6 +60,7 @@ program effpot_driver
   end if
 
   p%nx = nx; p%ny = ny; p%nz = nz; p%npd = npd; p%icon = icon; p%precision = precision; p%batch = 1
+  p%psi_complex = 1; p%reserved_ = 0   ! snapshots (mode 3) may carry a phase
   select case (core4)
   case ('OT ', 'OT')
     p%core = EFFPOT_CORE_OT

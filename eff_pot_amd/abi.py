@@ -21,6 +21,7 @@ class Params(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
                 ("core", C.c_int32), ("npd", C.c_int32), ("icon", C.c_int32),
                 ("precision", C.c_int32), ("batch", C.c_int32),
+                ("psi_complex", C.c_int32), ("reserved_", C.c_int32),
                 ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
                 ("n4", C.c_double), ("h_core", C.c_double),
                 ("eps4", C.c_double), ("sigma4", C.c_double),
@@ -75,11 +76,11 @@ H4OP = 2.359665   # modules.f90:191
 
 
 def make_params(n, xmax, n4=300.0, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO,
-                precision=PREC_FP64, batch=1, ny=None, nz=None, ymax=None, zmax=None, **kw):
+                precision=PREC_FP64, batch=1, ny=None, nz=None, ymax=None, zmax=None, psi_complex=1, **kw):
     d = dict(OT_DEFAULTS)
     d.update(kw)
     return Params(nx=n, ny=ny or n, nz=nz or n, core=core, npd=13, icon=icon,
-                  precision=precision, batch=batch, xmax=xmax, ymax=ymax or xmax,
+                  precision=precision, batch=batch, psi_complex=psi_complex, xmax=xmax, ymax=ymax or xmax,
                   zmax=zmax or xmax, n4=n4, h_core=h_core, **d)
 
 

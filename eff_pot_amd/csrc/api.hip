@@ -66,49 +66,50 @@ template <typename T> struct Ring {
 };
 
 // ---- kernels that need the ring --------------------------------------------
-template <typename T, int TX, int MODE>  // MODE 0 predict, 1 correct, 2 rk stage
-__global__ __launch_bounds__(kThreads) void k_step(StepArgs<T> a, Ring<T> rg) {
+template <typename T, typename V, int TX, int MODE>  // MODE 0 predict, 1 correct, 2 rk stage
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs<T, V> a, Ring<T> rg) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
   // resolve the roles of this slot into the argument block, then run the
   // shared kernel body (the StepArgs pointers are slot-0 based; offsets are
   // re-applied by the body, so pass slot-relative pointers minus the offset).
   const long so = (long)s * a.slot;
-  StepArgs<T> b = a;
-  b.cur = rg.p(s, R_CUR) - so;
-  b.out = (MODE == 1 ? rg.p(s, R_SPARE) : rg.p(s, R_OUT)) - so;
-  b.o1 = rg.p(s, R_O1) - so; b.o2 = rg.p(s, R_O2) - so; b.o3 = rg.p(s, R_O3) - so;
-  b.h1 = rg.h(s, R_H1) - so; b.h2 = rg.h(s, R_H2) - so;
-  if constexpr (MODE == 0) k_pc_predict_body<T, TX>(b);
-  else if constexpr (MODE == 1) k_pc_correct_body<T, TX>(b);
-  else k_rk_stage_body<T, TX>(b);
+  auto P = [&](int r) { return reinterpret_cast<V*>(rg.p(s, r)) - so; };
+  auto H = [&](int r) { return reinterpret_cast<V*>(rg.h(s, r)) - so; };
+  StepArgs<T, V> b = a;
+  b.cur = P(R_CUR);
+  b.out = MODE == 1 ? P(R_SPARE) : P(R_OUT);
+  b.o1 = P(R_O1); b.o2 = P(R_O2); b.o3 = P(R_O3);
+  b.h1 = H(R_H1); b.h2 = H(R_H2);
+  if constexpr (MODE == 0) k_pc_predict_body<T, V, TX>(b);
+  else if constexpr (MODE == 1) k_pc_correct_body<T, V, TX>(b);
+  else k_rk_stage_body<T, V, TX>(b);
 }
 
-template <typename T>
+template <typename T, typename V>
 __global__ void k_renorm_ring(Ring<T> rg, T* den, const double* scale, const int* active, long N) {
   const int s = blockIdx.y;
   if (!active[s]) return;
   const T f = (T)scale[s];
-  cx<T>* psi = rg.p(s, R_CUR);
+  V* psi = reinterpret_cast<V*>(rg.p(s, R_CUR));
   T* d = den + (long)s * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    cx<T> p = f * psi[i];
+    V p = f * psi[i];
     psi[i] = p;
-    d[i] = p.x * p.x + p.y * p.y;
+    d[i] = norm2(p);
   }
 }
 
 // partial sums of |psi_cur|^2 in the same [slot][block][2] layout as the steps
-template <typename T>
+template <typename T, typename V>
 __global__ __launch_bounds__(kThreads) void k_norm(Ring<T> rg, const int* active, long N, double* partial) {
   __shared__ double red[8];
   const int s = blockIdx.y;
   if (!active[s]) return;
-  const cx<T>* psi = rg.p(s, R_CUR);
+  const V* psi = reinterpret_cast<const V*>(rg.p(s, R_CUR));
   double v[2] = {0.0, 0.0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    cx<T> p = psi[i];
-    v[1] += (double)(p.x * p.x + p.y * p.y);
+    v[1] += (double)norm2(psi[i]);
   }
   block_sum<2>(v, red);
   if (threadIdx.x == 0) {
@@ -117,15 +118,14 @@ __global__ __launch_bounds__(kThreads) void k_norm(Ring<T> rg, const int* active
   }
 }
 
-template <typename T>
+template <typename T, typename V>
 __global__ void k_den_from_cur(Ring<T> rg, T* den, const int* active, long N) {
   const int s = blockIdx.y;
   if (!active[s]) return;
-  const cx<T>* psi = rg.p(s, R_CUR);
+  const V* psi = reinterpret_cast<const V*>(rg.p(s, R_CUR));
   T* d = den + (long)s * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    cx<T> p = psi[i];
-    d[i] = p.x * p.x + p.y * p.y;
+    d[i] = norm2(psi[i]);
   }
 }
 
@@ -169,6 +169,7 @@ struct effpot_ctx {
   State<double> s64;
   State<float> s32;
   bool mixed = false;
+  bool cplx = true;  // psi stored as complex (else real: N values per buffer)
   double* imptab = nullptr;  // [B][npot+2]
   int* d_active = nullptr;   // current mask
   int* d_ones = nullptr;
@@ -187,6 +188,7 @@ struct effpot_ctx {
   double* h_pin = nullptr;    // pinned host buffer
   int tx = 64, zc = 16;
   bool graphs = true;               // replay captured PC steps (EFFPOT_GRAPHS=0 disables)
+  bool tune_log = false;            // EFFPOT_TUNE_LOG=1 prints the autotuner's timings
   hipGraphExec_t pc_exec[2] = {nullptr, nullptr};  // [0] FP64, [1] FP32
   double pc_dt[2] = {0.0, 0.0};
   std::vector<void*> allocs;
@@ -325,9 +327,14 @@ template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, 
   LAUNCHCHK();
   return EFFPOT_OK;
 }
-template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* in, T* const* out, int nf, T scale, const int* act) {
+template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* in, T* const* out, int nf, T scale,
+                                    const int* act, const T* aff0 = nullptr, const T* aff1 = nullptr) {
   XArgs<T> a{};
-  for (int f = 0; f < nf; f++) { a.r[f] = out[f]; a.c[f] = in[f]; }
+  for (int f = 0; f < nf; f++) {
+    a.r[f] = out[f]; a.c[f] = in[f];
+    a.aff0[f] = aff0 ? aff0[f] : T(0);
+    a.aff1[f] = aff1 ? aff1[f] : T(1);
+  }
   a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = scale; a.twN = s.tw[0]; a.active = act;
   const int C = kThreads / n1_of(ctx->nx);
   dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
@@ -403,23 +410,38 @@ template <typename T> int ring_op(effpot_ctx* ctx, State<T>& s, int op, const in
   return EFFPOT_OK;
 }
 
-template <typename T> StepArgs<T> step_args(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
-  StepArgs<T> a{};
-  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dt = (T)dt;
-  a.active = act; a.slot = ctx->N; a.pc = s.pc; a.q = s.q; a.pot4 = s.pot4; a.uimp = s.uimp; a.den = s.den;
-  a.partial = ctx->d_partial; a.nblk = nblk(ctx);
-  return a;
-}
+// Runge-Kutta-Gill stage coefficients (unused by the PC kernels).
+struct StageCoef { double arun = 0, brun = 0, crun = 0; int first = 0; };
 
-template <typename T, int MODE> int launch_step(effpot_ctx* ctx, State<T>& s, StepArgs<T>& a) {
+template <typename T, typename V, int MODE>
+int launch_step_v(effpot_ctx* ctx, State<T>& s, double dt, const int* act, StageCoef c) {
+  StepArgs<T, V> a{};
+  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dt = (T)dt;
+  a.active = act; a.slot = ctx->N;
+  a.pc = reinterpret_cast<V*>(s.pc); a.q = reinterpret_cast<V*>(s.q); a.pcq_slot = ctx->N;
+  a.pot4 = s.pot4; a.uimp = s.uimp; a.den = s.den;
+  a.partial = ctx->d_partial; a.nblk = nblk(ctx);
+  a.arun = (T)c.arun; a.brun = (T)c.brun; a.crun = (T)c.crun; a.first_stage = c.first;
   Ring<T> rg = ring_of(ctx, s);
   dim3 g = sgrid(ctx);
-#define M_(TXX) hipLaunchKernelGGL((k_step<T, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
+#define M_(TXX) hipLaunchKernelGGL((k_step<T, V, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
   TX_SWITCH(ctx->tx, M_)
 #undef M_
   LAUNCHCHK();
   return EFFPOT_OK;
 }
+template <typename T, int MODE>
+int launch_step(effpot_ctx* ctx, State<T>& s, double dt, const int* act, StageCoef c = StageCoef()) {
+  if (ctx->cplx) return launch_step_v<T, cx<T>, MODE>(ctx, s, dt, act, c);
+  return launch_step_v<T, T, MODE>(ctx, s, dt, act, c);
+}
+// Launch a ring kernel templated on <T, V> with V chosen by the context's psi type.
+#define PSI_LAUNCH(KERNEL, T_, GRID, ...)                                                          \
+  do {                                                                                               \
+    if (ctx->cplx) hipLaunchKernelGGL((KERNEL<T_, cx<T_>>), GRID, dim3(256), 0, ctx->st, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<T_, T_>), GRID, dim3(256), 0, ctx->st, __VA_ARGS__);              \
+    LAUNCHCHK();                                                                                     \
+  } while (0)
 
 template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act) {
   hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
@@ -448,8 +470,12 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
     RC(fft_zconv(ctx, s, ca, 1, act));
     cx<T>* o[3] = {s.spec[1], s.spec[2], s.spec[3]};
     RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
+    // the alpha_s convolution is stored as falfs = 1 - denalf/den0s (term_alfa.f90:157-159),
+    // the only form its consumers use
     T* r[3] = {s.dencg, s.delj4, s.denalf};
-    RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act));
+    const T a0[3] = {T(0), T(0), T(1)};
+    const T a1[3] = {T(1), T(1), T(-1.0 / ctx->p.den0s)};
+    RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
   }
   // derden + FFT sources
   {
@@ -523,17 +549,14 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
 
 // ---- integrators ----------------------------------------------------------------
 template <typename T> int renorm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
-  hipLaunchKernelGGL((k_renorm_ring<T>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, ring_of(ctx, s), s.den,
-                     ctx->d_scale, act, ctx->N);
-  LAUNCHCHK();
+  PSI_LAUNCH(k_renorm_ring, T, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), s.den, (const double*)ctx->d_scale, act,
+             ctx->N);
   return EFFPOT_OK;
 }
 
 // scale[s] = sqrt(n4 / (sum |psi|^2 dxyz)) for the live psi
 template <typename T> int norm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
-  hipLaunchKernelGGL((k_norm<T>), dim3(nblk(ctx), ctx->B), dim3(kThreads), 0, ctx->st, ring_of(ctx, s), act, ctx->N,
-                     ctx->d_partial);
-  LAUNCHCHK();
+  PSI_LAUNCH(k_norm, T, dim3(nblk(ctx), ctx->B), ring_of(ctx, s), act, ctx->N, ctx->d_partial);
   return reduce<2>(ctx, 1, act);
 }
 
@@ -544,10 +567,10 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, double dt, con
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
   const double crun[4] = {0.5, 1.0 - 1.0 / r2, 1.0 + 1.0 / r2, 0.5};
   for (int j = 0; j < 4; j++) {
-    StepArgs<T> a = step_args(ctx, s, dt, act);
-    a.arun = (T)arun[j]; a.brun = (T)brun[j]; a.crun = (T)crun[j]; a.first_stage = j == 0;
+    StageCoef c;
+    c.arun = arun[j]; c.brun = brun[j]; c.crun = crun[j]; c.first = j == 0;
     RC(ring_op(ctx, s, j == 0 ? RING_BEFORE_RK0 : RING_BEFORE_RKJ, act));
-    RC((launch_step<T, 2>(ctx, s, a)));
+    RC((launch_step<T, 2>(ctx, s, dt, act, c)));
     RC(ring_op(ctx, s, j == 0 ? RING_AFTER_RK0 : RING_AFTER_RKJ, act));
     if (j <= 2) RC(poten_t(ctx, s, act));
   }
@@ -557,11 +580,10 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, double dt, con
 
 // Hamming predictor-modifier-corrector (steppc.f90:35-159).
 template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
-  StepArgs<T> a = step_args(ctx, s, dt, act);
-  RC((launch_step<T, 0>(ctx, s, a)));
+  RC((launch_step<T, 0>(ctx, s, dt, act)));
   RC(ring_op(ctx, s, RING_AFTER_PRED, act));
   RC(poten_t(ctx, s, act));
-  RC((launch_step<T, 1>(ctx, s, a)));
+  RC((launch_step<T, 1>(ctx, s, dt, act)));
   RC(ring_op(ctx, s, RING_AFTER_CORR, act));
   RC(reduce<2>(ctx, 1, act));
   return EFFPOT_OK;
@@ -581,7 +603,9 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
   a.uimp = s.uimp; a.pot4 = s.pot4; a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.psi_all = s.psi; a.ring = s.ring;
   dim3 g = sgrid(ctx);
-#define M_(TXX) hipLaunchKernelGGL((k_energy<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
+#define M_(TXX)                                                                                   \
+  if (ctx->cplx) hipLaunchKernelGGL((k_energy<T, cx<T>, TXX>), g, dim3(kThreads), 0, ctx->st, a); \
+  else hipLaunchKernelGGL((k_energy<T, T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
   TX_SWITCH(ctx->tx, M_)
 #undef M_
   LAUNCHCHK();
@@ -661,7 +685,9 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->tx = p->nx % 64 == 0 ? 64 : (p->nx % 32 == 0 ? 32 : (p->nx % 16 == 0 ? 16 : 8));
   ctx->zc = std::min(16, p->nz);
   ctx->mixed = p->precision == EFFPOT_PREC_MIXED;
+  ctx->cplx = p->psi_complex != 0;
   if (const char* g = std::getenv("EFFPOT_GRAPHS")) ctx->graphs = std::atoi(g) != 0;
+  if (const char* g = std::getenv("EFFPOT_TUNE_LOG")) ctx->tune_log = std::atoi(g) != 0;
   ctx->h_active.assign(ctx->B, 1);
   ctx->have_imp.assign(ctx->B, 0);
   int rc;
@@ -763,14 +789,11 @@ static int den_of_slot(effpot_ctx* ctx, int slot) {
   std::vector<int> m(ctx->B, 0);
   m[slot] = 1;
   HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
-  hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st,
-                     ring_of(ctx, ctx->s64), ctx->s64.den, ctx->d_mask_tmp, ctx->N);
-  LAUNCHCHK();
-  if (ctx->mixed) {
-    hipLaunchKernelGGL((k_den_from_cur<float>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st,
-                       ring_of(ctx, ctx->s32), ctx->s32.den, ctx->d_mask_tmp, ctx->N);
-    LAUNCHCHK();
-  }
+  PSI_LAUNCH(k_den_from_cur, double, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s64), ctx->s64.den,
+             (const int*)ctx->d_mask_tmp, ctx->N);
+  if (ctx->mixed)
+    PSI_LAUNCH(k_den_from_cur, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s32), ctx->s32.den,
+               (const int*)ctx->d_mask_tmp, ctx->N);
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }
@@ -781,11 +804,22 @@ int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi) {
   hipSetDevice(ctx->dev);
   RC(reset_slot(ctx, ctx->s64, slot));
   cx<double>* dst = ctx->s64.psi + (long)slot * 5 * ctx->N;  // buffer 0 = CUR after reset
-  HIPCHK(hipMemcpyAsync(dst, psi, sizeof(double) * 2 * ctx->N, hipMemcpyHostToDevice, ctx->st));
+  const long nval = ctx->cplx ? 2 * ctx->N : ctx->N;        // doubles per state
+  if (ctx->cplx) {
+    HIPCHK(hipMemcpy(dst, psi, sizeof(double) * 2 * ctx->N, hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> re(ctx->N);
+    for (long i = 0; i < ctx->N; i++) {
+      if (psi[2 * i + 1] != 0.0)
+        return ctx->fail(EFFPOT_ERR_ARG, "complex wavefunction uploaded to a context with psi_complex = 0");
+      re[i] = psi[2 * i];
+    }
+    HIPCHK(hipMemcpy(dst, re.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice));
+  }
   if (ctx->mixed) {
     RC(reset_slot(ctx, ctx->s32, slot));
     hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, ctx->st, (const double*)dst,
-                       (float*)(ctx->s32.psi + (long)slot * 5 * ctx->N), 2 * ctx->N);
+                       (float*)(ctx->s32.psi + (long)slot * 5 * ctx->N), nval);
     LAUNCHCHK();
   }
   return den_of_slot(ctx, slot);
@@ -809,9 +843,14 @@ int effpot_download_psi(effpot_ctx* ctx, int slot, double* psi) {
   hipSetDevice(ctx->dev);
   int idx;
   RC(cur_index(ctx, ctx->s64, slot, &idx));
-  HIPCHK(hipMemcpyAsync(psi, ctx->s64.psi + ((long)slot * 5 + idx) * ctx->N, sizeof(double) * 2 * ctx->N,
-                        hipMemcpyDeviceToHost, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
+  const cx<double>* src = ctx->s64.psi + ((long)slot * 5 + idx) * ctx->N;
+  if (ctx->cplx) {
+    HIPCHK(hipMemcpy(psi, src, sizeof(double) * 2 * ctx->N, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<double> re(ctx->N);
+    HIPCHK(hipMemcpy(re.data(), src, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    for (long i = 0; i < ctx->N; i++) { psi[2 * i] = re[i]; psi[2 * i + 1] = 0.0; }
+  }
   return EFFPOT_OK;
 }
 
@@ -831,6 +870,8 @@ int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out) {
   if (which < 0 || which > 12) return ctx->fail(EFFPOT_ERR_ARG, "unknown field");
   HIPCHK(hipMemcpyAsync(out, src[which] + (long)slot * ctx->N, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  if (which == 3)  // the device keeps falfs = 1 - denalf/den0s
+    for (long i = 0; i < ctx->N; i++) out[i] = (1.0 - out[i]) * ctx->p.den0s;
   return EFFPOT_OK;
 }
 
@@ -1020,7 +1061,7 @@ static int widen_slot(effpot_ctx* ctx, int b) {
   RC(reset_slot(ctx, ctx->s64, b));
   hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, ctx->st,
                      (const float*)(ctx->s32.psi + ((long)b * 5 + idx) * ctx->N),
-                     (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), 2 * ctx->N);
+                     (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), ctx->cplx ? 2 * ctx->N : ctx->N);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -1077,9 +1118,7 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
   if (any_s) {
     RC(push_mask(ctx, 0, ms));
     if (ctx->mixed && !is32) {
-      hipLaunchKernelGGL((k_den_from_cur<double>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st,
-                         ring_of(ctx, ctx->s64), ctx->s64.den, dS, ctx->N);
-      LAUNCHCHK();
+      PSI_LAUNCH(k_den_from_cur, double, pw_grid(ctx->N, B), ring_of(ctx, ctx->s64), ctx->s64.den, dS, ctx->N);
     }
     RC(poten_t(ctx, s, dS));
     for (int b = 0; b < B; b++)
@@ -1170,11 +1209,10 @@ template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  StepArgs<T> a = step_args(ctx, s, 1e-4, ctx->d_active);
   auto once = [&]() -> int {
     switch (which) {
-      case 0: return launch_step<T, 0>(ctx, s, a);
-      case 1: return launch_step<T, 1>(ctx, s, a);
+      case 0: return launch_step<T, 0>(ctx, s, 1e-4, ctx->d_active);
+      case 1: return launch_step<T, 1>(ctx, s, 1e-4, ctx->d_active);
       case 2: return poten_t(ctx, s, ctx->d_active);
       default: return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
     }
@@ -1202,10 +1240,16 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     for (int zc : {4, 8, 16, 32}) {
       if (zc > ctx->nz) continue;
       ctx->tx = tx; ctx->zc = zc;
-      double t0 = 0, t1 = 0;
-      if (ctx->mixed) { RC(time_kernel_t(ctx, ctx->s32, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s32, 1, reps, &t1)); }
-      else { RC(time_kernel_t(ctx, ctx->s64, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s64, 1, reps, &t1)); }
-      if (t0 + t1 < best) { best = t0 + t1; best_tx = tx; best_zc = zc; }
+      double t0 = 0, t1 = 0, t2 = 0;
+      if (ctx->mixed) {
+        RC(time_kernel_t(ctx, ctx->s32, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s32, 1, reps, &t1));
+        RC(time_kernel_t(ctx, ctx->s32, 2, reps, &t2));
+      } else {
+        RC(time_kernel_t(ctx, ctx->s64, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s64, 1, reps, &t1));
+        RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t2));
+      }
+      if (ctx->tune_log) std::fprintf(stderr, "autotune tx=%d zc=%d pred=%.4f corr=%.4f poten=%.4f ms\n", tx, zc, t0, t1, t2);
+      if (t0 + t1 + t2 < best) { best = t0 + t1 + t2; best_tx = tx; best_zc = zc; }
     }
   }
   ctx->tx = best_tx; ctx->zc = best_zc;

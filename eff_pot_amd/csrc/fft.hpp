@@ -325,7 +325,8 @@ template <typename T> struct XArgs {
   cx<T>* c[4];
   long rslot, cslot;
   int nrows;  // ny*nz (even)
-  T scale;    // c2r only
+  T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
+  T aff0[4], aff1[4];
   const cx<T>* twN;
   const int* active;
 };
@@ -395,12 +396,13 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
   }
 }
 
-// Half-spectrum rows -> real rows (scaled by `scale`); the imaginary parts of
+// Half-spectrum rows -> real rows (out = aff0 + scale*aff1*x per field); the imaginary parts of
 // the self-conjugate bins 0 and N/2 are ignored, as a c2r transform does.
 template <typename T, int N>
 __global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
   const cx<T>* twN = rm.twN;
-  const T scale = rm.scale;
+  const T off = rm.aff0[blockIdx.y];
+  const T scale = rm.scale * rm.aff1[blockIdx.y];
   constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1, NH = N / 2 + 1;
   constexpr int P = C + 1;
   if (!rm.active[blockIdx.z]) return;
@@ -458,8 +460,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
     if (pr >= npairs) continue;
     cx<T> z = lds[i * P + c];
     T* ra = dst + (long)(2 * pr) * N;
-    ra[i] = scale * z.x;
-    ra[N + i] = scale * z.y;
+    ra[i] = off + scale * z.x;
+    ra[N + i] = off + scale * z.y;
   }
 }
 

@@ -105,24 +105,32 @@ template <int TX> inline dim3 zmarch_grid(int nx, int ny, int nz, int zc, int ns
 // ---------------------------------------------------------------------------
 // H psi = (lap(psi) h2o2m4 - pot4 psi) - uimp psi   (steppc.f90:53, steprk.f90:304
 // with timec = 1 and the impurity term taken real for imaginary time)
-template <typename T> EP_DEV cx<T> hpsi_of(cx<T> psi, const cx<T> (&d2)[3], T h2o2m4, T p4, T u) {
-  cx<T> lap = (d2[0] + d2[1]) + d2[2];
-  cx<T> a = h2o2m4 * lap - p4 * psi;
+template <typename T, typename V> EP_DEV V hpsi_of(V psi, const V (&d2)[3], T h2o2m4, T p4, T u) {
+  V lap = (d2[0] + d2[1]) + d2[2];
+  V a = h2o2m4 * lap - p4 * psi;
   return a - u * psi;
 }
+template <typename T> EP_DEV T norm2(cx<T> v) { return v.x * v.x + v.y * v.y; }
+template <typename T> EP_DEV T norm2(T v) { return v * v; }
+template <typename T> EP_DEV T dotre(cx<T> a, cx<T> b) { return a.x * b.x + a.y * b.y; }
+template <typename T> EP_DEV T dotre(T a, T b) { return a * b; }
 
-template <typename T> struct StepArgs {
+// V = cx<T> (complex psi) or T (real psi: imaginary time from a real state
+// keeps the imaginary part exactly zero, so the real path is the same
+// arithmetic on half the bytes).
+template <typename T, typename V> struct StepArgs {
   Grid g;
   StenCoef<T> sc;
   int zc;
   T h2o2m4, dt;
   const int* active;
   long slot;  // elements per slot (N)
-  const cx<T>* cur;   // psi read through the stencil
-  cx<T>* out;         // psi written (never == cur)
-  cx<T>* o1; cx<T>* o2; cx<T>* o3;  // psi history psiold(ioldp(1..3))
-  cx<T>* h1; cx<T>* h2;             // hpsi history hpsiold(ioldh(1..2))
-  cx<T>* pc; cx<T>* q;
+  const V* cur;   // psi read through the stencil
+  V* out;         // psi written (never == cur)
+  V* o1; V* o2; V* o3;  // psi history psiold(ioldp(1..3))
+  V* h1; V* h2;         // hpsi history hpsiold(ioldh(1..2))
+  V* pc; V* q;          // slot stride of pc/q is `pcq_slot` elements of V
+  long pcq_slot;
   const T* pot4; const T* uimp;
   T* den;
   double* partial;  // [slot][block][2]: sum c9|pc| (or 0), sum |psi_new|^2
@@ -133,63 +141,63 @@ template <typename T> struct StepArgs {
 };
 
 // Hamming predictor + modifier (steppc.f90:53-64).  out = o3's buffer.
-template <typename T, int TX>
-EP_DEV void k_pc_predict_body(const StepArgs<T>& a) {
+template <typename T, typename V, int TX>
+EP_DEV void k_pc_predict_body(const StepArgs<T, V>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ cx<T> lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
   const long so = (long)s * a.slot;
-  const cx<T>* cur = a.cur + so;
-  cx<T>* o3 = a.o3 + so; cx<T>* h1 = a.h1 + so; cx<T>* h2 = a.h2 + so; cx<T>* pc = a.pc + so;
+  const V* cur = a.cur + so;
+  V* o3 = a.o3 + so; V* h1 = a.h1 + so; V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
   const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
   const T c4o3dt = T(4.0 / 3.0) * a.dt, c112 = T(112.0 / 121.0);
   const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, cx<T>, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, cx<T> psi, cx<T>(&)[3], cx<T>(&d2)[3]) {
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      cx<T> hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      cx<T> pred = o3[i] + c4o3dt * ((T(2) * hp - h1[i]) + T(2) * h2[i]);
-      cx<T> pcv = pc[i];
-      cx<T> pn = pred - c112 * pcv;
+      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
+      V pred = o3[i] + c4o3dt * ((T(2) * hp - h1[i]) + T(2) * h2[i]);
+      V pcv = pc[i];
+      V pn = pred - c112 * pcv;
       o3[i] = pn;  // modified psi goes to the retired psi_{n-3} buffer
       pc[i] = pred;
       h2[i] = hp;
-      den[i] = pn.x * pn.x + pn.y * pn.y;
+      den[i] = norm2(pn);
     });
 }
 
 // Hamming corrector + final value (steppc.f90:135-151), errHe and norm partials.
 // Here o3 holds psi_n (psiold(ioldp(3)) after the modifier), o2 psi_{n-2},
 // h1 = H psi_n, h2 = H psi_{n-1} (after the ioldh swap).
-template <typename T, int TX>
-EP_DEV void k_pc_correct_body(const StepArgs<T>& a) {
+template <typename T, typename V, int TX>
+EP_DEV void k_pc_correct_body(const StepArgs<T, V>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ cx<T> lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
   __shared__ double red[8];
   const long so = (long)s * a.slot;
-  const cx<T>* cur = a.cur + so;
-  cx<T>* out = a.out + so;
-  const cx<T>* o3 = a.o3 + so; const cx<T>* o2 = a.o2 + so;
-  const cx<T>* h1 = a.h1 + so; const cx<T>* h2 = a.h2 + so; cx<T>* pc = a.pc + so;
+  const V* cur = a.cur + so;
+  V* out = a.out + so;
+  const V* o3 = a.o3 + so; const V* o2 = a.o2 + so;
+  const V* h1 = a.h1 + so; const V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
   const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
   const T c9 = T(9.0 / 121.0), dt3 = T(3) * a.dt;
   double v[2] = {0.0, 0.0};
   const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, cx<T>, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, cx<T> psi, cx<T>(&)[3], cx<T>(&d2)[3]) {
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      cx<T> hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      cx<T> corr = T(0.125) * ((T(9) * o3[i] - o2[i]) + dt3 * ((hp + T(2) * h1[i]) - h2[i]));
-      cx<T> pcv = pc[i] - corr;
-      cx<T> pn = corr + c9 * pcv;
+      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
+      V corr = T(0.125) * ((T(9) * o3[i] - o2[i]) + dt3 * ((hp + T(2) * h1[i]) - h2[i]));
+      V pcv = pc[i] - corr;
+      V pn = corr + c9 * pcv;
       pc[i] = pcv;
       out[i] = pn;
-      T d = pn.x * pn.x + pn.y * pn.y;
+      T d = norm2(pn);
       den[i] = d;
-      v[0] += (double)(c9 * sqrt(pcv.x * pcv.x + pcv.y * pcv.y));
+      v[0] += (double)(c9 * sqrt(norm2(pcv)));
       v[1] += (double)d;
     });
   block_sum<2>(v, red);
@@ -203,30 +211,30 @@ EP_DEV void k_pc_correct_body(const StepArgs<T>& a) {
 // One Runge-Kutta-Gill stage (steprk.f90:301-316).  At the first stage the host
 // has rotated the history so that o1 := cur (psi at the start) and h1 is the
 // retired buffer that receives H psi.
-template <typename T, int TX>
-EP_DEV void k_rk_stage_body(const StepArgs<T>& a) {
+template <typename T, typename V, int TX>
+EP_DEV void k_rk_stage_body(const StepArgs<T, V>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ cx<T> lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
   __shared__ double red[8];
   const long so = (long)s * a.slot;
-  const cx<T>* cur = a.cur + so;
-  cx<T>* out = a.out + so; cx<T>* h1 = a.h1 + so; cx<T>* q = a.q + so;
+  const V* cur = a.cur + so;
+  V* out = a.out + so; V* h1 = a.h1 + so; V* q = a.q + (long)s * a.pcq_slot;
   const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
   double v[2] = {0.0, 0.0};
   const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, cx<T>, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, cx<T> psi, cx<T>(&)[3], cx<T>(&d2)[3]) {
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      cx<T> hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      cx<T> qv = q[i];
-      cx<T> a2 = a.arun * (hp - a.brun * qv);
+      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
+      V qv = q[i];
+      V a2 = a.arun * (hp - a.brun * qv);
       q[i] = (qv + T(3) * a2) - a.crun * hp;
       if (a.first_stage) h1[i] = hp;
-      cx<T> pn = psi + a.dt * a2;
+      V pn = psi + a.dt * a2;
       out[i] = pn;
-      T d = pn.x * pn.x + pn.y * pn.y;
+      T d = norm2(pn);
       den[i] = d;
       v[1] += (double)d;
     });
@@ -272,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
       T cg = dencg[i];
       s0[i] = d * cg * (a.cp4 + a.cpp4 * cg);
       if (a.otc) {
-        T fa = T(1) - denalf[i] / a.den0s;
+        T fa = denalf[i];  // holds falfs
         s1[i] = fa * d1[0]; s2[i] = fa * d1[1]; s3[i] = fa * d1[2];
       }
     });
@@ -305,7 +313,6 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   const T *ix = a.ix + so, *iy = a.iy + so, *iz = a.iz + so;
   const T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
   T *src = a.src + so, *div = a.div + so;
-  const T inv0 = a.den0s;
   auto gfa = [&](const T* comp, int x, int y, int z) -> T {
     if (g.periodic) {
       x = (x + g.nx) % g.nx; y = (y + g.ny) % g.ny; z = (z + g.nz) % g.nz;
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
       return T(0);
     }
     long i = g.idx(x, y, z);
-    return (T(1) - denalf[i] / inv0) * comp[i];
+    return denalf[i] * comp[i];  // denalf holds falfs
   };
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int ntx = (g.nx + TX - 1) / TX;
@@ -372,30 +379,29 @@ template <typename T> struct EnergyArgs {
   double* partial;
   int nblk;
 };
-template <typename T, int TX>
+template <typename T, typename V, int TX>
 __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ cx<T> lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
   __shared__ double red[4 * kEnergySums];
   const long so = (long)s * a.slot;
-  const cx<T>* psi = a.psi_all + ((long)s * 5 + a.ring[8 * s]) * a.slot;
+  const V* psi = reinterpret_cast<const V*>(a.psi_all + ((long)s * 5 + a.ring[8 * s]) * a.slot);
   double v[kEnergySums];
 #pragma unroll
   for (int k = 0; k < kEnergySums; k++) v[k] = 0.0;
   const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, cx<T>, TX, true, true>(psi, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, cx<T> p, cx<T>(&d1)[3], cx<T>(&d2)[3]) {
+  zmarch<T, V, TX, true, true>(psi, a.g, a.sc, z0, z1, lds,
+    [&](int x, int y, int z, bool in, V p, V(&d1)[3], V(&d2)[3]) {
       if (!in) return;
       const long i = so + a.g.idx(x, y, z);
       const T d = a.den[i], cg = a.dencg[i];
-      T ek = ((d1[0].x * d1[0].x + d1[0].y * d1[0].y) + (d1[1].x * d1[1].x + d1[1].y * d1[1].y)) +
-             (d1[2].x * d1[2].x + d1[2].y * d1[2].y);
+      T ek = (norm2(d1[0]) + norm2(d1[1])) + norm2(d1[2]);
       v[0] += (double)ek;
       v[1] += (double)(a.delj4[i] * d);
       v[2] += (double)(d * cg * cg * (a.aux1 + a.aux2 * cg));
       if (a.otc) {
-        T fa = T(1) - a.denalf[i] / a.den0s;
+        T fa = a.denalf[i];  // holds falfs
         v[3] += (double)(fa * ((a.dx[i] * a.ix[i] + a.dy[i] * a.iy[i]) + a.dz[i] * a.iz[i]));
       }
       v[4] += (double)(d * (T(1) + tanh(a.beta * (d - a.den_m))));
@@ -404,8 +410,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
       v[7] += (double)d * (a.x0 + a.hx * x);
       v[8] += (double)d * (a.y0 + a.hy * y);
       v[9] += (double)d * (a.z0 + a.hz * z);
-      cx<T> hp = hpsi_of(p, d2, a.h2o2m4, a.pot4[i], a.uimp[i]);
-      v[10] -= (double)(p.x * hp.x + p.y * hp.y);
+      V hp = hpsi_of(p, d2, a.h2o2m4, a.pot4[i], a.uimp[i]);
+      v[10] -= (double)dotre(p, hp);
     });
   block_sum<kEnergySums>(v, red);
   if (threadIdx.x == 0) {

@@ -19,7 +19,8 @@ def scan_positions(natoms, npos, zmax_frac=1.6):
 def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, tol=1e-9, tol_fp32=1e-7,
              max_steps=60000, warm=False, seed=20261203, h_core=2.377):
     xmax = spacing * n / 2
-    p = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=npos)
+    p = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=npos,
+                    psi_complex=0)
     zs = scan_positions(natoms, npos)
     R = 2.22 * natoms ** (1.0 / 3.0)
     dt = default_dt(p)
@@ -27,7 +28,8 @@ def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, to
                      tol_fp32=tol_fp32, check_finite=0)
     warm_psi = None
     if warm:
-        pw = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=1)
+        pw = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=1,
+                         psi_complex=0)
         with Context(pw) as c:
             c.init_fermi(0.0218, R, 0.6, 1e-6, seed)
             imp = make_morse(D=0.0)

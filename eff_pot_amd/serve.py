@@ -63,7 +63,7 @@ def grid_for(natoms):
 
 def class_params(n, slots, precision=PREC_MIXED, h_core=2.377):
     return make_params(n, xmax=SPACING * n / 2, n4=300.0, core=CORE_OTC, h_core=h_core, precision=precision,
-                       batch=slots)
+                       batch=slots, psi_complex=0)
 
 
 def plan_slots(hbm_budget_bytes, weights=None, precision=PREC_MIXED, max_slots=64):

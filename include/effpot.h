@@ -61,6 +61,11 @@ typedef struct effpot_params {
   int32_t icon;       /* EFFPOT_ICON_ZERO (13, default) | EFFPOT_ICON_PERIODIC */
   int32_t precision;  /* EFFPOT_PREC_FP64 | EFFPOT_PREC_MIXED               */
   int32_t batch;      /* number of independent solves (slots), >= 1         */
+  int32_t psi_complex;/* 1: complex psi (e.g. a real-time snapshot); 0: real psi
+                         (imaginary time from a real start keeps Im psi = 0
+                         exactly, so the real path is bit-identical on half the
+                         bytes); uploads with Im psi != 0 are then rejected   */
+  int32_t reserved_;
   double xmax, ymax, zmax; /* box half widths; h = 2*xmax/nx (DFT4He3d.f90:205-207) */
   double n4;          /* number of He atoms the density is normalised to   */
   double h_core;      /* LJ core radius h4 (A)                             */
@@ -221,9 +226,9 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
  * driver loop); the solve of one slot is the same whatever the other slots do. */
 int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots);
 
-/* Autotuner (no reference counterpart): times the H psi + predictor and
- * corrector kernels for every stencil tile (x width 64/32/16/8 dividing nx,
- * z chunk 4/8/16/32) on the live state and keeps the fastest.  Overwrites the
+/* Autotuner (no reference counterpart): times the H psi + predictor,
+ * corrector and poten kernels for every stencil tile (x width 64/32/16/8
+ * dividing nx, z chunk 4/8/16/32) on the live state and keeps the fastest.  Overwrites the
  * integrator state like effpot_kernel_time: call it before uploading states. */
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
 /* Sets the stencil tile explicitly (x width, z chunk). */

@@ -87,7 +87,7 @@ def _req(ctx, slot, z, seed):
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
 def test_same_result_alone_and_in_the_mix(precision, gpu_available):
     print(f"[mix] precision={precision}", flush=True)
-    opts = SolveOpts(dt=1.9e-4, tol=1e-9, n_rk=3, pener=50, max_steps=6000, polish_steps=6000, tol_fp32=1e-7,
+    opts = SolveOpts(dt=1.9e-4, tol=1e-8, n_rk=3, pener=50, max_steps=40000, polish_steps=40000, tol_fp32=3e-8,
                      check_finite=1)
     zs, seeds = [0.5, 1.5, -2.0], [1, 2, 3]
     alone = []
@@ -195,3 +195,26 @@ def test_graph_replay_matches_eager(gpu_available, monkeypatch):
         assert a == b
     for a, b in zip(traj["1"][1], traj["0"][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_real_psi_path_is_bit_identical(precision, gpu_available):
+    """Imaginary time from a real start keeps Im psi = 0 exactly, so the real
+    storage path must reproduce the complex path bit for bit."""
+    print(f"[real psi] precision={precision}", flush=True)
+    opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=25, max_steps=200, polish_steps=60, tol_fp32=0.0,
+                     check_finite=1)
+    out = {}
+    for cplx in (1, 0):
+        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision,
+                        psi_complex=cplx)
+        with ep.Context(p) as ctx:
+            _req(ctx, 0, 1.0, 5)
+            r = ctx.solve(opts)[0]
+            out[cplx] = (r.e.etot, r.e.eimpu, r.e.ekin4, r.e.mu, r.steps, ctx.psi())
+    a, b = out[1], out[0]
+    assert a[:5] == b[:5]
+    assert np.array_equal(a[5], b[5])
+    with ep.Context(make_params(32, xmax=12.0, psi_complex=0)) as ctx:
+        with pytest.raises(ep.EffpotError):
+            ctx.upload_psi(np.full(ctx.shape, 0.1 + 0.01j))
