@@ -11,8 +11,9 @@ predictor-corrector steps in imaginary time with renormalisation to N until
 
 A "step" of this benchmark is one batch of B such solves run to convergence
 (B impurity positions z0 = 0.5 R * (0.95 .. 1.05)); value = solves / second.
-Modes: --mode mixed (default): FP32 bulk + FP64 reductions + FP64 polish,
-batched; --mode fp64: the plain FP64 path, one solve per context.
+Modes: --mode mixed (default): FP64 wavefunction and integrator, FP32 mean
+field (all FFTs and density stencils), FP64 reductions and energy checks,
+FP64 final polish; --mode fp64: the plain FP64 path, one solve per context.
 
 Also reported: the live roofline of the dominant kernel (H psi + corrector),
 timed with HIP events on the library's stream, and the CPU baseline (the
@@ -40,11 +41,18 @@ def parse():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--tol", type=float, default=1e-9)
-    ap.add_argument("--tol-fp32", type=float, default=1e-8)
+    ap.add_argument("--tol-fp32", type=float, default=1e-9)
     ap.add_argument("--cpu-steps", type=int, default=4, help="PC steps timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256"], default="solve128",
+                    help="solve128 (default, the headline); the others are the batched scan, the serving stream, "
+                         "the 256^3 stress test and the small CPU-runnable case")
+    ap.add_argument("--npos", type=int, default=48)
+    ap.add_argument("--nreq", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=0.5)
+    ap.add_argument("--stress-steps", type=int, default=300)
     return ap.parse_args()
 
 
@@ -74,6 +82,97 @@ def solve_batch(ctx, p, R, zs, opts, seed=20261203):
     return ctx.solve(opts)
 
 
+def common_line(a, metric, value, unit, elapsed, dtype, workload, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+         "ms_per_step": 1000.0 * elapsed / max(a.steps, 1), "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded Fermi droplets + 1e-6 noise)",
+         "config": {"workload": workload}}
+    d["config"].update(extra)
+    return d
+
+
+def run_cfg0(a):
+    """3 RK + 1997 PC FP64 steps on the N=300, 64^3, h=0.75 A droplet."""
+    import eff_pot_amd as ep
+    p = ep.make_params(64, xmax=24.0, n4=300.0, core=ep.CORE_OTC, h_core=2.377, psi_complex=0)
+    opts = ep.SolveOpts(dt=ep.default_dt(p), tol=0.0, n_rk=3, pener=100, max_steps=2000, polish_steps=0,
+                        tol_fp32=0.0, check_finite=0)
+    with ep.Context(p) as c:
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            c.init_fermi(0.0218, 2.22 * 300 ** (1 / 3), 0.6, 1e-6, 20261203)
+            c.set_impurity(ep.make_morse())
+            r = c.solve(opts)[0]
+        el = time.perf_counter() - t0
+    return common_line(a, "2000-step relaxations/s (N=300, 64^3)", a.steps / el, "runs/s", el, "f64",
+                       "He4 N=300, 64^3 h=0.75 A, OTC h_core=2.377, Morse at the centre, 3 RK + 1997 PC steps",
+                       E_tot_K=r.e.etot, E_imp_K=r.e.eimpu, step_ms=1000 * el / a.steps / 2000)
+
+
+def run_scan(a):
+    from eff_pot_amd import scan
+    prec = 1 if a.mode == "mixed" else 0
+    t0 = time.perf_counter()
+    res, meta = scan.run_scan(n=96, spacing=0.6, natoms=1000.0, npos=a.npos, precision=prec, tol=a.tol,
+                              tol_fp32=a.tol_fp32, max_steps=a.max_steps)
+    el = time.perf_counter() - t0
+    return common_line(a, "V_eff(z0) scan: converged positions/s (N=1000, 96^3)", len(res) / el, "solves/s", el,
+                       "fp32 fields + f64" if prec else "f64",
+                       f"He4 N=1000, 96^3 h=0.6 A, {a.npos} impurity positions z0 = 0..1.6R in one batched context",
+                       mode=a.mode, curve=[[r["z0"], r["eimp"], r["steps"], r["converged"], r["inside"]] for r in res])
+
+
+def run_serve(a):
+    from eff_pot_amd import serve
+    reqs = serve.make_stream(a.nreq, rate=a.rate, seed=7)
+    plan = {64: 8, 96: 6, 128: 4}
+    srv = serve.Server(plan, tol=a.tol, tol_fp32=a.tol_fp32, max_steps=a.max_steps,
+                       precision=1 if a.mode == "mixed" else 0)
+    t0 = time.perf_counter()
+    try:
+        srv.run(reqs)
+    finally:
+        srv.close()
+    el = time.perf_counter() - t0
+    lat = [r.finish - r.arrival for r in reqs if not r.failed]
+    ok = [r for r in reqs if r.converged]
+    return common_line(a, "served impurity-energy solves/s (continuous batching)", len(ok) / el, "solves/s", el,
+                       "fp32 fields + f64" if a.mode == "mixed" else "f64",
+                       f"{a.nreq} requests, Poisson rate {a.rate}/s, N log-uniform [100,3000], grids 64/96/128 at 0.75 A",
+                       p50_latency_s=serve.percentile(lat, 0.5), p99_latency_s=serve.percentile(lat, 0.99),
+                       slots=plan, converged=len(ok), grids={g: sum(1 for r in reqs if r.grid == g) for g in (64, 96, 128)})
+
+
+def run_stress(a):
+    """256^3, h=0.4 A, N=5000: FP64 single solve and a batch of 8 mixed, timed over a bounded number of steps."""
+    import eff_pot_amd as ep
+    out = {}
+    R = 2.22 * 5000 ** (1 / 3)
+    for label, prec, B in (("fp64_single", 0, 1), ("mixed_batch8", 1, 8)):
+        p = ep.make_params(256, xmax=51.2, n4=5000.0, core=ep.CORE_OTC, h_core=2.377, precision=prec, batch=B,
+                           psi_complex=0)
+        opts = ep.SolveOpts(dt=ep.default_dt(p), tol=0.0, n_rk=3, pener=100, max_steps=a.stress_steps,
+                            polish_steps=0, tol_fp32=0.0, check_finite=0)
+        with ep.Context(p) as c:
+            for b in range(B):
+                c.init_fermi(0.0218, R, 0.6, 1e-6, 11 + b, slot=b)
+                c.set_impurity(ep.make_morse(rimp=(0.0, 0.0, 0.5 * R * b / max(B - 1, 1))), slot=b)
+            t0 = time.perf_counter()
+            r = c.solve(opts)
+            el = time.perf_counter() - t0
+            s = 4 if prec else 8
+            ms_pred, ms_corr, ms_pot = c.kernel_time(0, 5), c.kernel_time(1, 5), c.kernel_time(2, 3)
+        n = 256 ** 3
+        per_pt = 76 if prec else 88
+        out[label] = {"ms_per_step": 1000 * el / a.stress_steps, "slot_steps_per_s": B * a.stress_steps / el,
+                      "E_imp_K": [x.e.eimpu for x in r],
+                      "pred_GBs": per_pt * n * B / (ms_pred * 1e-3) / 1e9,
+                      "corr_GBs": per_pt * n * B / (ms_corr * 1e-3) / 1e9,
+                      "poten_ms": ms_pot}
+    return common_line(a, "256^3 stress: slot-steps/s", out["mixed_batch8"]["slot_steps_per_s"], "slot-steps/s",
+                       1.0, "f64 / fp32 fields", "He4 N=5000, 256^3 h=0.4 A", **out)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -82,6 +181,10 @@ def main():
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import eff_pot_amd as ep
+    if a.workload != "solve128":
+        fn = {"cfg0": run_cfg0, "scan": run_scan, "serve": run_serve, "stress256": run_stress}[a.workload]
+        print(json.dumps(fn(a)), flush=True)
+        return
 
     B = a.batch if a.mode == "mixed" else 1
     p, R = workload(a.n, a.mode, B)
@@ -117,7 +220,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if a.mode == "mixed" else "f64",
+        "dtype": "f64 state, fp32 fields" if a.mode == "mixed" else "f64",
         "data": "synthetic (seeded Fermi droplet + 1e-6 noise)",
         "config": {"workload": f"He4 N=1000 droplet, {a.n}^3 grid h={2 * p.xmax / a.n:.3f} A, OTC h_core=2.377, "
                                f"Morse D=30 re=4 a=1 at z0~0.5R, to |dE/E|<1e-9/step",
@@ -127,14 +230,17 @@ def main():
     }
     # ---- roofline of the dominant kernel: H psi + corrector ------------------
     if not a.no_roofline:
-        s = 4 if a.mode == "mixed" else 8
         Npts = a.n ** 3
         ms = ctx.kernel_time(1, reps=20)
-        bytes_alg = 19 * s * Npts * B     # 14 reads + 5 writes of s-byte reals per point and slot
+        # H psi + corrector on a real wavefunction, per point and slot: reads psi, psi_n,
+        # psi_{n-2}, Hpsi_n, Hpsi_{n-1}, pc (FP64) and pot4, uimp; writes psi_{n+1}, pc (FP64)
+        # and den.  pot4/uimp/den are FP64 (88 B) or FP32 in the mixed mode (76 B).
+        per_pt = 76 if a.mode == "mixed" else 88
+        bytes_alg = per_pt * Npts * B
         ach = bytes_alg / (ms * 1e-3) / 1e9
         tx = tile[0] if tile else 64
-        line["roofline"] = {"bound": "hbm", "kernel": f"k_step<{'float' if s == 4 else 'double'},{tx},1> "
-                                                      f"(H psi + Hamming corrector)",
+        line["roofline"] = {"bound": "hbm", "kernel": f"k_step<double,double,{'float' if a.mode == 'mixed' else 'double'},"
+                                                      f"{tx},1> (H psi + Hamming corrector)",
                             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": ach / HBM_PEAK_GBS, "traffic": None, "ms_per_launch": ms,
                             "bytes_per_launch": bytes_alg}

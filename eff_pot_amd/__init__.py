@@ -52,6 +52,7 @@ def lib():
                                         C.c_uint64, _dp]
         L.effpot_set_impurity.argtypes = [vp, C.c_int, C.POINTER(Impurity)]
         L.effpot_set_natoms.argtypes = [vp, C.c_int, C.c_double]
+        L.effpot_set_dt.argtypes = [vp, C.c_int, C.c_double]
         L.effpot_poten.argtypes = [vp]
         L.effpot_energy.argtypes = [vp, C.POINTER(Energies)]
         L.effpot_step_rk.argtypes = [vp, C.c_double]
@@ -142,6 +143,9 @@ class Context:
 
     def set_natoms(self, n4, slot=0):
         self._chk(lib().effpot_set_natoms(self.h, slot, n4))
+
+    def set_dt(self, dt, slot=0):
+        self._chk(lib().effpot_set_dt(self.h, slot, dt))
 
     def set_impurity(self, imp: Impurity, slot=0):
         self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))

@@ -8,6 +8,7 @@
 // corrector) and no host round trip is needed between steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -66,37 +67,37 @@ template <typename T> struct Ring {
 };
 
 // ---- kernels that need the ring --------------------------------------------
-template <typename T, typename V, int TX, int MODE>  // MODE 0 predict, 1 correct, 2 rk stage
-__global__ __launch_bounds__(kThreads) void k_step(StepArgs<T, V> a, Ring<T> rg) {
+template <typename T, typename V, typename P, int TX, int MODE>  // MODE 0 predict, 1 correct, 2 rk stage
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs<T, V, P> a, Ring<T> rg) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
   // resolve the roles of this slot into the argument block, then run the
   // shared kernel body (the StepArgs pointers are slot-0 based; offsets are
   // re-applied by the body, so pass slot-relative pointers minus the offset).
   const long so = (long)s * a.slot;
-  auto P = [&](int r) { return reinterpret_cast<V*>(rg.p(s, r)) - so; };
+  auto PP = [&](int r) { return reinterpret_cast<V*>(rg.p(s, r)) - so; };
   auto H = [&](int r) { return reinterpret_cast<V*>(rg.h(s, r)) - so; };
-  StepArgs<T, V> b = a;
-  b.cur = P(R_CUR);
-  b.out = MODE == 1 ? P(R_SPARE) : P(R_OUT);
-  b.o1 = P(R_O1); b.o2 = P(R_O2); b.o3 = P(R_O3);
+  StepArgs<T, V, P> b = a;
+  b.cur = PP(R_CUR);
+  b.out = MODE == 1 ? PP(R_SPARE) : PP(R_OUT);
+  b.o1 = PP(R_O1); b.o2 = PP(R_O2); b.o3 = PP(R_O3);
   b.h1 = H(R_H1); b.h2 = H(R_H2);
-  if constexpr (MODE == 0) k_pc_predict_body<T, V, TX>(b);
-  else if constexpr (MODE == 1) k_pc_correct_body<T, V, TX>(b);
-  else k_rk_stage_body<T, V, TX>(b);
+  if constexpr (MODE == 0) k_pc_predict_body<T, V, P, TX>(b);
+  else if constexpr (MODE == 1) k_pc_correct_body<T, V, P, TX>(b);
+  else k_rk_stage_body<T, V, P, TX>(b);
 }
 
-template <typename T, typename V>
-__global__ void k_renorm_ring(Ring<T> rg, T* den, const double* scale, const int* active, long N) {
+template <typename T, typename V, typename P>
+__global__ void k_renorm_ring(Ring<T> rg, P* den, const double* scale, const int* active, long N) {
   const int s = blockIdx.y;
   if (!active[s]) return;
   const T f = (T)scale[s];
   V* psi = reinterpret_cast<V*>(rg.p(s, R_CUR));
-  T* d = den + (long)s * N;
+  P* d = den + (long)s * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
     V p = f * psi[i];
     psi[i] = p;
-    d[i] = norm2(p);
+    d[i] = (P)norm2(p);
   }
 }
 
@@ -118,25 +119,25 @@ __global__ __launch_bounds__(kThreads) void k_norm(Ring<T> rg, const int* active
   }
 }
 
-template <typename T, typename V>
-__global__ void k_den_from_cur(Ring<T> rg, T* den, const int* active, long N) {
+template <typename T, typename V, typename P>
+__global__ void k_den_from_cur(Ring<T> rg, P* den, const int* active, long N) {
   const int s = blockIdx.y;
   if (!active[s]) return;
   const V* psi = reinterpret_cast<const V*>(rg.p(s, R_CUR));
-  T* d = den + (long)s * N;
+  P* d = den + (long)s * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    d[i] = norm2(psi[i]);
+    d[i] = (P)norm2(psi[i]);
   }
 }
 
 template <typename T, int TX, int ID>
 __global__ __launch_bounds__(kThreads) void k_deriv1(Grid g, StenCoef<T> sc, int zc, const T* f, T* df, int axis) {
-  __shared__ T lds[Tile<TX>::LX * Tile<TX>::LY];
-  const int z0 = blockIdx.y * zc, z1 = min(z0 + zc, g.nz);
-  zmarch<T, T, TX, ID == 1, ID == 2>(f, g, sc, z0, z1, lds, [&](int x, int y, int z, bool in, T, T(&d1)[3], T(&d2)[3]) {
-    if (!in) return;
-    df[g.idx(x, y, z)] = ID == 1 ? d1[axis] : d2[axis];
-  });
+  __shared__ T lds[2 * Tile<TX>::PL];
+  zmarch<T, T, TX, ID == 1, ID == 2>(f, g, sc, TilePos<TX>(g, zc), lds, [](long) { return NoPre{}; },
+    [&](int x, int y, int z, bool in, T, T(&d1)[3], T(&d2)[3], const NoPre&) {
+      if (!in) return;
+      df[g.idx(x, y, z)] = ID == 1 ? d1[axis] : d2[axis];
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -181,16 +182,20 @@ struct effpot_ctx {
   double* d_red = nullptr;    // [B][16]
   double* d_scale = nullptr;  // [B]
   double* d_n4 = nullptr;     // [B] atoms each slot is normalised to
+  double* d_dt = nullptr;     // [B] time step each slot is advanced with
+  std::vector<double> h_dt;   // per-slot override (0: the solve options' dt)
+  std::vector<double> dt_dev; // what d_dt holds
   std::vector<double> h_n4;
   int* d_bad = nullptr;
   int* d_mask = nullptr;      // [5][B] per-purpose activity masks of effpot_tick
   int* h_mask = nullptr;      // pinned staging for d_mask
+  std::vector<int> mask_dev;  // what d_mask holds (-1: unknown)
+  bool corr_den = true;       // corrector stores the density (false when a renormalisation follows)
   double* h_pin = nullptr;    // pinned host buffer
   int tx = 64, zc = 16;
   bool graphs = true;               // replay captured PC steps (EFFPOT_GRAPHS=0 disables)
   bool tune_log = false;            // EFFPOT_TUNE_LOG=1 prints the autotuner's timings
   hipGraphExec_t pc_exec[2] = {nullptr, nullptr};  // [0] FP64, [1] FP32
-  double pc_dt[2] = {0.0, 0.0};
   std::vector<void*> allocs;
   std::string err;
 
@@ -246,11 +251,13 @@ template <typename T> int dalloc(effpot_ctx* ctx, T** p, size_t count) {
   return EFFPOT_OK;
 }
 
-template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s) {
+// fields_only: the FP32 field state of a MIXED context (no wavefunction ring).
+template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_only = false) {
   const long N = ctx->N, M = ctx->M, B = ctx->B;
+  const long NP = fields_only ? 0 : N;
   int rc;
 #define A_(ptr, cnt) if ((rc = dalloc(ctx, &(ptr), (size_t)(cnt))) != EFFPOT_OK) return rc
-  A_(s.psi, B * 5 * N); A_(s.hp, B * 2 * N); A_(s.pc, B * N); A_(s.q, B * N);
+  A_(s.psi, B * 5 * NP); A_(s.hp, B * 2 * NP); A_(s.pc, B * NP); A_(s.q, B * NP);
   T** reals[] = {&s.den, &s.pot4, &s.uimp, &s.dencg, &s.delj4, &s.denalf, &s.ua, &s.ualphas, &s.dx, &s.dy,
                  &s.dz, &s.sto1, &s.intx, &s.inty, &s.intz, &s.div, &s.src[0], &s.src[1], &s.src[2], &s.src[3]};
   for (T** r : reals) A_(*r, B * N);
@@ -260,10 +267,12 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s) {
   A_(s.ring, B * 8);
 #undef A_
   // zero the whole state (history, pc, q, fields)
-  HIPCHK(hipMemsetAsync(s.psi, 0, sizeof(cx<T>) * B * 5 * N, ctx->st));
-  HIPCHK(hipMemsetAsync(s.hp, 0, sizeof(cx<T>) * B * 2 * N, ctx->st));
-  HIPCHK(hipMemsetAsync(s.pc, 0, sizeof(cx<T>) * B * N, ctx->st));
-  HIPCHK(hipMemsetAsync(s.q, 0, sizeof(cx<T>) * B * N, ctx->st));
+  if (!fields_only) {
+    HIPCHK(hipMemsetAsync(s.psi, 0, sizeof(cx<T>) * B * 5 * N, ctx->st));
+    HIPCHK(hipMemsetAsync(s.hp, 0, sizeof(cx<T>) * B * 2 * N, ctx->st));
+    HIPCHK(hipMemsetAsync(s.pc, 0, sizeof(cx<T>) * B * N, ctx->st));
+    HIPCHK(hipMemsetAsync(s.q, 0, sizeof(cx<T>) * B * N, ctx->st));
+  }
   for (T** r : reals) HIPCHK(hipMemsetAsync(*r, 0, sizeof(T) * B * N, ctx->st));
   // twiddles
   int ns[3] = {ctx->nx, ctx->ny, ctx->nz};
@@ -413,27 +422,48 @@ template <typename T> int ring_op(effpot_ctx* ctx, State<T>& s, int op, const in
 // Runge-Kutta-Gill stage coefficients (unused by the PC kernels).
 struct StageCoef { double arun = 0, brun = 0, crun = 0; int first = 0; };
 
-template <typename T, typename V, int MODE>
-int launch_step_v(effpot_ctx* ctx, State<T>& s, double dt, const int* act, StageCoef c) {
-  StepArgs<T, V> a{};
-  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dt = (T)dt;
+template <typename T, typename V, typename P, int MODE>
+int launch_step_v(effpot_ctx* ctx, State<T>& s, State<P>& f, const int* act, StageCoef c) {
+  StepArgs<T, V, P> a{};
+  a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dtv = ctx->d_dt;
   a.active = act; a.slot = ctx->N;
   a.pc = reinterpret_cast<V*>(s.pc); a.q = reinterpret_cast<V*>(s.q); a.pcq_slot = ctx->N;
-  a.pot4 = s.pot4; a.uimp = s.uimp; a.den = s.den;
+  a.pot4 = f.pot4; a.uimp = f.uimp; a.den = f.den;
   a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.arun = (T)c.arun; a.brun = (T)c.brun; a.crun = (T)c.crun; a.first_stage = c.first;
+  a.write_den = ctx->corr_den ? 1 : 0;
   Ring<T> rg = ring_of(ctx, s);
   dim3 g = sgrid(ctx);
-#define M_(TXX) hipLaunchKernelGGL((k_step<T, V, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
+#define M_(TXX) hipLaunchKernelGGL((k_step<T, V, P, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
   TX_SWITCH(ctx->tx, M_)
 #undef M_
   LAUNCHCHK();
   return EFFPOT_OK;
 }
+// f32: the step reads/writes the FP32 mean fields of a MIXED context.
 template <typename T, int MODE>
-int launch_step(effpot_ctx* ctx, State<T>& s, double dt, const int* act, StageCoef c = StageCoef()) {
-  if (ctx->cplx) return launch_step_v<T, cx<T>, MODE>(ctx, s, dt, act, c);
-  return launch_step_v<T, T, MODE>(ctx, s, dt, act, c);
+int launch_step(effpot_ctx* ctx, State<T>& s, const int* act, bool f32, StageCoef c = StageCoef()) {
+  if (f32) {
+    if (ctx->cplx) return launch_step_v<T, cx<T>, float, MODE>(ctx, s, ctx->s32, act, c);
+    return launch_step_v<T, T, float, MODE>(ctx, s, ctx->s32, act, c);
+  }
+  if (ctx->cplx) return launch_step_v<T, cx<T>, T, MODE>(ctx, s, s, act, c);
+  return launch_step_v<T, T, T, MODE>(ctx, s, s, act, c);
+}
+
+// d_dt[s] = per-slot override, else `dt` (uploaded only when it changes).
+int push_dt(effpot_ctx* ctx, double dt) {
+  bool change = false;
+  std::vector<double> v(ctx->B);
+  for (int b = 0; b < ctx->B; b++) {
+    v[b] = ctx->h_dt[b] > 0 ? ctx->h_dt[b] : dt;
+    if (v[b] != ctx->dt_dev[b]) change = true;
+  }
+  if (!change) return EFFPOT_OK;
+  HIPCHK(hipStreamSynchronize(ctx->st));  // kernels in flight read d_dt
+  HIPCHK(hipMemcpy(ctx->d_dt, v.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice));
+  ctx->dt_dev = v;
+  return EFFPOT_OK;
 }
 // Launch a ring kernel templated on <T, V> with V chosen by the context's psi type.
 #define PSI_LAUNCH(KERNEL, T_, GRID, ...)                                                          \
@@ -442,6 +472,19 @@ int launch_step(effpot_ctx* ctx, State<T>& s, double dt, const int* act, StageCo
     else hipLaunchKernelGGL((KERNEL<T_, T_>), GRID, dim3(256), 0, ctx->st, __VA_ARGS__);              \
     LAUNCHCHK();                                                                                     \
   } while (0)
+// Same for <T, V, P> kernels writing fields of precision P.
+#define PSI_LAUNCH_P(KERNEL, T_, P_, GRID, ...)                                                          \
+  do {                                                                                                     \
+    if (ctx->cplx) hipLaunchKernelGGL((KERNEL<T_, cx<T_>, P_>), GRID, dim3(256), 0, ctx->st, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<T_, T_, P_>), GRID, dim3(256), 0, ctx->st, __VA_ARGS__);              \
+    LAUNCHCHK();                                                                                           \
+  } while (0)
+// den (precision of the fields in use) = |psi_cur|^2 for the slots in `act`.
+int den_from_cur(effpot_ctx* ctx, const int* act, bool f32) {
+  if (f32) PSI_LAUNCH_P(k_den_from_cur, double, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s64), ctx->s32.den, act, ctx->N);
+  else PSI_LAUNCH_P(k_den_from_cur, double, double, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s64), ctx->s64.den, act, ctx->N);
+  return EFFPOT_OK;
+}
 
 template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act) {
   hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
@@ -548,9 +591,13 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
 }
 
 // ---- integrators ----------------------------------------------------------------
-template <typename T> int renorm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
-  PSI_LAUNCH(k_renorm_ring, T, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), s.den, (const double*)ctx->d_scale, act,
-             ctx->N);
+template <typename T> int renorm_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
+  if (f32)
+    PSI_LAUNCH_P(k_renorm_ring, T, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), ctx->s32.den,
+                 (const double*)ctx->d_scale, act, ctx->N);
+  else
+    PSI_LAUNCH_P(k_renorm_ring, T, T, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), s.den, (const double*)ctx->d_scale,
+                 act, ctx->N);
   return EFFPOT_OK;
 }
 
@@ -561,7 +608,7 @@ template <typename T> int norm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
 }
 
 // RK-Gill step (steprk.f90:286-405).  Leaves the norm reduction in d_red/d_scale.
-template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
+template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   const double r2 = std::sqrt(2.0);
   const double arun[4] = {0.5, 1.0 - 1.0 / r2, 1.0 + 1.0 / r2, 1.0 / 6.0};
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
@@ -570,20 +617,20 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, double dt, con
     StageCoef c;
     c.arun = arun[j]; c.brun = brun[j]; c.crun = crun[j]; c.first = j == 0;
     RC(ring_op(ctx, s, j == 0 ? RING_BEFORE_RK0 : RING_BEFORE_RKJ, act));
-    RC((launch_step<T, 2>(ctx, s, dt, act, c)));
+    RC((launch_step<T, 2>(ctx, s, act, f32, c)));
     RC(ring_op(ctx, s, j == 0 ? RING_AFTER_RK0 : RING_AFTER_RKJ, act));
-    if (j <= 2) RC(poten_t(ctx, s, act));
+    if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
   }
   RC(reduce<2>(ctx, 1, act));
   return EFFPOT_OK;
 }
 
 // Hamming predictor-modifier-corrector (steppc.f90:35-159).
-template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, double dt, const int* act) {
-  RC((launch_step<T, 0>(ctx, s, dt, act)));
+template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
+  RC((launch_step<T, 0>(ctx, s, act, f32)));
   RC(ring_op(ctx, s, RING_AFTER_PRED, act));
-  RC(poten_t(ctx, s, act));
-  RC((launch_step<T, 1>(ctx, s, dt, act)));
+  RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
+  RC((launch_step<T, 1>(ctx, s, act, f32)));
   RC(ring_op(ctx, s, RING_AFTER_CORR, act));
   RC(reduce<2>(ctx, 1, act));
   return EFFPOT_OK;
@@ -693,7 +740,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   int rc;
   auto bail = [&](int r) { effpot_ctx_destroy(ctx); return r; };
   if ((rc = alloc_state(ctx, ctx->s64)) != EFFPOT_OK) return bail(rc);
-  if (ctx->mixed && (rc = alloc_state(ctx, ctx->s32)) != EFFPOT_OK) return bail(rc);
+  if (ctx->mixed && (rc = alloc_state(ctx, ctx->s32, true)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->imptab, (size_t)ctx->B * (ctx->npot + 2))) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_active, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -713,6 +760,10 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_red, (size_t)ctx->B * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_scale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_n4, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_dt, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->h_dt.assign(ctx->B, 0.0);
+  ctx->mask_dev.assign((size_t)ctx->B * 4, -1);
+  ctx->dt_dev.assign(ctx->B, -1.0);
   ctx->h_n4.assign(ctx->B, p->n4);
   if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
@@ -789,11 +840,8 @@ static int den_of_slot(effpot_ctx* ctx, int slot) {
   std::vector<int> m(ctx->B, 0);
   m[slot] = 1;
   HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
-  PSI_LAUNCH(k_den_from_cur, double, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s64), ctx->s64.den,
-             (const int*)ctx->d_mask_tmp, ctx->N);
-  if (ctx->mixed)
-    PSI_LAUNCH(k_den_from_cur, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, ctx->s32), ctx->s32.den,
-               (const int*)ctx->d_mask_tmp, ctx->N);
+  RC(den_from_cur(ctx, ctx->d_mask_tmp, false));
+  if (ctx->mixed) RC(den_from_cur(ctx, ctx->d_mask_tmp, true));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }
@@ -816,12 +864,7 @@ int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi) {
     }
     HIPCHK(hipMemcpy(dst, re.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice));
   }
-  if (ctx->mixed) {
-    RC(reset_slot(ctx, ctx->s32, slot));
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, ctx->st, (const double*)dst,
-                       (float*)(ctx->s32.psi + (long)slot * 5 * ctx->N), nval);
-    LAUNCHCHK();
-  }
+  (void)nval;
   return den_of_slot(ctx, slot);
 }
 
@@ -835,7 +878,6 @@ template <typename T> static int cur_index(effpot_ctx* ctx, State<T>& s, int slo
 
 // Which precision holds the live state of a slot: FP64 unless this is a mixed
 // context that has not been polished (the FP32 phase is the live one).
-static bool live32(effpot_ctx* ctx) { return ctx->mixed; }
 
 int effpot_download_psi(effpot_ctx* ctx, int slot, double* psi) {
   RC(check_ctx(ctx));
@@ -908,10 +950,18 @@ int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R, double a
   return effpot_upload_psi(ctx, slot, psi.data());
 }
 
+int effpot_set_dt(effpot_ctx* ctx, int slot, double dt) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B || dt < 0 || !std::isfinite(dt)) return ctx->fail(EFFPOT_ERR_DT, "bad slot time step");
+  ctx->h_dt[slot] = dt;
+  return EFFPOT_OK;
+}
+
 int effpot_set_natoms(effpot_ctx* ctx, int slot, double n4) {
   RC(check_ctx(ctx));
   if (slot < 0 || slot >= ctx->B || !(n4 > 0)) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or atom number");
   ctx->h_n4[slot] = n4;
+  HIPCHK(hipStreamSynchronize(ctx->st));  // kernels in flight read d_n4
   HIPCHK(hipMemcpy(ctx->d_n4 + slot, &n4, sizeof(double), hipMemcpyHostToDevice));
   return EFFPOT_OK;
 }
@@ -945,7 +995,6 @@ int effpot_poten(effpot_ctx* ctx) {
   RC(check_ctx(ctx));
   hipSetDevice(ctx->dev);
   RC(poten_t(ctx, ctx->s64, ctx->d_active));
-  if (ctx->mixed) RC(poten_t(ctx, ctx->s32, ctx->d_active));
   return EFFPOT_OK;
 }
 
@@ -960,14 +1009,16 @@ int effpot_step_rk(effpot_ctx* ctx, double dt) {
   RC(check_ctx(ctx));
   if (!(dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
   hipSetDevice(ctx->dev);
-  return step_rk_t(ctx, ctx->s64, dt, ctx->d_active);
+  RC(push_dt(ctx, dt));
+  return step_rk_t(ctx, ctx->s64, ctx->d_active);
 }
 
 int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he) {
   RC(check_ctx(ctx));
   if (!(dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
   hipSetDevice(ctx->dev);
-  RC(step_pc_t(ctx, ctx->s64, dt, ctx->d_active));
+  RC(push_dt(ctx, dt));
+  RC(step_pc_t(ctx, ctx->s64, ctx->d_active));
   if (err_he) {
     HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * ctx->B * 2, hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1047,62 +1098,62 @@ int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out) {
 
 // ---- continuous-batching engine ---------------------------------------------------
 // Device masks: 0 start, 1 rk, 2 pc, 3 energy check, 4 polish conversion.
+// Masks change only at phase transitions.  The tick does not wait for the
+// device, so a queued copy may still read the pinned staging buffer: wait for
+// the stream before rewriting it.
 static int push_mask(effpot_ctx* ctx, int k, const std::vector<int>& m) {
+  int* last = ctx->mask_dev.data() + (long)k * ctx->B;
+  if (std::equal(m.begin(), m.end(), last)) return EFFPOT_OK;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  std::copy(m.begin(), m.end(), last);
   int* h = ctx->h_mask + (long)k * ctx->B;
   std::memcpy(h, m.data(), sizeof(int) * ctx->B);
   HIPCHK(hipMemcpyAsync(ctx->d_mask + (long)k * ctx->B, h, sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
   return EFFPOT_OK;
 }
 
-// FP32 state of slot b -> FP64 live buffer with a fresh integrator history.
-static int widen_slot(effpot_ctx* ctx, int b) {
-  int idx;
-  RC(cur_index(ctx, ctx->s32, b, &idx));
-  RC(reset_slot(ctx, ctx->s64, b));
-  hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, ctx->st,
-                     (const float*)(ctx->s32.psi + ((long)b * 5 + idx) * ctx->N),
-                     (double*)(ctx->s64.psi + (long)b * 5 * ctx->N), ctx->cplx ? 2 * ctx->N : ctx->N);
-  LAUNCHCHK();
-  return EFFPOT_OK;
-}
-
 // One predictor-corrector step + renormalisation of the slots in mask dP.  The
 // launch sequence only references device memory (rings, masks, fields), so it
-// is captured once per precision and time step and replayed as a HIP graph.
+// is captured once per precision and replayed as a HIP graph (the time step of
+// each slot is read from device memory, so changing it needs no re-capture).
 template <typename T>
-static int pc_step_graph(effpot_ctx* ctx, State<T>& s, int which, double dt, const int* dP) {
+static int pc_step_graph(effpot_ctx* ctx, State<T>& s, bool f32, const int* dP) {
+  const int which = f32 ? 1 : 0;
   if (!ctx->graphs) {
-    RC(step_pc_t(ctx, s, dt, dP));
-    return renorm_t(ctx, s, dP);
-  }
-  if (ctx->pc_exec[which] && ctx->pc_dt[which] != dt) {
-    HIPCHK(hipGraphExecDestroy(ctx->pc_exec[which]));
-    ctx->pc_exec[which] = nullptr;
+    ctx->corr_den = false;
+    int rc = step_pc_t(ctx, s, dP, f32);
+    ctx->corr_den = true;
+    RC(rc);
+    return renorm_t(ctx, s, dP, f32);
   }
   if (!ctx->pc_exec[which]) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    int rc = step_pc_t(ctx, s, dt, dP);
-    if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, dP);
+    ctx->corr_den = false;
+    int rc = step_pc_t(ctx, s, dP, f32);
+    ctx->corr_den = true;
+    if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, dP, f32);
     hipError_t e = hipStreamEndCapture(ctx->st, &g);
     if (rc != EFFPOT_OK) return rc;
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&ctx->pc_exec[which], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-    ctx->pc_dt[which] = dt;
   }
   HIPCHK(hipGraphLaunch(ctx->pc_exec[which], ctx->st));
   return EFFPOT_OK;
 }
 
-template <typename T>
-static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve_opts* opt, effpot_slot* sl) {
+// Advance the slots whose field precision matches f32 (FP32 mean fields of a
+// MIXED context, else FP64) by one step of their own phase.  The wavefunction,
+// its history and the integrator always live in the FP64 state.
+static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, effpot_slot* sl) {
+  State<double>& s = ctx->s64;
   const int B = ctx->B;
   std::vector<int> ms(B, 0), mr(B, 0), mp(B, 0);
   bool any_s = false, any_r = false, any_p = false;
   for (int b = 0; b < B; b++) {
-    if (!ctx->h_active[b] || ((sl[b].fp32 != 0) != is32)) continue;
+    if (!ctx->h_active[b] || ((sl[b].fp32 != 0) != f32)) continue;
     if (sl[b].phase == EFFPOT_SLOT_START) { ms[b] = 1; any_s = true; }
     else if (sl[b].phase == EFFPOT_SLOT_RK) { mr[b] = 1; any_r = true; }
     else if (sl[b].phase == EFFPOT_SLOT_PC) { mp[b] = 1; any_p = true; }
@@ -1112,15 +1163,13 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
   const int* dP = ctx->d_mask + 2 * B;
   const int* dE = ctx->d_mask + 3 * B;
   std::vector<effpot_energies> e(B);
-  const double tol = is32 ? opt->tol_fp32 : opt->tol;
-  const int cap = is32 || !ctx->mixed ? opt->max_steps : opt->max_steps + std::max(opt->polish_steps, 0);
-  // START: poten of the new states; they take their first RK step below.
+  const double tol = f32 ? opt->tol_fp32 : opt->tol;
+  const int cap = f32 || !ctx->mixed ? opt->max_steps : opt->max_steps + std::max(opt->polish_steps, 0);
+  // START: fields of the new states; they take their first RK step below.
   if (any_s) {
     RC(push_mask(ctx, 0, ms));
-    if (ctx->mixed && !is32) {
-      PSI_LAUNCH(k_den_from_cur, double, pw_grid(ctx->N, B), ring_of(ctx, ctx->s64), ctx->s64.den, dS, ctx->N);
-    }
-    RC(poten_t(ctx, s, dS));
+    RC(den_from_cur(ctx, dS, f32));
+    RC(f32 ? poten_t(ctx, ctx->s32, dS) : poten_t(ctx, s, dS));
     for (int b = 0; b < B; b++)
       if (ms[b]) {
         sl[b].nrk = 0; sl[b].since = 0;
@@ -1132,8 +1181,8 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
   bool any_e = false;
   if (any_r) {
     RC(push_mask(ctx, 1, mr));
-    RC(step_rk_t(ctx, s, opt->dt, dR));
-    RC(renorm_t(ctx, s, dR));
+    RC(step_rk_t(ctx, s, dR, f32));
+    RC(renorm_t(ctx, s, dR, f32));
     for (int b = 0; b < B; b++)
       if (mr[b]) {
         sl[b].nrk++; sl[b].steps++;
@@ -1142,7 +1191,7 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
   }
   if (any_p) {
     RC(push_mask(ctx, 2, mp));
-    RC(pc_step_graph(ctx, s, is32 ? 1 : 0, opt->dt, dP));
+    RC(pc_step_graph(ctx, s, f32, dP));
     for (int b = 0; b < B; b++)
       if (mp[b]) {
         sl[b].steps++; sl[b].since++;
@@ -1150,11 +1199,13 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
       }
   }
   if (any_e) {
+    // energy checks always in FP64: density of the current psi, FP64 poten, energy
     RC(push_mask(ctx, 3, me));
+    if (f32) RC(den_from_cur(ctx, dE, false));
     if (opt->check_finite) {
       HIPCHK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int) * B, ctx->st));
-      hipLaunchKernelGGL((k_check_den<T>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const T*)s.den, ctx->N, ctx->N,
-                         dE, ctx->d_bad);
+      hipLaunchKernelGGL((k_check_den<double>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const double*)s.den,
+                         ctx->N, ctx->N, dE, ctx->d_bad);
       LAUNCHCHK();
     }
     RC(poten_t(ctx, s, dE));
@@ -1175,9 +1226,8 @@ static int tick_prec(effpot_ctx* ctx, State<T>& s, bool is32, const effpot_solve
       double rel = std::fabs(e[b].etot - q.eprev) / std::fabs(e[b].etot) / (double)std::max(q.since, 1);
       q.rel_de = rel; q.eprev = e[b].etot; q.e = e[b]; q.since = 0;
       if (rel < tol || q.steps >= cap) {
-        if (is32) {
-          RC(widen_slot(ctx, b));
-          q.fp32 = 0; q.phase = EFFPOT_SLOT_START;
+        if (f32) {
+          q.fp32 = 0;  // FP64 polish: same state and history, FP64 fields from here on
         } else {
           q.phase = EFFPOT_SLOT_DONE; q.converged = rel < tol;
         }
@@ -1199,21 +1249,26 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
       return ctx->fail(EFFPOT_ERR_STATE, "impurity not set for slot " + std::to_string(b));
     if (!ctx->mixed) slots[b].fp32 = 0;
   }
-  if (ctx->mixed) RC(tick_prec(ctx, ctx->s32, true, opt, slots));
-  RC(tick_prec(ctx, ctx->s64, false, opt, slots));
-  HIPCHK(hipStreamSynchronize(ctx->st));
+  RC(push_dt(ctx, opt->dt));
+  if (ctx->mixed) RC(tick_phase(ctx, true, opt, slots));
+  RC(tick_phase(ctx, false, opt, slots));
+  // No wait here: the slot phases are decided on the host and only energy
+  // checks read the device (they wait themselves), so the host queues the
+  // next steps while the device runs these.
+  HIPCHK(hipGetLastError());
   return EFFPOT_OK;
 }
 
-template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int which, int reps, double* ms) {
+template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int which, int reps, double* ms,
+                                               bool f32 = false) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   auto once = [&]() -> int {
     switch (which) {
-      case 0: return launch_step<T, 0>(ctx, s, 1e-4, ctx->d_active);
-      case 1: return launch_step<T, 1>(ctx, s, 1e-4, ctx->d_active);
-      case 2: return poten_t(ctx, s, ctx->d_active);
+      case 0: return launch_step<T, 0>(ctx, s, ctx->d_active, f32);
+      case 1: return launch_step<T, 1>(ctx, s, ctx->d_active, f32);
+      case 2: return f32 ? poten_t(ctx, ctx->s32, ctx->d_active) : poten_t(ctx, s, ctx->d_active);
       default: return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
     }
   };
@@ -1233,6 +1288,7 @@ template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
   RC(check_ctx(ctx));
   hipSetDevice(ctx->dev);
+  RC(push_dt(ctx, 1e-4));
   int best_tx = ctx->tx, best_zc = ctx->zc;
   double best = 1e30;
   for (int tx : {64, 32, 16, 8}) {
@@ -1241,13 +1297,9 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
       if (zc > ctx->nz) continue;
       ctx->tx = tx; ctx->zc = zc;
       double t0 = 0, t1 = 0, t2 = 0;
-      if (ctx->mixed) {
-        RC(time_kernel_t(ctx, ctx->s32, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s32, 1, reps, &t1));
-        RC(time_kernel_t(ctx, ctx->s32, 2, reps, &t2));
-      } else {
-        RC(time_kernel_t(ctx, ctx->s64, 0, reps, &t0)); RC(time_kernel_t(ctx, ctx->s64, 1, reps, &t1));
-        RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t2));
-      }
+      RC(time_kernel_t(ctx, ctx->s64, 0, reps, &t0, ctx->mixed));
+      RC(time_kernel_t(ctx, ctx->s64, 1, reps, &t1, ctx->mixed));
+      RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t2, ctx->mixed));
       if (ctx->tune_log) std::fprintf(stderr, "autotune tx=%d zc=%d pred=%.4f corr=%.4f poten=%.4f ms\n", tx, zc, t0, t1, t2);
       if (t0 + t1 + t2 < best) { best = t0 + t1 + t2; best_tx = tx; best_zc = zc; }
     }
@@ -1276,8 +1328,8 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
   RC(check_ctx(ctx));
   if (!ms_per_launch || reps < 1) return ctx->fail(EFFPOT_ERR_ARG, "bad arguments");
   hipSetDevice(ctx->dev);
-  if (ctx->mixed) return time_kernel_t(ctx, ctx->s32, which, reps, ms_per_launch);
-  return time_kernel_t(ctx, ctx->s64, which, reps, ms_per_launch);
+  RC(push_dt(ctx, 1e-4));
+  return time_kernel_t(ctx, ctx->s64, which, reps, ms_per_launch, ctx->mixed);
 }
 
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {
@@ -1307,5 +1359,6 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
     out[b].rel_de = sl[b].rel_de;
     if (sl[b].phase == EFFPOT_SLOT_FAILED) rc = ctx->fail(EFFPOT_ERR_NONFINITE, "non-finite density or energy (slot " + std::to_string(b) + ")");
   }
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return rc;
 }

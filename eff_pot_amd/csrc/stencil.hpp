@@ -13,6 +13,11 @@
 // in LDS with a 6-point halo in x and y for the x/y derivatives.  Boundary
 // mode icon 13 reads zeros outside the grid (the redef.f:133-170 boundary
 // rows), icon 8 wraps periodically.
+//
+// The march is software-pipelined: while plane z is differentiated and its
+// epilogue runs, the loads of plane z+1 (the new z-neighbour, the halo and the
+// pointwise operands the epilogue will need) are already in flight, and the
+// LDS plane is double-buffered so one barrier per plane suffices.
 #pragma once
 #include "common.hpp"
 
@@ -27,55 +32,124 @@ struct Grid {
 // Scaled 13-point coefficients for each axis (c_m / h^id), first and second order.
 template <typename T> struct StenCoef { T d1[3][13]; T d2[3][13]; };
 
-template <typename V> EP_DEV V gload(const V* f, int x, int y, int z, const Grid& g) {
+// Offset of column (x, y) within a plane, wrapped for icon 8; -1 outside the
+// grid for icon 13.
+EP_DEV long colofs(int x, int y, const Grid& g) {
   if (g.periodic) {
-    x = (x + g.nx) % g.nx; y = (y + g.ny) % g.ny; z = (z + g.nz) % g.nz;
-  } else if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) {
-    return vzero((V*)nullptr);
+    x = (x + g.nx) % g.nx; y = (y + g.ny) % g.ny;
+  } else if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) {
+    return -1;
   }
-  return f[g.idx(x, y, z)];
+  return (long)x + (long)g.nx * y;
+}
+// f at column offset c (from colofs) and plane z, with the boundary rule in z.
+template <typename V> EP_DEV V ldz(const V* f, long c, int z, const Grid& g) {
+  if (c < 0) return vzero((V*)nullptr);
+  if (z < 0 || z >= g.nz) {
+    if (!g.periodic) return vzero((V*)nullptr);
+    z = (z + g.nz) % g.nz;
+  }
+  return f[c + (long)g.nx * g.ny * z];
 }
 
 template <int TX> struct Tile {
-  static constexpr int TY = kThreads / TX, LX = TX + 12, LY = TY + 12;
+  static constexpr int TY = kThreads / TX, LX = TX + 12, LY = TY + 12, PL = LX * LY;
+  static constexpr int H = 12 * (TX + TY);              // halo points of a plane (no corners)
+  static constexpr int NH = (H + kThreads - 1) / kThreads;  // halo loads per thread
 };
 
-// The z-march.  epi(x, y, z, inside, v, d1[3], d2[3]) is called for every
-// plane by every thread (inside=false for threads past the grid edge, which
-// must still take part in barriers).  Threads past the edge still load their
-// (zero or wrapped) values: they fill the LDS centre rows that inside
-// threads read as halo.
-template <typename T, typename V, int TX, bool D1, bool D2, class Epi>
-EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc, int z0, int z1, V* lds,
-                   Epi&& epi) {
-  constexpr int TY = Tile<TX>::TY, LX = Tile<TX>::LX, LY = Tile<TX>::LY;
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int ntx = (g.nx + TX - 1) / TX;
-  const int bx = blockIdx.x % ntx, by = blockIdx.x / ntx;
-  const int x = bx * TX + tx, y = by * TY + ty;
-  const bool inside = x < g.nx && y < g.ny;
-  V q[13];
+// Workgroup -> tile order.  Workgroup i runs on XCD i % 8 and each XCD has its
+// own L2; giving every XCD a contiguous range of tiles keeps the neighbours
+// whose planes a tile reads as halo in the same L2.
+EP_DEV int xcd_order(int i, int G) { return (G & 7) ? i : (i & 7) * (G >> 3) + (i >> 3); }
+
+// Column tile and z chunk of this workgroup (blockIdx.z is the slot).
+template <int TX> struct TilePos {
+  int bx, by, z0, z1, tx, ty, x, y;
+  bool inside;
+  EP_DEV TilePos(const Grid& g, int zc) {
+    constexpr int TY = Tile<TX>::TY;
+    const int L = xcd_order(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = L % gridDim.x, chunk = L / gridDim.x;
+    const int ntx = (g.nx + TX - 1) / TX;
+    bx = tile % ntx; by = tile / ntx;
+    z0 = chunk * zc; z1 = min(z0 + zc, g.nz);
+    tx = threadIdx.x % TX; ty = threadIdx.x / TX;
+    x = bx * TX + tx; y = by * TY + ty;
+    inside = x < g.nx && y < g.ny;
+  }
+};
+
+// Halo point h of a tile plane -> LDS offset (lo) and global column (go).
+// Points 0 .. 12 TY-1 are the x strips (6 columns left and right of the
+// tile), the rest the y strips (6 rows below and above).
+template <int TX> EP_DEV void halo_point(int h, int bx, int by, const Grid& g, int& lo, long& go) {
+  constexpr int TY = Tile<TX>::TY, LX = Tile<TX>::LX;
+  if (h >= Tile<TX>::H) { lo = -1; go = -1; return; }
+  int lx, ly;
+  if (h < 12 * TY) {
+    const int r = h / 12, c = h - 12 * (h / 12);
+    ly = 6 + r; lx = c < 6 ? c : TX + c;
+  } else {
+    const int k = h - 12 * TY, r = k / TX, c = k - TX * (k / TX);
+    ly = r < 6 ? r : TY + r; lx = 6 + c;
+  }
+  lo = ly * LX + lx;
+  go = colofs(bx * TX + lx - 6, by * TY + ly - 6, g);
+}
+
+struct NoPre {};
+
+// The z-march.  pre(i) loads the pointwise operands of grid point i (called
+// for inside threads only, one plane ahead); epi(x, y, z, inside, v, d1, d2,
+// pre_value) is called for every plane by every thread (inside=false for
+// threads past the grid edge).  Threads past the edge still load their (zero
+// or wrapped) column: it is LDS halo of inside threads.  lds holds 2 planes.
+template <typename T, typename V, int TX, bool D1, bool D2, class Pre, class Epi>
+EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc, const TilePos<TX>& tp, V* lds,
+                   Pre&& pre, Epi&& epi) {
+  constexpr int LX = Tile<TX>::LX, PL = Tile<TX>::PL, NH = Tile<TX>::NH;
+  const int tx = tp.tx, ty = tp.ty, z0 = tp.z0, z1 = tp.z1;
+  if (z0 >= z1) return;  // uniform over the group
+  const long col = colofs(tp.x, tp.y, g);
+  const long plane = (long)g.nx * g.ny;
+  int lo[NH];
+  long go[NH];
+#pragma unroll
+  for (int k = 0; k < NH; k++) halo_point<TX>(threadIdx.x + k * kThreads, tp.bx, tp.by, g, lo[k], go[k]);
+  V q[13], hv[NH];
   const V zero = vzero((V*)nullptr);
 #pragma unroll
-  for (int k = 0; k < 12; k++) q[k] = gload(f, x, y, z0 - 6 + k, g);
+  for (int k = 0; k < 13; k++) q[k] = ldz(f, col, z0 - 6 + k, g);
+#pragma unroll
+  for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], z0, g);
+  using PV = decltype(pre(0L));
+  PV p0{};
+  if (tp.inside) p0 = pre(col + plane * z0);
+  lds[(ty + 6) * LX + tx + 6] = q[6];
+#pragma unroll
+  for (int k = 0; k < NH; k++)
+    if (lo[k] >= 0) lds[lo[k]] = hv[k];
+  __syncthreads();
+  int buf = 0;
   for (int z = z0; z < z1; z++) {
-    q[12] = gload(f, x, y, z + 6, g);
-    __syncthreads();
-    lds[(ty + 6) * LX + tx + 6] = q[6];
-    for (int e = threadIdx.x; e < LY * LX; e += kThreads) {
-      const int ly = e / LX, lx = e - ly * LX;
-      const bool inx = lx >= 6 && lx < TX + 6, iny = ly >= 6 && ly < TY + 6;
-      if (inx == iny) continue;  // centre (own thread) or unused corner
-      lds[e] = gload(f, bx * TX + lx - 6, by * TY + ly - 6, z, g);
+    const bool more = z + 1 < z1;
+    V qn = zero;
+    PV p1{};
+    if (more) {
+      qn = ldz(f, col, z + 7, g);
+#pragma unroll
+      for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], z + 1, g);
+      if (tp.inside) p1 = pre(col + plane * (z + 1));
     }
-    __syncthreads();
+    const V* pl = lds + buf * PL;
     V d1[3], d2[3];
     if constexpr (D1) {
       V ax = zero, ay = zero, az = zero;
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax = vfma(sc.d1[0][m], lds[(ty + 6) * LX + tx + m], ax);
-        ay = vfma(sc.d1[1][m], lds[(ty + m) * LX + tx + 6], ay);
+        ax = vfma(sc.d1[0][m], pl[(ty + 6) * LX + tx + m], ax);
+        ay = vfma(sc.d1[1][m], pl[(ty + m) * LX + tx + 6], ay);
         az = vfma(sc.d1[2][m], q[m], az);
       }
       d1[0] = ax; d1[1] = ay; d1[2] = az;
@@ -84,15 +158,26 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
       V ax = zero, ay = zero, az = zero;
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax = vfma(sc.d2[0][m], lds[(ty + 6) * LX + tx + m], ax);
-        ay = vfma(sc.d2[1][m], lds[(ty + m) * LX + tx + 6], ay);
+        ax = vfma(sc.d2[0][m], pl[(ty + 6) * LX + tx + m], ax);
+        ay = vfma(sc.d2[1][m], pl[(ty + m) * LX + tx + 6], ay);
         az = vfma(sc.d2[2][m], q[m], az);
       }
       d2[0] = ax; d2[1] = ay; d2[2] = az;
     }
-    epi(x, y, z, inside, q[6], d1, d2);
+    epi(tp.x, tp.y, z, tp.inside, q[6], d1, d2, p0);
+    if (more) {
+      V* nl = lds + (buf ^ 1) * PL;
+      nl[(ty + 6) * LX + tx + 6] = q[7];
+#pragma unroll
+      for (int k = 0; k < NH; k++)
+        if (lo[k] >= 0) nl[lo[k]] = hv[k];
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 12; k++) q[k] = q[k + 1];
+    q[12] = qn;
+    p0 = p1;
+    buf ^= 1;
   }
 }
 
@@ -117,12 +202,15 @@ template <typename T> EP_DEV T dotre(T a, T b) { return a * b; }
 
 // V = cx<T> (complex psi) or T (real psi: imaginary time from a real state
 // keeps the imaginary part exactly zero, so the real path is the same
-// arithmetic on half the bytes).
-template <typename T, typename V> struct StepArgs {
+// arithmetic on half the bytes, up to the compiler's FMA contraction).
+// P: precision of the fields the step reads/writes (pot4, uimp, den): T, or
+// float when a FP64 state is driven by FP32 mean fields.
+template <typename T, typename V, typename P> struct StepArgs {
   Grid g;
   StenCoef<T> sc;
   int zc;
-  T h2o2m4, dt;
+  T h2o2m4;
+  const double* dtv;  // time step of each slot
   const int* active;
   long slot;  // elements per slot (N)
   const V* cur;   // psi read through the stencil
@@ -131,72 +219,77 @@ template <typename T, typename V> struct StepArgs {
   V* h1; V* h2;         // hpsi history hpsiold(ioldh(1..2))
   V* pc; V* q;          // slot stride of pc/q is `pcq_slot` elements of V
   long pcq_slot;
-  const T* pot4; const T* uimp;
-  T* den;
+  const P* pot4; const P* uimp;
+  P* den;
   double* partial;  // [slot][block][2]: sum c9|pc| (or 0), sum |psi_new|^2
   int nblk;
   // RK-Gill stage coefficients
   T arun, brun, crun;
   int first_stage;
+  int write_den;  // corrector: also store the density (no renormalisation follows)
 };
 
 // Hamming predictor + modifier (steppc.f90:53-64).  out = o3's buffer.
-template <typename T, typename V, int TX>
-EP_DEV void k_pc_predict_body(const StepArgs<T, V>& a) {
+template <typename T, typename V, typename P, int TX>
+EP_DEV void k_pc_predict_body(const StepArgs<T, V, P>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[2 * Tile<TX>::PL];
   const long so = (long)s * a.slot;
   const V* cur = a.cur + so;
-  V* o3 = a.o3 + so; V* h1 = a.h1 + so; V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
-  const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
-  const T c4o3dt = T(4.0 / 3.0) * a.dt, c112 = T(112.0 / 121.0);
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
+  V* o3 = a.o3 + so; const V* h1 = a.h1 + so; V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
+  const P* pot4 = a.pot4 + so; const P* uimp = a.uimp + so; P* den = a.den + so;
+  const T c4o3dt = T(4.0 / 3.0) * (T)a.dtv[s], c112 = T(112.0 / 121.0);
+  struct Op { V o3, h1, h2, pc; T p4, u; };
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
+    [&](long i) { return Op{o3[i], h1[i], h2[i], pc[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      V pred = o3[i] + c4o3dt * ((T(2) * hp - h1[i]) + T(2) * h2[i]);
-      V pcv = pc[i];
-      V pn = pred - c112 * pcv;
+      V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      V pred = o.o3 + c4o3dt * ((T(2) * hp - o.h1) + T(2) * o.h2);
+      V pn = pred - c112 * o.pc;
       o3[i] = pn;  // modified psi goes to the retired psi_{n-3} buffer
       pc[i] = pred;
       h2[i] = hp;
-      den[i] = norm2(pn);
+      den[i] = (P)norm2(pn);
     });
 }
 
 // Hamming corrector + final value (steppc.f90:135-151), errHe and norm partials.
 // Here o3 holds psi_n (psiold(ioldp(3)) after the modifier), o2 psi_{n-2},
-// h1 = H psi_n, h2 = H psi_{n-1} (after the ioldh swap).
-template <typename T, typename V, int TX>
-EP_DEV void k_pc_correct_body(const StepArgs<T, V>& a) {
+// h1 = H psi_n, h2 = H psi_{n-1} (after the ioldh swap).  The density is
+// written only when no renormalisation (which rewrites it) follows.
+template <typename T, typename V, typename P, int TX>
+EP_DEV void k_pc_correct_body(const StepArgs<T, V, P>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[2 * Tile<TX>::PL];
   __shared__ double red[8];
   const long so = (long)s * a.slot;
   const V* cur = a.cur + so;
   V* out = a.out + so;
   const V* o3 = a.o3 + so; const V* o2 = a.o2 + so;
   const V* h1 = a.h1 + so; const V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
-  const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
-  const T c9 = T(9.0 / 121.0), dt3 = T(3) * a.dt;
+  const P* pot4 = a.pot4 + so; const P* uimp = a.uimp + so; P* den = a.den + so;
+  const T c9 = T(9.0 / 121.0), dt3 = T(3) * (T)a.dtv[s];
+  const bool wden = a.write_den;
   double v[2] = {0.0, 0.0};
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
+  struct Op { V o3, o2, h1, h2, pc; T p4, u; };
+  const TilePos<TX> tp(a.g, a.zc);
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, tp, lds,
+    [&](long i) { return Op{o3[i], o2[i], h1[i], h2[i], pc[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      V corr = T(0.125) * ((T(9) * o3[i] - o2[i]) + dt3 * ((hp + T(2) * h1[i]) - h2[i]));
-      V pcv = pc[i] - corr;
+      V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      V corr = T(0.125) * ((T(9) * o.o3 - o.o2) + dt3 * ((hp + T(2) * o.h1) - o.h2));
+      V pcv = o.pc - corr;
       V pn = corr + c9 * pcv;
       pc[i] = pcv;
       out[i] = pn;
       T d = norm2(pn);
-      den[i] = d;
+      if (wden) den[i] = (P)d;
       v[0] += (double)(c9 * sqrt(norm2(pcv)));
       v[1] += (double)d;
     });
@@ -211,31 +304,32 @@ EP_DEV void k_pc_correct_body(const StepArgs<T, V>& a) {
 // One Runge-Kutta-Gill stage (steprk.f90:301-316).  At the first stage the host
 // has rotated the history so that o1 := cur (psi at the start) and h1 is the
 // retired buffer that receives H psi.
-template <typename T, typename V, int TX>
-EP_DEV void k_rk_stage_body(const StepArgs<T, V>& a) {
+template <typename T, typename V, typename P, int TX>
+EP_DEV void k_rk_stage_body(const StepArgs<T, V, P>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[2 * Tile<TX>::PL];
   __shared__ double red[8];
   const long so = (long)s * a.slot;
   const V* cur = a.cur + so;
   V* out = a.out + so; V* h1 = a.h1 + so; V* q = a.q + (long)s * a.pcq_slot;
-  const T* pot4 = a.pot4 + so; const T* uimp = a.uimp + so; T* den = a.den + so;
+  const T dt = (T)a.dtv[s];
+  const P* pot4 = a.pot4 + so; const P* uimp = a.uimp + so; P* den = a.den + so;
   double v[2] = {0.0, 0.0};
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3]) {
+  struct Op { V q; T p4, u; };
+  zmarch<T, V, TX, false, true>(cur, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
+    [&](long i) { return Op{q[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      V hp = hpsi_of(psi, d2, a.h2o2m4, pot4[i], uimp[i]);
-      V qv = q[i];
-      V a2 = a.arun * (hp - a.brun * qv);
-      q[i] = (qv + T(3) * a2) - a.crun * hp;
+      V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      V a2 = a.arun * (hp - a.brun * o.q);
+      q[i] = (o.q + T(3) * a2) - a.crun * hp;
       if (a.first_stage) h1[i] = hp;
-      V pn = psi + a.dt * a2;
+      V pn = psi + dt * a2;
       out[i] = pn;
       T d = norm2(pn);
-      den[i] = d;
+      den[i] = (P)d;
       v[1] += (double)d;
     });
   block_sum<2>(v, red);
@@ -266,22 +360,22 @@ template <typename T, int TX>
 __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ T lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ T lds[2 * Tile<TX>::PL];
   const long so = (long)s * a.slot;
   const T* den = a.den + so; const T* dencg = a.dencg + so; const T* denalf = a.denalf + so;
   T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
   T *s0 = a.s0 + so, *s1 = a.s1 + so, *s2 = a.s2 + so, *s3 = a.s3 + so;
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, T, TX, true, false>(den, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, T d, T(&d1)[3], T(&)[3]) {
+  const bool otc = a.otc;
+  struct Op { T cg, fa; };
+  zmarch<T, T, TX, true, false>(den, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
+    [&](long i) { return Op{dencg[i], otc ? denalf[i] : T(0)}; },
+    [&](int x, int y, int z, bool in, T d, T(&d1)[3], T(&)[3], const Op& o) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
       dx[i] = d1[0]; dy[i] = d1[1]; dz[i] = d1[2];
-      T cg = dencg[i];
-      s0[i] = d * cg * (a.cp4 + a.cpp4 * cg);
-      if (a.otc) {
-        T fa = denalf[i];  // holds falfs
-        s1[i] = fa * d1[0]; s2[i] = fa * d1[1]; s3[i] = fa * d1[2];
+      s0[i] = d * o.cg * (a.cp4 + a.cpp4 * o.cg);
+      if (otc) {  // o.fa holds falfs
+        s1[i] = o.fa * d1[0]; s2[i] = o.fa * d1[1]; s3[i] = o.fa * d1[2];
       }
     });
 }
@@ -303,60 +397,96 @@ template <typename T> struct DivArgs {
 template <typename T, int TX>
 __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   constexpr int TY = Tile<TX>::TY, LX = Tile<TX>::LX, LY = Tile<TX>::LY;
+  constexpr int NXS = TY * LX, NYS = LY * TX;  // x strip (x halo), y strip (y halo) of a plane
+  constexpr int KX = (NXS + kThreads - 1) / kThreads, KY = (NYS + kThreads - 1) / kThreads;
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ T lx_[TY * LX];  // falfs*intx with x halo
-  __shared__ T ly_[LY * TX];  // falfs*inty with y halo
+  __shared__ T lx_[2 * NXS];  // falfs*intx
+  __shared__ T ly_[2 * NYS];  // falfs*inty
   const long so = (long)s * a.slot;
   const Grid& g = a.g;
-  const T* denalf = a.denalf + so;
+  const T* fa = a.denalf + so;  // denalf holds falfs
   const T *ix = a.ix + so, *iy = a.iy + so, *iz = a.iz + so;
   const T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
   T *src = a.src + so, *div = a.div + so;
-  auto gfa = [&](const T* comp, int x, int y, int z) -> T {
-    if (g.periodic) {
-      x = (x + g.nx) % g.nx; y = (y + g.ny) % g.ny; z = (z + g.nz) % g.nz;
-    } else if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) {
-      return T(0);
-    }
-    long i = g.idx(x, y, z);
-    return denalf[i] * comp[i];  // denalf holds falfs
-  };
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int ntx = (g.nx + TX - 1) / TX;
-  const int bx = blockIdx.x % ntx, by = blockIdx.x / ntx;
-  const int x = bx * TX + tx, y = by * TY + ty;
-  const bool inside = x < g.nx && y < g.ny;
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, g.nz);
-  T q[13];
+  const TilePos<TX> tp(g, a.zc);
+  if (tp.z0 >= tp.z1) return;
+  const int tx = tp.tx, ty = tp.ty;
+  const long plane = (long)g.nx * g.ny;
+  const long col = colofs(tp.x, tp.y, g);
+  long cx_[KX], cy_[KY];
 #pragma unroll
-  for (int k = 0; k < 12; k++) q[k] = gfa(iz, x, y, z0 - 6 + k);
-  for (int z = z0; z < z1; z++) {
-    q[12] = gfa(iz, x, y, z + 6);
-    __syncthreads();
-    for (int e = threadIdx.x; e < TY * LX; e += kThreads) {
-      const int r = e / LX, c = e - r * LX;
-      lx_[e] = gfa(ix, bx * TX + c - 6, by * TY + r, z);
+  for (int k = 0; k < KX; k++) {
+    const int e = threadIdx.x + k * kThreads, r = e / LX, c = e - LX * (e / LX);
+    cx_[k] = e < NXS ? colofs(tp.bx * TX + c - 6, tp.by * TY + r, g) : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < KY; k++) {
+    const int e = threadIdx.x + k * kThreads, r = e / TX, c = e - TX * (e / TX);
+    cy_[k] = e < NYS ? colofs(tp.bx * TX + c, tp.by * TY + r - 6, g) : -1;
+  }
+  auto prod = [&](const T* comp, long c, int z) { return ldz(fa, c, z, g) * ldz(comp, c, z, g); };
+  struct Op { T dx, dy, dz, ix, iy, iz; };
+  auto pre = [&](long i) { return Op{dx[i], dy[i], dz[i], ix[i], iy[i], iz[i]}; };
+  T q[13], hx[KX], hy[KY];
+#pragma unroll
+  for (int k = 0; k < 13; k++) q[k] = prod(iz, col, tp.z0 - 6 + k);
+#pragma unroll
+  for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], tp.z0);
+#pragma unroll
+  for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], tp.z0);
+  Op p0{};
+  if (tp.inside) p0 = pre(col + plane * tp.z0);
+#pragma unroll
+  for (int k = 0; k < KX; k++)
+    if (threadIdx.x + k * kThreads < NXS) lx_[threadIdx.x + k * kThreads] = hx[k];
+#pragma unroll
+  for (int k = 0; k < KY; k++)
+    if (threadIdx.x + k * kThreads < NYS) ly_[threadIdx.x + k * kThreads] = hy[k];
+  __syncthreads();
+  int buf = 0;
+  for (int z = tp.z0; z < tp.z1; z++) {
+    const bool more = z + 1 < tp.z1;
+    T qn = T(0);
+    Op p1{};
+    if (more) {
+      qn = prod(iz, col, z + 7);
+#pragma unroll
+      for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], z + 1);
+#pragma unroll
+      for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], z + 1);
+      if (tp.inside) p1 = pre(col + plane * (z + 1));
     }
-    for (int e = threadIdx.x; e < LY * TX; e += kThreads) {
-      const int r = e / TX, c = e - r * TX;
-      ly_[e] = gfa(iy, bx * TX + c, by * TY + r - 6, z);
-    }
-    __syncthreads();
-    if (inside) {
+    if (tp.inside) {
+      const T* px = lx_ + buf * NXS;
+      const T* py = ly_ + buf * NYS;
       T ax = T(0), ay = T(0), az = T(0);
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax += a.sc.d1[0][m] * lx_[ty * LX + tx + m];
-        ay += a.sc.d1[1][m] * ly_[(ty + m) * TX + tx];
+        ax += a.sc.d1[0][m] * px[ty * LX + tx + m];
+        ay += a.sc.d1[1][m] * py[(ty + m) * TX + tx];
         az += a.sc.d1[2][m] * q[m];
       }
-      const long i = g.idx(x, y, z);
+      const long i = g.idx(tp.x, tp.y, z);
       div[i] = (ax + ay) + az;
-      src[i] = (dx[i] * ix[i] + dy[i] * iy[i]) + dz[i] * iz[i];
+      src[i] = (p0.dx * p0.ix + p0.dy * p0.iy) + p0.dz * p0.iz;
     }
+    if (more) {
+      T* nx_ = lx_ + (buf ^ 1) * NXS;
+      T* ny_ = ly_ + (buf ^ 1) * NYS;
+#pragma unroll
+      for (int k = 0; k < KX; k++)
+        if (threadIdx.x + k * kThreads < NXS) nx_[threadIdx.x + k * kThreads] = hx[k];
+#pragma unroll
+      for (int k = 0; k < KY; k++)
+        if (threadIdx.x + k * kThreads < NYS) ny_[threadIdx.x + k * kThreads] = hy[k];
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 12; k++) q[k] = q[k + 1];
+    q[12] = qn;
+    p0 = p1;
+    buf ^= 1;
   }
 }
 
@@ -383,16 +513,15 @@ template <typename T, typename V, int TX>
 __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
-  __shared__ V lds[Tile<TX>::LX * Tile<TX>::LY];
+  __shared__ V lds[2 * Tile<TX>::PL];
   __shared__ double red[4 * kEnergySums];
   const long so = (long)s * a.slot;
   const V* psi = reinterpret_cast<const V*>(a.psi_all + ((long)s * 5 + a.ring[8 * s]) * a.slot);
   double v[kEnergySums];
 #pragma unroll
   for (int k = 0; k < kEnergySums; k++) v[k] = 0.0;
-  const int z0 = blockIdx.y * a.zc, z1 = min(z0 + a.zc, a.g.nz);
-  zmarch<T, V, TX, true, true>(psi, a.g, a.sc, z0, z1, lds,
-    [&](int x, int y, int z, bool in, V p, V(&d1)[3], V(&d2)[3]) {
+  zmarch<T, V, TX, true, true>(psi, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds, [](long) { return NoPre{}; },
+    [&](int x, int y, int z, bool in, V p, V(&d1)[3], V(&d2)[3], const NoPre&) {
       if (!in) return;
       const long i = so + a.g.idx(x, y, z);
       const T d = a.den[i], cg = a.dencg[i];

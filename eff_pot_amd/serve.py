@@ -12,6 +12,11 @@ classes' kernels overlap on the GPU through their own streams.  Each slot's
 trajectory depends only on its own request (deterministic per-slot kernels and
 reductions), so a result is the same alone and in the mix.
 
+A request whose mean field turns out too stiff for the class time step (its
+energy stops being finite) is restarted from its initial state with half the
+time step, in the same slot and without touching the other slots
+(effpot_set_dt), up to `max_retries` times.
+
 There is no reference counterpart: the reference runs one density file per
 process (loop.sh:9-13).
 """
@@ -47,6 +52,8 @@ class Request:
     steps: int = 0
     converged: bool = False
     failed: bool = False
+    dt: float = 0.0       # time step of the final attempt
+    retries: int = 0
 
 
 def radius(natoms):
@@ -82,7 +89,7 @@ def plan_slots(hbm_budget_bytes, weights=None, precision=PREC_MIXED, max_slots=6
 class _Lane:
     """One grid class: a context, its slots and the thread that ticks it."""
 
-    def __init__(self, n, slots, opts_kw, precision):
+    def __init__(self, n, slots, opts_kw, precision, max_retries=2):
         self.n = n
         self.p = class_params(n, slots, precision)
         self.ctx = Context(self.p)
@@ -93,6 +100,7 @@ class _Lane:
             self.ctx.set_active(s, False)
         self.opts = SolveOpts(dt=default_dt(self.p), **opts_kw)
         self.precision = precision
+        self.max_retries = max_retries
         self.queue = []          # arrived, waiting for a slot
         self.lock = threading.Lock()
         self.done = []
@@ -102,6 +110,13 @@ class _Lane:
 
     def admit(self, req, now):
         s = self.free_slots()[0]
+        req.dt = self.opts.dt
+        req.retries = 0
+        req.start = now
+        self._load(s, req)
+
+    def _load(self, s, req):
+        self.ctx.set_dt(req.dt, slot=s)
         self.ctx.set_active(s, True)
         self.ctx.set_natoms(req.natoms, slot=s)
         self.ctx.init_fermi(0.0218, radius(req.natoms), 0.6, 1e-6, req.seed, slot=s)
@@ -111,7 +126,6 @@ class _Lane:
         self.slots[s].steps = 0
         self.slots[s].converged = 0
         req.grid = self.n
-        req.start = now
         self.req[s] = req
 
     def step(self, t0):
@@ -121,7 +135,14 @@ class _Lane:
         now = time.perf_counter() - t0
         for s in range(self.B):
             r = self.req[s]
-            if r is not None and self.slots[s].phase in (SLOT_DONE, SLOT_FAILED):
+            if r is None:
+                continue
+            if self.slots[s].phase == SLOT_FAILED and r.retries < self.max_retries:
+                r.retries += 1
+                r.dt *= 0.5
+                self._load(s, r)
+                continue
+            if self.slots[s].phase in (SLOT_DONE, SLOT_FAILED):
                 r.finish = now
                 r.eimp = self.slots[s].e.eimpu
                 r.etot = self.slots[s].e.etot
@@ -136,10 +157,10 @@ class _Lane:
 
 class Server:
     def __init__(self, slots_per_class, tol=1e-9, tol_fp32=1e-7, max_steps=60000, pener=50,
-                 precision=PREC_MIXED):
+                 precision=PREC_MIXED, max_retries=2):
         kw = dict(tol=tol, n_rk=3, pener=pener, max_steps=max_steps, polish_steps=max_steps,
                   tol_fp32=tol_fp32, check_finite=0)
-        self.lanes = {n: _Lane(n, k, kw, precision) for n, k in slots_per_class.items() if k > 0}
+        self.lanes = {n: _Lane(n, k, kw, precision, max_retries) for n, k in slots_per_class.items() if k > 0}
 
     def close(self):
         for ln in self.lanes.values():

@@ -63,8 +63,9 @@ typedef struct effpot_params {
   int32_t batch;      /* number of independent solves (slots), >= 1         */
   int32_t psi_complex;/* 1: complex psi (e.g. a real-time snapshot); 0: real psi
                          (imaginary time from a real start keeps Im psi = 0
-                         exactly, so the real path is bit-identical on half the
-                         bytes); uploads with Im psi != 0 are then rejected   */
+                         exactly, so the real path does the same arithmetic on
+                         half the bytes, equal up to FMA-contraction rounding);
+                         uploads with Im psi != 0 are then rejected           */
   int32_t reserved_;
   double xmax, ymax, zmax; /* box half widths; h = 2*xmax/nx (DFT4He3d.f90:205-207) */
   double n4;          /* number of He atoms the density is normalised to   */
@@ -182,6 +183,11 @@ int effpot_init_fermi(effpot_ctx* ctx, int slot, double rho0, double R,
 /* Atoms slot `slot` is normalised to (namelist n4, per slot; default
  * params.n4).  Set it before effpot_init_fermi for that slot. */
 int effpot_set_natoms(effpot_ctx* ctx, int slot, double n4);
+
+/* Time step of one slot for effpot_tick / effpot_solve (0: the options' dt).
+ * Lets a server advance heterogeneous requests, and retry a request whose
+ * mean field became too stiff for its step, without touching the others. */
+int effpot_set_dt(effpot_ctx* ctx, int slot, double dt);
 
 /* ---- impurity field (potenimpini.f90 / updatepoten+update+potenimp) ---- */
 int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp);

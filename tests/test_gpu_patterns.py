@@ -3,8 +3,9 @@
 - scan: every slot of a batched context equals the same position solved
   alone; the warm start (relaxed pure droplet) converges to the same E_tot;
 - server: requests of different droplet sizes arriving over time, routed to
-  grid classes; each result equals the request solved alone (1e-9 K) and the
-  latencies are recorded."""
+  grid classes; each result equals the request solved alone (1e-9 K), a
+  request that fails (non-finite energy) is retried with smaller steps and
+  fails the same way alone, and the latencies are recorded."""
 import numpy as np
 import pytest
 
@@ -55,13 +56,27 @@ def test_server_results_match_alone(gpu_available):
         srv.run(reqs)
     finally:
         srv.close()
-    for r in reqs:
-        assert r.grid == 64 and r.converged and not r.failed
+    # rid 5 (100 atoms, D = 60 K) collapses into the impurity well: the density
+    # peak grows past 0.18 A^-3 and the energy stops being finite near step 1900
+    # (the FP64 oracle fails at the same step).  The server retries it with
+    # dt/2 and dt/4, reports it failed, and fails it identically when alone.
+    for r in reqs[:5]:
+        assert r.grid == 64 and r.converged and not r.failed and r.retries == 0
         assert r.finish > r.start >= r.arrival
-    # the same requests alone, one context each
+    bad = reqs[5]
+    assert bad.failed and not bad.converged and bad.retries == 2
+    alone = serve.Request(rid=5, natoms=bad.natoms, D=bad.D, re=bad.re, a=bad.a, pos=bad.pos, seed=bad.seed)
+    srv = serve.Server({64: 1}, tol=1e-8, tol_fp32=1e-7, max_steps=30000, precision=PREC_MIXED)
+    try:
+        srv.run([alone])
+    finally:
+        srv.close()
+    assert alone.failed and alone.retries == bad.retries and alone.steps == bad.steps and alone.dt == bad.dt
+    # converged requests alone, one context each, through effpot_solve
     for r in reqs[:3]:
         p = serve.class_params(64, 1, PREC_MIXED)
         with ep.Context(p) as c:
+            c.set_dt(r.dt)
             c.set_natoms(r.natoms)
             c.init_fermi(0.0218, serve.radius(r.natoms), 0.6, 1e-6, r.seed)
             c.set_impurity(make_morse(rimp=r.pos, D=r.D, re=r.re, a=r.a))

