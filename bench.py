@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--tol", type=float, default=1e-9)
@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--nreq", type=int, default=32)
     ap.add_argument("--rate", type=float, default=0.5)
     ap.add_argument("--stress-steps", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=20261203, help="noise seed of the initial droplets")
     return ap.parse_args()
 
 
@@ -75,11 +76,47 @@ def positions(R, B):
 
 
 def solve_batch(ctx, p, R, zs, opts, seed=20261203):
+    """Load the B solves and tick them to convergence; returns the slots and the
+    seconds from the start of the batch to each slot's convergence."""
     import eff_pot_amd as ep
     for b, z0 in enumerate(zs):
         ctx.init_fermi(0.0218, R, 0.6, 1e-6, seed, slot=b)
         ctx.set_impurity(ep.make_morse(rimp=(0.0, 0.0, z0)), slot=b)
-    return ctx.solve(opts)
+    slots = ctx.new_slots()
+    for b in range(len(zs)):
+        slots[b].phase = ep.SLOT_START
+        slots[b].fp32 = 1 if p.precision == ep.PREC_MIXED else 0
+    lat = [None] * len(zs)
+    t0 = time.perf_counter()
+    while any(x is None for x in lat):
+        ctx.tick(opts, slots)
+        for b in range(len(zs)):
+            if lat[b] is None and slots[b].phase in (ep.SLOT_DONE, ep.SLOT_FAILED):
+                lat[b] = time.perf_counter() - t0
+    return [slots[b] for b in range(len(zs))], lat
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def traffic_of(kernel_key):
+    """HBM bytes per launch of a kernel, measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (calibrated against a copy kernel) and recorded in
+    profiles/traffic.json; None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel_key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def common_line(a, metric, value, unit, elapsed, dtype, workload, **extra):
@@ -195,12 +232,13 @@ def main():
     ctx = ep.Context(p)
     tile = ctx.autotune(reps=5) if not a.no_tune else None   # setup, before any state is loaded
     for _ in range(a.warmup):
-        solve_batch(ctx, p, R, zs, opts)
+        solve_batch(ctx, p, R, zs, opts, seed=a.seed)
     ctx.sync()
     t0 = time.perf_counter()
-    results = []
+    results, lats = [], []
     for _ in range(a.steps):
-        results = solve_batch(ctx, p, R, zs, opts)
+        results, lat = solve_batch(ctx, p, R, zs, opts, seed=a.seed)
+        lats += lat
     ctx.sync()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -222,6 +260,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 state, fp32 fields" if a.mode == "mixed" else "f64",
         "data": "synthetic (seeded Fermi droplet + 1e-6 noise)",
+        "p50_latency_s": percentile(lats, 0.5),
+        "p99_latency_s": percentile(lats, 0.99),
         "config": {"workload": f"He4 N=1000 droplet, {a.n}^3 grid h={2 * p.xmax / a.n:.3f} A, OTC h_core=2.377, "
                                f"Morse D=30 re=4 a=1 at z0~0.5R, to |dE/E|<1e-9/step",
                    "mode": a.mode, "batch": B, "dt": dt, "converged": conv, "stencil_tile": tile,
@@ -233,16 +273,18 @@ def main():
         Npts = a.n ** 3
         ms = ctx.kernel_time(1, reps=20)
         # H psi + corrector on a real wavefunction, per point and slot: reads psi, psi_n,
-        # psi_{n-2}, Hpsi_n, Hpsi_{n-1}, pc (FP64) and pot4, uimp; writes psi_{n+1}, pc (FP64)
-        # and den.  pot4/uimp/den are FP64 (88 B) or FP32 in the mixed mode (76 B).
-        per_pt = 76 if a.mode == "mixed" else 88
+        # psi_{n-2}, Hpsi_n, Hpsi_{n-1}, pc (FP64) and pot4, uimp; writes psi_{n+1} and pc
+        # (FP64); the density is stored by the renormalisation that follows.  pot4/uimp are
+        # FP64 (80 B) or FP32 in the mixed mode (72 B).
+        per_pt = 72 if a.mode == "mixed" else 80
         bytes_alg = per_pt * Npts * B
         ach = bytes_alg / (ms * 1e-3) / 1e9
         tx = tile[0] if tile else 64
         line["roofline"] = {"bound": "hbm", "kernel": f"k_step<double,double,{'float' if a.mode == 'mixed' else 'double'},"
                                                       f"{tx},1> (H psi + Hamming corrector)",
                             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "ms_per_launch": ms,
+                            "frac": ach / HBM_PEAK_GBS,
+                            "traffic": traffic_of(f"corrector_{a.mode}_{a.n}_b{B}"), "ms_per_launch": ms,
                             "bytes_per_launch": bytes_alg}
         line["poten_ms"] = ctx.kernel_time(2, reps=5)
     ctx.close()

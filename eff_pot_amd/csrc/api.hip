@@ -154,6 +154,7 @@ template <typename T> struct State {
   cx<T>* tw[3] = {};    // twiddles per axis
   StenCoef<T> sc;
   int* ring = nullptr;  // [B][8]
+  T* corr = nullptr;    // [B][N] FP32 fields only: defect correction added to pot4
 };
 
 }  // namespace
@@ -261,6 +262,7 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
   T** reals[] = {&s.den, &s.pot4, &s.uimp, &s.dencg, &s.delj4, &s.denalf, &s.ua, &s.ualphas, &s.dx, &s.dy,
                  &s.dz, &s.sto1, &s.intx, &s.inty, &s.intz, &s.div, &s.src[0], &s.src[1], &s.src[2], &s.src[3]};
   for (T** r : reals) A_(*r, B * N);
+  if (fields_only) A_(s.corr, B * N);
   for (int j = 0; j < 8; j++) A_(s.spec[j], B * M);
   for (int j = 0; j < 3; j++) A_(s.kern[j], M);
   A_(s.tw[0], ctx->nx); A_(s.tw[1], ctx->ny); A_(s.tw[2], ctx->nz);
@@ -274,6 +276,7 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
     HIPCHK(hipMemsetAsync(s.q, 0, sizeof(cx<T>) * B * N, ctx->st));
   }
   for (T** r : reals) HIPCHK(hipMemsetAsync(*r, 0, sizeof(T) * B * N, ctx->st));
+  if (s.corr) HIPCHK(hipMemsetAsync(s.corr, 0, sizeof(T) * B * N, ctx->st));
   // twiddles
   int ns[3] = {ctx->nx, ctx->ny, ctx->nz};
   for (int a = 0; a < 3; a++) {
@@ -427,7 +430,10 @@ int launch_step_v(effpot_ctx* ctx, State<T>& s, State<P>& f, const int* act, Sta
   StepArgs<T, V, P> a{};
   a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.h2o2m4 = (T)ctx->p.h2o2m4; a.dtv = ctx->d_dt;
   a.active = act; a.slot = ctx->N;
-  a.pc = reinterpret_cast<V*>(s.pc); a.q = reinterpret_cast<V*>(s.q); a.pcq_slot = ctx->N;
+  // pc and q are allocated (and reset per slot) as complex arrays of N per
+  // slot; a real wavefunction uses the first half of its slot's range.
+  a.pc = reinterpret_cast<V*>(s.pc); a.q = reinterpret_cast<V*>(s.q);
+  a.pcq_slot = ctx->N * (long)(sizeof(cx<T>) / sizeof(V));
   a.pot4 = f.pot4; a.uimp = f.uimp; a.den = f.den;
   a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.arun = (T)c.arun; a.brun = (T)c.brun; a.crun = (T)c.crun; a.first_stage = c.first;
@@ -583,7 +589,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
     a.h2o2m4 = (T)ctx->p.h2o2m4; a.alphas = (T)ctx->p.alphas; a.C = (T)ctx->p.C; a.beta = (T)ctx->p.beta;
     a.den_m = (T)ctx->p.den_m;
     a.delj4 = s.delj4; a.dencg = s.dencg; a.sto1 = s.sto1; a.ua = s.ua; a.div = s.div; a.den = s.den;
-    a.pot4 = s.pot4; a.ualphas = s.ualphas;
+    a.pot4 = s.pot4; a.ualphas = s.ualphas; a.corr = s.corr;
     hipLaunchKernelGGL((k_pot4<T>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, a);
     LAUNCHCHK();
   }
@@ -832,6 +838,7 @@ template <typename T> static int reset_slot(effpot_ctx* ctx, State<T>& s, int sl
   HIPCHK(hipMemsetAsync(s.q + (long)slot * N, 0, sizeof(cx<T>) * N, ctx->st));
   int init[8] = {0, 1, 2, 3, 4, 0, 1, 0};
   HIPCHK(hipMemcpyAsync(s.ring + 8 * slot, init, sizeof(init), hipMemcpyHostToDevice, ctx->st));
+  if (ctx->s32.corr) HIPCHK(hipMemsetAsync(ctx->s32.corr + (long)slot * N, 0, sizeof(float) * N, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }
@@ -1168,11 +1175,16 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
   // START: fields of the new states; they take their first RK step below.
   if (any_s) {
     RC(push_mask(ctx, 0, ms));
+    if (f32) {  // a new solve starts without a defect correction
+      hipLaunchKernelGGL((k_zero_slots<float>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, ctx->s32.corr, dS, ctx->N);
+      LAUNCHCHK();
+    }
     RC(den_from_cur(ctx, dS, f32));
     RC(f32 ? poten_t(ctx, ctx->s32, dS) : poten_t(ctx, s, dS));
     for (int b = 0; b < B; b++)
-      if (ms[b]) {
-        sl[b].nrk = 0; sl[b].since = 0;
+      if (ms[b]) {  // a new solve: counters start from zero
+        sl[b].nrk = 0; sl[b].since = 0; sl[b].steps = 0; sl[b].converged = 0;
+        sl[b].eprev = 0.0; sl[b].rel_de = 0.0;
         if (opt->n_rk > 0) { sl[b].phase = EFFPOT_SLOT_RK; mr[b] = 1; any_r = true; }
         else { sl[b].phase = EFFPOT_SLOT_PC; }
       }
@@ -1210,6 +1222,17 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
     }
     RC(poten_t(ctx, s, dE));
     RC(energy_t(ctx, s, dE, e.data()));
+    if (f32) {
+      // Defect correction of the FP32 mean field: evaluate it at the same
+      // state, add the difference to the FP64 field (pot4 + uimp) to the
+      // correction carried by every later FP32 poten, and continue from the
+      // FP64 field of this state (as the FP64 path does after its energy check).
+      RC(den_from_cur(ctx, dE, true));
+      RC(poten_t(ctx, ctx->s32, dE));
+      hipLaunchKernelGGL(k_defect, pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const double*)s.pot4,
+                         (const double*)s.uimp, ctx->s32.pot4, (const float*)ctx->s32.uimp, ctx->s32.corr, dE, ctx->N);
+      LAUNCHCHK();
+    }
     std::vector<int> bad(B, 0);
     if (opt->check_finite) {
       HIPCHK(hipMemcpyAsync(bad.data(), ctx->d_bad, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->st));
@@ -1267,7 +1290,12 @@ template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int
   auto once = [&]() -> int {
     switch (which) {
       case 0: return launch_step<T, 0>(ctx, s, ctx->d_active, f32);
-      case 1: return launch_step<T, 1>(ctx, s, ctx->d_active, f32);
+      case 1: {  // as in a relaxation: the renormalisation that follows stores the density
+        ctx->corr_den = false;
+        int rc = launch_step<T, 1>(ctx, s, ctx->d_active, f32);
+        ctx->corr_den = true;
+        return rc;
+      }
       case 2: return f32 ? poten_t(ctx, ctx->s32, ctx->d_active) : poten_t(ctx, s, ctx->d_active);
       default: return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
     }

@@ -154,6 +154,7 @@ template <typename T> struct Pot4Args {
   T a0, a1, den0s, h2o2m4, alphas, C, beta, den_m;
   const T *delj4, *dencg, *sto1, *ua, *div, *den;
   T* pot4; T* ualphas;
+  const T* corr;  // defect correction added to pot4 (FP32 fields of a mixed context), or null
 };
 template <typename T> __global__ void k_pot4(Pot4Args<T> a) {
   const int s = blockIdx.y;
@@ -170,7 +171,33 @@ template <typename T> __global__ void k_pot4(Pot4Args<T> a) {
     }
     T t = tanh(a.beta * (d - a.den_m));
     T pen = a.C * ((T(1) + t) + a.beta * d * (T(1) - t * t));
-    a.pot4[k] = p + pen;
+    a.pot4[k] = a.corr ? (p + pen) + a.corr[k] : p + pen;
+  }
+}
+
+// array[slot][i] = 0 for the slots in the mask (a kernel, so it can sit in code
+// that callers may capture into a graph)
+template <typename T> __global__ void k_zero_slots(T* a, const int* active, long N) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    a[(long)s * N + i] = T(0);
+}
+
+// Defect correction of FP32 fields from the FP64 ones at the same state:
+// corr += (pot4_64 + uimp_64) - (pot4_32 + uimp_32), evaluated in FP64 (pot4_32
+// already holds the previous correction), then pot4_32 = pot4_64 + uimp_64 -
+// uimp_32 so the FP32 step sees the FP64 potential up to one rounding.
+__global__ void k_defect(const double* p64, const double* u64, float* p32, const float* u32, float* corr,
+                         const int* active, long N) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const long so = (long)s * N;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const long k = so + i;
+    const double want = p64[k] + u64[k];
+    corr[k] = (float)((double)corr[k] + (want - ((double)p32[k] + (double)u32[k])));
+    p32[k] = (float)(want - (double)u32[k]);
   }
 }
 

@@ -134,7 +134,7 @@ typedef struct effpot_result {
 
 /* Per-slot progress of the continuous-batching engine (effpot_tick). */
 #define EFFPOT_SLOT_IDLE 0    /* no solve in this slot                           */
-#define EFFPOT_SLOT_START 1   /* state uploaded: next tick runs poten + RK start  */
+#define EFFPOT_SLOT_START 1   /* state uploaded: next tick zeroes the counters, runs poten + RK start */
 #define EFFPOT_SLOT_RK 2      /* in the Runge-Kutta-Gill starter steps            */
 #define EFFPOT_SLOT_PC 3      /* in the predictor-corrector phase                 */
 #define EFFPOT_SLOT_DONE 4    /* converged (or out of steps): results valid        */

@@ -79,57 +79,73 @@ def test_converged_mixed_vs_oracle_and_fp64(gpu_available):
     assert l2 < 1e-4
 
 
-def _req(ctx, slot, z, seed):
+# requests (z0, seed, Morse D) and the slot each takes in the mix: request 3
+# reuses slot 0 as soon as request 0 is done, while requests 1 and 2 run
+REQS = [(0.5, 1, 25.0), (1.5, 2, 30.0), (-2.0, 3, 35.0), (1.0, 4, 28.0)]
+SLOT_OF = [0, 1, 2, 0]
+
+
+def _req(ctx, slot, k):
+    z, seed, D = REQS[k]
     ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, seed, slot=slot)
-    ctx.set_impurity(make_morse(rimp=(0.0, 0.0, z), D=25.0 + 5 * slot), slot=slot)
+    ctx.set_impurity(make_morse(rimp=(0.0, 0.0, z), D=D), slot=slot)
 
 
-@pytest.mark.parametrize("precision", [0, PREC_MIXED])
-def test_same_result_alone_and_in_the_mix(precision, gpu_available):
-    print(f"[mix] precision={precision}", flush=True)
+@pytest.mark.parametrize("precision,cplx", [(0, 1), (PREC_MIXED, 1), (0, 0), (PREC_MIXED, 0)])
+def test_same_result_alone_and_in_the_mix(precision, cplx, gpu_available):
+    print(f"[mix] precision={precision} complex={cplx}", flush=True)
     opts = SolveOpts(dt=1.9e-4, tol=1e-8, n_rk=3, pener=50, max_steps=40000, polish_steps=40000, tol_fp32=3e-8,
                      check_finite=1)
-    zs, seeds = [0.5, 1.5, -2.0], [1, 2, 3]
+
+    def params():
+        return make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision, batch=3,
+                           psi_complex=cplx)
+
     alone = []
-    for k in range(3):
-        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision, batch=3)
-        with ep.Context(p) as ctx:
+    for k in range(4):
+        sk = SLOT_OF[k]
+        with ep.Context(params()) as ctx:
             for s in range(3):
-                if s != k:
+                if s != sk:
                     ctx.set_active(s, False)
-            _req(ctx, k, zs[k], seeds[k])
+            _req(ctx, sk, k)
             slots = ctx.new_slots()
-            slots[k].phase = ep.SLOT_START
-            slots[k].fp32 = 1 if precision else 0
-            while slots[k].phase not in (ep.SLOT_DONE, ep.SLOT_FAILED):
+            slots[sk].phase = ep.SLOT_START
+            slots[sk].fp32 = 1 if precision else 0
+            while slots[sk].phase not in (ep.SLOT_DONE, ep.SLOT_FAILED):
                 ctx.tick(opts, slots)
-            alone.append((slots[k].e.eimpu, slots[k].e.etot, slots[k].steps, ctx.psi(k)))
-    # the mix: requests admitted at different times into one context
-    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision, batch=3)
-    with ep.Context(p) as ctx:
+            alone.append((slots[sk].e.eimpu, slots[sk].e.etot, slots[sk].steps, ctx.psi(sk)))
+    # the mix: requests admitted at different times into one context, and a
+    # slot re-used (its state is reset while the other slots keep stepping)
+    with ep.Context(params()) as ctx:
         slots = ctx.new_slots()
         admit = {0: 0, 1: 37, 2: 180}
+        running = {}  # slot -> request
+        results = {}
         t = 0
-        done = set()
-        psis = {}
-        while len(done) < 3:
+        while len(results) < 4:
             for k, at in admit.items():
                 if at == t:
-                    _req(ctx, k, zs[k], seeds[k])
-                    slots[k].phase = ep.SLOT_START
-                    slots[k].fp32 = 1 if precision else 0
+                    _req(ctx, SLOT_OF[k], k)
+                    slots[SLOT_OF[k]].phase = ep.SLOT_START
+                    slots[SLOT_OF[k]].fp32 = 1 if precision else 0
+                    running[SLOT_OF[k]] = k
             ctx.tick(opts, slots)
-            for k in range(3):
-                if k not in done and slots[k].phase in (ep.SLOT_DONE, ep.SLOT_FAILED):
-                    done.add(k)
-                    psis[k] = ctx.psi(k)
+            for s, k in list(running.items()):
+                if slots[s].phase in (ep.SLOT_DONE, ep.SLOT_FAILED):
+                    results[k] = (slots[s].phase, slots[s].converged, slots[s].e.eimpu, slots[s].e.etot,
+                                  slots[s].steps, ctx.psi(s))
+                    del running[s]
+                    if k == 0:
+                        admit[3] = t + 1
             t += 1
-        for k in range(3):
+        for k in range(4):
             ea, et, st, pa = alone[k]
-            assert slots[k].phase == ep.SLOT_DONE and slots[k].converged
-            assert slots[k].steps == st
-            assert abs(slots[k].e.eimpu - ea) <= 1e-9 and abs(slots[k].e.etot - et) <= 1e-9
-            assert np.linalg.norm(psis[k] - pa) <= 1e-12 * np.linalg.norm(pa)
+            ph, conv, e_imp, e_tot, steps, psi = results[k]
+            assert ph == ep.SLOT_DONE and conv
+            assert steps == st
+            assert abs(e_imp - ea) <= 1e-9 and abs(e_tot - et) <= 1e-9, (k, e_imp - ea, e_tot - et)
+            assert np.linalg.norm(psi - pa) <= 1e-12 * np.linalg.norm(pa)
 
 
 def test_nonfinite_density_is_reported(gpu_available):
