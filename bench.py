@@ -10,10 +10,14 @@ predictor-corrector steps in imaginary time with renormalisation to N until
 |dE/E| per step < 1e-9 (FP64 state at the end in every mode).
 
 A "step" of this benchmark is one batch of B such solves run to convergence
-(B impurity positions z0 = 0.5 R * (0.95 .. 1.05)); value = solves / second.
+(B droplets that differ in the seed of their initial noise, all with the
+impurity at z0 = 0.5 R); value = solves / second.
 Modes: --mode mixed (default): FP64 wavefunction and integrator, FP32 mean
-field (all FFTs and density stencils), FP64 reductions and energy checks,
-FP64 final polish; --mode fp64: the plain FP64 path, one solve per context.
+field (all FFTs and density stencils) with a defect correction refreshed from
+the FP64 field at every energy check, FP64 reductions and energy checks, FP64
+final polish; --mode fp64: the plain FP64 path, one solve per context.  The
+mixed run also solves slot 0's input with the plain FP64 path (untimed) and
+reports |dE_imp| and the speed-up against it.
 
 Also reported: the live roofline of the dominant kernel (H psi + corrector),
 timed with HIP events on the library's stream, and the CPU baseline (the
@@ -46,6 +50,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-ref", action="store_true",
+                    help="skip the plain FP64 solve of slot 0's input that gives |dE_imp| and the speed-up")
     ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256"], default="solve128",
                     help="solve128 (default, the headline); the others are the batched scan, the serving stream, "
                          "the 256^3 stress test and the small CPU-runnable case")
@@ -70,9 +76,9 @@ def workload(n, mode, batch):
 
 
 def positions(R, B):
-    if B == 1:
-        return [0.5 * R]
-    return [0.5 * R * (0.95 + 0.10 * b / (B - 1)) for b in range(B)]
+    """Every slot solves the headline workload (impurity at z0 = 0.5 R); the
+    slots differ in the seed of their initial noise."""
+    return [0.5 * R] * B
 
 
 def solve_batch(ctx, p, R, zs, opts, seed=20261203):
@@ -80,7 +86,7 @@ def solve_batch(ctx, p, R, zs, opts, seed=20261203):
     seconds from the start of the batch to each slot's convergence."""
     import eff_pot_amd as ep
     for b, z0 in enumerate(zs):
-        ctx.init_fermi(0.0218, R, 0.6, 1e-6, seed, slot=b)
+        ctx.init_fermi(0.0218, R, 0.6, 1e-6, seed + b, slot=b)
         ctx.set_impurity(ep.make_morse(rimp=(0.0, 0.0, z0)), slot=b)
     slots = ctx.new_slots()
     for b in range(len(zs)):
@@ -88,11 +94,16 @@ def solve_batch(ctx, p, R, zs, opts, seed=20261203):
         slots[b].fp32 = 1 if p.precision == ep.PREC_MIXED else 0
     lat = [None] * len(zs)
     t0 = time.perf_counter()
+    t_report = t0
     while any(x is None for x in lat):
         ctx.tick(opts, slots)
         for b in range(len(zs)):
             if lat[b] is None and slots[b].phase in (ep.SLOT_DONE, ep.SLOT_FAILED):
                 lat[b] = time.perf_counter() - t0
+        if slots[0].since == 0 and time.perf_counter() - t_report > 30.0:  # after an energy check
+            t_report = time.perf_counter()
+            print(f"[bench] {t_report - t0:.0f} s: steps {[slots[b].steps for b in range(len(zs))]}, "
+                  f"converged {sum(x is not None for x in lat)}/{len(zs)}", file=sys.stderr, flush=True)
     return [slots[b] for b in range(len(zs))], lat
 
 
@@ -151,7 +162,8 @@ def run_scan(a):
     prec = 1 if a.mode == "mixed" else 0
     t0 = time.perf_counter()
     res, meta = scan.run_scan(n=96, spacing=0.6, natoms=1000.0, npos=a.npos, precision=prec, tol=a.tol,
-                              tol_fp32=a.tol_fp32, max_steps=a.max_steps)
+                              tol_fp32=a.tol_fp32, max_steps=a.max_steps,
+                              progress=lambda m: print("[scan] " + m, file=sys.stderr, flush=True))
     el = time.perf_counter() - t0
     return common_line(a, "V_eff(z0) scan: converged positions/s (N=1000, 96^3)", len(res) / el, "solves/s", el,
                        "fp32 fields + f64" if prec else "f64",
@@ -167,7 +179,9 @@ def run_serve(a):
                        precision=1 if a.mode == "mixed" else 0)
     t0 = time.perf_counter()
     try:
-        srv.run(reqs)
+        srv.run(reqs, progress=lambda r: print(f"[serve] {time.perf_counter() - t0:.0f} s: request {r.rid} "
+                                               f"grid {r.grid} steps {r.steps} failed {r.failed}",
+                                               file=sys.stderr, flush=True))
     finally:
         srv.close()
     el = time.perf_counter() - t0
@@ -265,7 +279,7 @@ def main():
         "config": {"workload": f"He4 N=1000 droplet, {a.n}^3 grid h={2 * p.xmax / a.n:.3f} A, OTC h_core=2.377, "
                                f"Morse D=30 re=4 a=1 at z0~0.5R, to |dE/E|<1e-9/step",
                    "mode": a.mode, "batch": B, "dt": dt, "converged": conv, "stencil_tile": tile,
-                   "steps_per_solve": steps_per_solve, "E_imp_K": eimp,
+                   "steps_per_solve": steps_per_solve, "steps": [r.steps for r in results], "E_imp_K": eimp,
                    "E_tot_K": [r.e.etot for r in results]},
     }
     # ---- roofline of the dominant kernel: H psi + corrector ------------------
@@ -288,6 +302,18 @@ def main():
                             "bytes_per_launch": bytes_alg}
         line["poten_ms"] = ctx.kernel_time(2, reps=5)
     ctx.close()
+    # ---- the plain FP64 form on slot 0's input (untimed reference) -----------
+    if a.mode == "mixed" and not a.no_ref:
+        pr, _ = workload(a.n, "fp64", 1)
+        with ep.Context(pr) as cr:
+            cr.autotune(reps=5)
+            tr0 = time.perf_counter()
+            (r64,), _ = solve_batch(cr, pr, R, zs[:1], opts, seed=a.seed)
+            tr = time.perf_counter() - tr0
+        line["fp64_reference"] = {"solves_per_s": 1.0 / tr, "E_imp_K": r64.e.eimpu, "steps": r64.steps,
+                                  "dE_imp_K": abs(results[0].e.eimpu - r64.e.eimpu),
+                                  "max_dE_imp_K_all_slots": max(abs(r.e.eimpu - r64.e.eimpu) for r in results),
+                                  "speedup": line["value"] * tr}
     # ---- CPU baseline: the FP64 oracle on the same workload, bounded sample ---
     if not a.no_cpu:
         from oracle.oracle import Oracle

@@ -97,6 +97,48 @@ module effpot_c
       type(effpot_solve_opts), intent(in) :: opt
       type(effpot_result), intent(out) :: res
     end function
+    ! per-step replacements of steprk(deltat) / steppc(deltat, errHe, ...)
+    integer(c_int) function effpot_step_rk(ctx, dt) bind(c, name="effpot_step_rk")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function effpot_step_pc(ctx, dt, err_he) bind(c, name="effpot_step_pc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: err_he
+    end function
+    integer(c_int) function effpot_renormalize(ctx) bind(c, name="effpot_renormalize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function effpot_set_natoms(ctx, slot, n4) bind(c, name="effpot_set_natoms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), value :: n4
+    end function
+    integer(c_int) function effpot_set_dt(ctx, slot, dt) bind(c, name="effpot_set_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function effpot_download_den(ctx, slot, den) bind(c, name="effpot_download_den")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: den(*)
+    end function
+    ! derivnd(id, nn, h, iv, f, df, icon) on a real (nx,ny,nz) array, grid and icon of the context
+    integer(c_int) function effpot_derivnd(ctx, id, iv, f, df) bind(c, name="effpot_derivnd")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: id, iv
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: df(*)
+    end function
   end interface
 
 contains

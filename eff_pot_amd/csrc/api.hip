@@ -26,34 +26,10 @@ using namespace ep;
 
 namespace {
 
-enum Role { R_CUR = 0, R_O1, R_O2, R_O3, R_SPARE, R_H1, R_H2, R_OUT };
-enum RingOp { RING_RESET = 0, RING_BEFORE_RK0, RING_AFTER_RK0, RING_BEFORE_RKJ, RING_AFTER_RKJ, RING_AFTER_PRED,
-              RING_AFTER_CORR };
-
 __global__ void k_ring(int* ring, const int* active, int B, int op) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B || !active[s]) return;
-  int* r = ring + 8 * s;
-  switch (op) {
-    case RING_RESET:
-      r[R_CUR] = 0; r[R_O1] = 1; r[R_O2] = 2; r[R_O3] = 3; r[R_SPARE] = 4; r[R_H1] = 0; r[R_H2] = 1; r[R_OUT] = 0;
-      break;
-    case RING_BEFORE_RK0: {
-      r[R_OUT] = r[R_O3]; r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = r[R_CUR];
-      int t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
-    } break;
-    case RING_AFTER_RK0: r[R_CUR] = r[R_OUT]; break;
-    case RING_BEFORE_RKJ: r[R_OUT] = r[R_SPARE]; break;
-    case RING_AFTER_RKJ: r[R_SPARE] = r[R_CUR]; r[R_CUR] = r[R_OUT]; break;
-    case RING_AFTER_PRED: {
-      int t = r[R_CUR]; r[R_CUR] = r[R_O3]; r[R_O3] = t;
-      t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
-    } break;
-    case RING_AFTER_CORR: {
-      int fin = r[R_SPARE], mod = r[R_CUR], n = r[R_O3];
-      r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = n; r[R_SPARE] = mod; r[R_CUR] = fin;
-    } break;
-  }
+  ring_apply(ring + 8 * s, op);
 }
 
 // Pointer of ring role r of slot s: all + (s*nbuf + ring[s][r]) * N.
@@ -75,8 +51,15 @@ __global__ __launch_bounds__(kThreads) void k_step(StepArgs<T, V, P> a, Ring<T> 
   // shared kernel body (the StepArgs pointers are slot-0 based; offsets are
   // re-applied by the body, so pass slot-relative pointers minus the offset).
   const long so = (long)s * a.slot;
-  auto PP = [&](int r) { return reinterpret_cast<V*>(rg.p(s, r)) - so; };
-  auto H = [&](int r) { return reinterpret_cast<V*>(rg.h(s, r)) - so; };
+  int r8[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) r8[k] = rg.ring[8 * s + k];
+  // the corrector runs before the ring is rotated: it sees the roles that
+  // RING_AFTER_PRED gives (the rotation is applied once per step by the
+  // norm reduction that follows)
+  if constexpr (MODE == 1) ring_apply(r8, RING_AFTER_PRED);
+  auto PP = [&](int r) { return reinterpret_cast<V*>(rg.psi + ((long)s * 5 + r8[r]) * rg.N) - so; };
+  auto H = [&](int r) { return reinterpret_cast<V*>(rg.hp + ((long)s * 2 + r8[r]) * rg.N) - so; };
   StepArgs<T, V, P> b = a;
   b.cur = PP(R_CUR);
   b.out = MODE == 1 ? PP(R_SPARE) : PP(R_OUT);
@@ -492,9 +475,10 @@ int den_from_cur(effpot_ctx* ctx, const int* act, bool f32) {
   return EFFPOT_OK;
 }
 
-template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act) {
+template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring = nullptr, int op1 = -1,
+                            int op2 = -1) {
   hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
-                     act, mode, (const double*)ctx->d_n4, ctx->dxyz, ctx->d_scale);
+                     act, mode, (const double*)ctx->d_n4, ctx->dxyz, ctx->d_scale, ring, op1, op2);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -634,11 +618,10 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
 // Hamming predictor-modifier-corrector (steppc.f90:35-159).
 template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   RC((launch_step<T, 0>(ctx, s, act, f32)));
-  RC(ring_op(ctx, s, RING_AFTER_PRED, act));
   RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
-  RC((launch_step<T, 1>(ctx, s, act, f32)));
-  RC(ring_op(ctx, s, RING_AFTER_CORR, act));
-  RC(reduce<2>(ctx, 1, act));
+  RC((launch_step<T, 1>(ctx, s, act, f32)));  // roles as after RING_AFTER_PRED
+  // norm sums, then both role rotations of the step in the same launch
+  RC(reduce<2>(ctx, 1, act, s.ring, RING_AFTER_PRED, RING_AFTER_CORR));
   return EFFPOT_OK;
 }
 

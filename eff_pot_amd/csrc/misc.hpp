@@ -218,10 +218,40 @@ __global__ void k_renorm(cx<T>* psi, T* den, const double* scale, const int* act
 
 // ---- deterministic reduction: partial[s][nblk][K] -> out[s][K] -------------
 // mode 1: also scale[s] = sqrt(n4[s] / (out[s][1] * dxyz)).
+// ---- buffer roles of the psi / H psi history --------------------------------
+enum Role { R_CUR = 0, R_O1, R_O2, R_O3, R_SPARE, R_H1, R_H2, R_OUT };
+enum RingOp { RING_RESET = 0, RING_BEFORE_RK0, RING_AFTER_RK0, RING_BEFORE_RKJ, RING_AFTER_RKJ, RING_AFTER_PRED,
+              RING_AFTER_CORR };
+
+// Role rotation of one slot's buffers (the reference's ioldp/ioldh rotation).
+EP_HD void ring_apply(int* r, int op) {
+  switch (op) {
+    case RING_RESET:
+      r[R_CUR] = 0; r[R_O1] = 1; r[R_O2] = 2; r[R_O3] = 3; r[R_SPARE] = 4; r[R_H1] = 0; r[R_H2] = 1; r[R_OUT] = 0;
+      break;
+    case RING_BEFORE_RK0: {
+      r[R_OUT] = r[R_O3]; r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = r[R_CUR];
+      int t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
+    } break;
+    case RING_AFTER_RK0: r[R_CUR] = r[R_OUT]; break;
+    case RING_BEFORE_RKJ: r[R_OUT] = r[R_SPARE]; break;
+    case RING_AFTER_RKJ: r[R_SPARE] = r[R_CUR]; r[R_CUR] = r[R_OUT]; break;
+    case RING_AFTER_PRED: {
+      int t = r[R_CUR]; r[R_CUR] = r[R_O3]; r[R_O3] = t;
+      t = r[R_H1]; r[R_H1] = r[R_H2]; r[R_H2] = t;
+    } break;
+    case RING_AFTER_CORR: {
+      int fin = r[R_SPARE], mod = r[R_CUR], n = r[R_O3];
+      r[R_O3] = r[R_O2]; r[R_O2] = r[R_O1]; r[R_O1] = n; r[R_SPARE] = mod; r[R_CUR] = fin;
+    } break;
+    default: break;
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(kThreads) void k_reduce(const double* partial, int nblk, double* out,
                                                       const int* active, int mode, const double* n4, double dxyz,
-                                                      double* scale) {
+                                                      double* scale, int* ring, int op1, int op2) {
   __shared__ double red[4 * K];
   const int s = blockIdx.x;
   if (!active[s]) return;
@@ -238,6 +268,10 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* partial, int 
 #pragma unroll
     for (int k = 0; k < K; k++) out[s * K + k] = v[k];
     if (mode == 1) scale[s] = sqrt(n4[s] / (v[1] * dxyz));
+    if (ring) {  // role rotations of the step that produced these sums
+      ring_apply(ring + 8 * s, op1);
+      ring_apply(ring + 8 * s, op2);
+    }
   }
 }
 

@@ -6,9 +6,12 @@ once and every slot starts from it instead of the Fermi profile.  Positions
 whose impurity lies outside the box are computed (the Morse field there is
 only its tail inside the grid) and flagged.
 """
+import time
+
 import numpy as np
 
 from . import Context, SolveOpts, make_params, make_morse, default_dt, CORE_OTC, PREC_MIXED, POT_NULL
+from . import SLOT_START, SLOT_DONE, SLOT_FAILED
 
 
 def scan_positions(natoms, npos, zmax_frac=1.6):
@@ -17,7 +20,9 @@ def scan_positions(natoms, npos, zmax_frac=1.6):
 
 
 def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, tol=1e-9, tol_fp32=1e-7,
-             max_steps=60000, warm=False, seed=20261203, h_core=2.377):
+             max_steps=60000, warm=False, seed=20261203, h_core=2.377, progress=None):
+    """Relax all positions in one batched context (ticked until every slot is
+    done); `progress(msg)` is called about every 30 s."""
     xmax = spacing * n / 2
     p = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=npos,
                     psi_complex=0)
@@ -44,7 +49,20 @@ def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, to
         else:
             ctx.init_fermi(0.0218, R, 0.6, 1e-6, seed, slot=b)
         ctx.set_impurity(make_morse(rimp=(0.0, 0.0, z)), slot=b)
-    res = ctx.solve(opts)
+    slots = ctx.new_slots()
+    for b in range(npos):
+        slots[b].phase = SLOT_START
+        slots[b].fp32 = 1 if precision == PREC_MIXED else 0
+    t0 = time.perf_counter()
+    t_report = t0
+    while any(slots[b].phase not in (SLOT_DONE, SLOT_FAILED) for b in range(npos)):
+        ctx.tick(opts, slots)
+        if progress and slots[0].since == 0 and time.perf_counter() - t_report > 30.0:
+            t_report = time.perf_counter()
+            done = sum(slots[b].phase in (SLOT_DONE, SLOT_FAILED) for b in range(npos))
+            progress(f"{t_report - t0:.0f} s: {done}/{npos} positions converged, "
+                     f"steps {min(slots[b].steps for b in range(npos))}..{max(slots[b].steps for b in range(npos))}")
+    res = [slots[b] for b in range(npos)]
     ctx.close()
     out = []
     for z, r in zip(zs, res):

@@ -128,7 +128,7 @@ class _Lane:
         req.grid = self.n
         self.req[s] = req
 
-    def step(self, t0):
+    def step(self, t0, progress=None):
         busy = any(r is not None for r in self.req)
         if busy:
             self.ctx.tick(self.opts, self.slots)
@@ -150,6 +150,8 @@ class _Lane:
                 r.converged = bool(self.slots[s].converged)
                 r.failed = self.slots[s].phase == SLOT_FAILED
                 self.done.append(r)
+                if progress:
+                    progress(r)
                 self.req[s] = None
                 self.ctx.set_active(s, False)
         return busy
@@ -166,9 +168,10 @@ class Server:
         for ln in self.lanes.values():
             ln.ctx.close()
 
-    def run(self, requests):
+    def run(self, requests, progress=None):
         """Serve `requests` (with .arrival seconds from start) in real time.
-        Returns them completed, with start/finish/latency fields set."""
+        Returns them completed, with start/finish/latency fields set;
+        `progress(request)` is called as each one completes."""
         pending = [(r.arrival, r.rid, r) for r in requests]
         heapq.heapify(pending)
         for r in requests:
@@ -195,7 +198,7 @@ class Server:
                     no_more = not any(it[2].grid == ln.n for it in pending)
                 while ln.queue and ln.free_slots():
                     ln.admit(ln.queue.pop(0), time.perf_counter() - t0)
-                busy = ln.step(t0)
+                busy = ln.step(t0, progress)
                 if not busy:
                     if no_more and not ln.queue:
                         return

@@ -85,10 +85,14 @@ REQS = [(0.5, 1, 25.0), (1.5, 2, 30.0), (-2.0, 3, 35.0), (1.0, 4, 28.0)]
 SLOT_OF = [0, 1, 2, 0]
 
 
-def _req(ctx, slot, k):
-    z, seed, D = REQS[k]
+def _load(ctx, slot, z, seed, D=30.0):
     ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, seed, slot=slot)
     ctx.set_impurity(make_morse(rimp=(0.0, 0.0, z), D=D), slot=slot)
+
+
+def _req(ctx, slot, k):
+    z, seed, D = REQS[k]
+    _load(ctx, slot, z, seed, D)
 
 
 @pytest.mark.parametrize("precision,cplx", [(0, 1), (PREC_MIXED, 1), (0, 0), (PREC_MIXED, 0)])
@@ -185,22 +189,26 @@ def test_fortran_driver_labels(gpu_available, tmp_path):
     assert float(line.split()[4]) == eint
 
 
-def test_graph_replay_matches_eager(gpu_available, monkeypatch):
+@pytest.mark.parametrize("precision,cplx", [(0, 1), (PREC_MIXED, 0)])
+def test_graph_replay_matches_eager(precision, cplx, gpu_available, monkeypatch):
     """Every replay of the captured predictor-corrector step, with eager energy
-    work and synchronisation in between, equals the eager launch sequence."""
-    print("[graph] replay vs eager", flush=True)
-    opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=1, max_steps=60, polish_steps=0, tol_fp32=0.0,
-                     check_finite=0)
+    work and synchronisation in between, equals the eager launch sequence
+    (mixed: FP32 phase with its defect corrections, then the FP64 polish)."""
+    print(f"[graph] replay vs eager precision={precision} complex={cplx}", flush=True)
+    opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=1 if precision == 0 else 5, max_steps=60, polish_steps=30,
+                     tol_fp32=0.0, check_finite=0)
     traj = {}
     for graphs in ("1", "0"):
         monkeypatch.setenv("EFFPOT_GRAPHS", graphs)
-        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=2)
+        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=2, precision=precision,
+                        psi_complex=cplx)
         with ep.Context(p) as ctx:
             for s in range(2):
-                _req(ctx, s, [0.5, -1.0][s], 11 + s)
+                _load(ctx, s, [0.5, -1.0][s], 11 + s, 25.0 + 5 * s)
             slots = ctx.new_slots()
             for s in range(2):
                 slots[s].phase = ep.SLOT_START
+                slots[s].fp32 = 1 if precision else 0
             es = []
             while any(slots[s].phase not in (ep.SLOT_DONE, ep.SLOT_FAILED) for s in range(2)):
                 ctx.tick(opts, slots)
@@ -226,7 +234,7 @@ def test_real_psi_path_is_bit_identical(precision, gpu_available):
         p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, precision=precision,
                         psi_complex=cplx)
         with ep.Context(p) as ctx:
-            _req(ctx, 0, 1.0, 5)
+            _load(ctx, 0, 1.0, 5, 25.0)
             r = ctx.solve(opts)[0]
             out[cplx] = (r.e.etot, r.e.eimpu, r.e.ekin4, r.e.mu, r.steps, ctx.psi())
     a, b = out[1], out[0]
