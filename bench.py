@@ -52,7 +52,9 @@ def parse():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-ref", action="store_true",
                     help="skip the plain FP64 solve of slot 0's input that gives |dE_imp| and the speed-up")
-    ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256"], default="solve128",
+    ap.add_argument("--tile", type=int, nargs=2, default=None, help="stencil tile (TX, z chunk) instead of autotuning")
+    ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256", "corrector"],
+                    default="solve128",
                     help="solve128 (default, the headline); the others are the batched scan, the serving stream, "
                          "the 256^3 stress test and the small CPU-runnable case")
     ap.add_argument("--npos", type=int, default=48)
@@ -194,6 +196,33 @@ def run_serve(a):
                        slots=plan, converged=len(ok), grids={g: sum(1 for r in reqs if r.grid == g) for g in (64, 96, 128)})
 
 
+def run_corrector(a):
+    """The headline context with B loaded slots, 3 RK steps to fill the
+    history, then the corrector timed inside 20 whole predictor-corrector
+    steps: the workload the traffic counters are read on."""
+    import eff_pot_amd as ep
+    B = a.batch if a.mode == "mixed" else 1
+    p, R = workload(a.n, a.mode, B)
+    dt = ep.default_dt(p)
+    with ep.Context(p) as c:
+        tile = tuple(a.tile) if a.tile else c.autotune(reps=5)
+        c.set_tiles(*tile)
+        for b, z0 in enumerate(positions(R, B)):
+            c.init_fermi(0.0218, R, 0.6, 1e-6, a.seed + b, slot=b)
+            c.set_impurity(ep.make_morse(rimp=(0.0, 0.0, z0)), slot=b)
+        c.poten()
+        for _ in range(3):
+            c.step_rk(dt)
+            c.renormalize()
+        ms = c.kernel_time(3, reps=20)
+    per_pt = 72 if a.mode == "mixed" else 80
+    byt = per_pt * a.n ** 3 * B
+    return common_line(a, "H psi + corrector GB/s (algorithmic bytes)", byt / (ms * 1e-3) / 1e9, "GB/s", ms * 1e-3,
+                       "f64 state, fp32 fields" if a.mode == "mixed" else "f64",
+                       f"corrector of the {a.n}^3 N=1000 workload, {B} slots, inside whole PC steps",
+                       mode=a.mode, batch=B, stencil_tile=list(tile), ms_per_launch=ms, bytes_per_launch=byt)
+
+
 def run_stress(a):
     """256^3, h=0.4 A, N=5000: FP64 single solve and a batch of 8 mixed, timed over a bounded number of steps."""
     import eff_pot_amd as ep
@@ -233,7 +262,8 @@ def main():
         sys.exit(2)
     import eff_pot_amd as ep
     if a.workload != "solve128":
-        fn = {"cfg0": run_cfg0, "scan": run_scan, "serve": run_serve, "stress256": run_stress}[a.workload]
+        fn = {"cfg0": run_cfg0, "scan": run_scan, "serve": run_serve, "stress256": run_stress,
+              "corrector": run_corrector}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 
@@ -244,7 +274,11 @@ def main():
                         tol_fp32=a.tol_fp32, check_finite=0)
     zs = positions(R, B)
     ctx = ep.Context(p)
-    tile = ctx.autotune(reps=5) if not a.no_tune else None   # setup, before any state is loaded
+    if a.tile:
+        tile = tuple(a.tile)
+        ctx.set_tiles(*tile)
+    else:
+        tile = ctx.autotune(reps=5) if not a.no_tune else None   # setup, before any state is loaded
     for _ in range(a.warmup):
         solve_batch(ctx, p, R, zs, opts, seed=a.seed)
     ctx.sync()
@@ -285,7 +319,13 @@ def main():
     # ---- roofline of the dominant kernel: H psi + corrector ------------------
     if not a.no_roofline:
         Npts = a.n ** 3
-        ms = ctx.kernel_time(1, reps=20)
+        # corrector launches timed with HIP events on the library's stream inside
+        # whole predictor-corrector steps of the converged states (the caches
+        # hold what a relaxation leaves in them); back-to-back launches of the
+        # corrector alone are reported too (their inputs partly stay in the
+        # 256 MB last-level cache, so they read faster than a relaxation sees)
+        ms = ctx.kernel_time(3, reps=20)
+        ms_b2b = ctx.kernel_time(1, reps=20)
         # H psi + corrector on a real wavefunction, per point and slot: reads psi, psi_n,
         # psi_{n-2}, Hpsi_n, Hpsi_{n-1}, pc (FP64) and pot4, uimp; writes psi_{n+1} and pc
         # (FP64); the density is stored by the renormalisation that follows.  pot4/uimp are
@@ -299,6 +339,7 @@ def main():
                             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": ach / HBM_PEAK_GBS,
                             "traffic": traffic_of(f"corrector_{a.mode}_{a.n}_b{B}"), "ms_per_launch": ms,
+                            "ms_per_launch_back_to_back": ms_b2b,
                             "bytes_per_launch": bytes_alg}
         line["poten_ms"] = ctx.kernel_time(2, reps=5)
     ctx.close()

@@ -147,6 +147,7 @@ struct effpot_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
   int nx, ny, nz, nxh, B;
+  int nxp;  // half-spectrum row pitch: nx/2+1 rounded up to a multiple of 16
   long N, M;
   double hx, hy, hz, dxyz;
   int npot;
@@ -314,6 +315,7 @@ template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, 
   XArgs<T> a{};
   for (int f = 0; f < nf; f++) { a.r[f] = in[f]; a.c[f] = out[f]; }
   a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = T(1); a.twN = s.tw[0]; a.active = act;
+  a.pitch = ctx->nxp;
   const int C = kThreads / n1_of(ctx->nx);
   dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
 #define M_(NN) hipLaunchKernelGGL((k_fft_x_r2c<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
@@ -331,6 +333,7 @@ template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* 
     a.aff1[f] = aff1 ? aff1[f] : T(1);
   }
   a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = scale; a.twN = s.tw[0]; a.active = act;
+  a.pitch = ctx->nxp;
   const int C = kThreads / n1_of(ctx->nx);
   dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
 #define M_(NN) hipLaunchKernelGGL((k_fft_x_c2r<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
@@ -339,9 +342,13 @@ template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* 
   LAUNCHCHK();
   return EFFPOT_OK;
 }
-LineMap ymap(effpot_ctx* ctx) { return LineMap{ctx->nxh, (long)ctx->nxh * ctx->ny, (long)ctx->nxh, ctx->nxh * ctx->nz}; }
+// y lines: (kx, z) runs of nxp, element stride nxp; z lines: (kx, ky) runs of
+// nxp, element stride nxp*ny.  Padding lines (kx >= nxh) are skipped.
+LineMap ymap(effpot_ctx* ctx) {
+  return LineMap{ctx->nxp, (long)ctx->nxp * ctx->ny, (long)ctx->nxp, ctx->nxp * ctx->nz, ctx->nxh};
+}
 LineMap zmap(effpot_ctx* ctx) {
-  return LineMap{ctx->nxh * ctx->ny, 0, (long)ctx->nxh * ctx->ny, ctx->nxh * ctx->ny};
+  return LineMap{ctx->nxp, (long)ctx->nxp, (long)ctx->nxp * ctx->ny, ctx->nxp * ctx->ny, ctx->nxh};
 }
 template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx<T>* const* f, int nf, int sign, const int* act) {
   StridedArgs<T> a{};
@@ -688,7 +695,7 @@ int check_ctx(effpot_ctx* ctx) { return ctx ? EFFPOT_OK : EFFPOT_ERR_ARG; }
 
 int64_t effpot_footprint_bytes(const effpot_params* p) {
   if (!p) return -1;
-  const int64_t N = (int64_t)p->nx * p->ny * p->nz, M = (int64_t)(p->nx / 2 + 1) * p->ny * p->nz;
+  const int64_t N = (int64_t)p->nx * p->ny * p->nz, M = (int64_t)((p->nx / 2 + 1 + 15) / 16 * 16) * p->ny * p->nz;
   const int64_t per64 = (2 * 9 + 20) * N * 8 + 8 * M * 16;
   int64_t per = per64;
   if (p->precision == EFFPOT_PREC_MIXED) per += per64 / 2;
@@ -709,7 +716,8 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
   ctx->nx = p->nx; ctx->ny = p->ny; ctx->nz = p->nz; ctx->nxh = p->nx / 2 + 1; ctx->B = p->batch;
-  ctx->N = (long)p->nx * p->ny * p->nz; ctx->M = (long)ctx->nxh * p->ny * p->nz;
+  ctx->nxp = (ctx->nxh + 15) / 16 * 16;
+  ctx->N = (long)p->nx * p->ny * p->nz; ctx->M = (long)ctx->nxp * p->ny * p->nz;
   ctx->hx = 2.0 * std::fabs(p->xmax) / p->nx;
   ctx->hy = 2.0 * std::fabs(p->ymax) / p->ny;
   ctx->hz = 2.0 * std::fabs(p->zmax) / p->nz;
@@ -767,7 +775,8 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
       hipMemcpy(ctx->d_ones, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   // one-time k-space kernels in FP64 (fforma / initcg / kalfs)
-  KernParams kp{p->nx, p->ny, p->nz, ctx->nxh, ctx->hx, ctx->hy, ctx->hz, p->eps4, p->sigma4, p->h_core, p->l};
+  KernParams kp{p->nx, p->ny, p->nz, ctx->nxh, ctx->hx, ctx->hy, ctx->hz, p->eps4, p->sigma4, p->h_core, p->l,
+                ctx->nxp};
   hipLaunchKernelGGL(k_kspace, dim3(512), dim3(256), 0, ctx->st, kp, ctx->s64.kern[0], ctx->s64.kern[1],
                      ctx->s64.kern[2]);
   if (hipGetLastError() != hipSuccess) return bail(EFFPOT_ERR_DEVICE);
@@ -1062,7 +1071,8 @@ int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out) {
   RC(fft_x_r2c(ctx, s, r, c, 1, ctx->d_mask_tmp));
   RC(fft_strided(ctx, s, 1, c, 1, -1, ctx->d_mask_tmp));
   RC(fft_strided(ctx, s, 2, c, 1, -1, ctx->d_mask_tmp));
-  HIPCHK(hipMemcpyAsync(out, s.spec[0], sizeof(double) * 2 * ctx->M, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<double>) * ctx->nxh, s.spec[0], sizeof(cx<double>) * ctx->nxp,
+                          sizeof(cx<double>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }
@@ -1075,7 +1085,8 @@ int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out) {
   std::vector<int> m(ctx->B, 0);
   m[0] = 1;
   HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipMemcpyAsync(s.spec[0], in, sizeof(double) * 2 * ctx->M, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpy2DAsync(s.spec[0], sizeof(cx<double>) * ctx->nxp, in, sizeof(cx<double>) * ctx->nxh,
+                          sizeof(cx<double>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyHostToDevice, ctx->st));
   cx<double>* c[1] = {s.spec[0]};
   double* r[1] = {s.src[0]};
   RC(fft_strided(ctx, s, 2, c, 1, +1, ctx->d_mask_tmp));
@@ -1265,8 +1276,42 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
   return EFFPOT_OK;
 }
 
+// The corrector timed inside whole predictor-corrector steps: the caches hold
+// what a relaxation leaves in them (poten runs in between), not the
+// corrector's own previous launch.
+template <typename T> static int time_corrector_in_steps(effpot_ctx* ctx, State<T>& s, int reps, double* ms,
+                                                         bool f32) {
+  const int* act = ctx->d_active;
+  std::vector<hipEvent_t> ev(2 * reps);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = EFFPOT_OK;
+  for (int r = 0; r < reps && rc == EFFPOT_OK; r++) {
+    rc = launch_step<T, 0>(ctx, s, act, f32);
+    if (rc == EFFPOT_OK) rc = f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act);
+    if (rc != EFFPOT_OK) break;
+    HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
+    ctx->corr_den = false;
+    rc = launch_step<T, 1>(ctx, s, act, f32);
+    ctx->corr_den = true;
+    HIPCHK(hipEventRecord(ev[2 * r + 1], ctx->st));
+    if (rc == EFFPOT_OK) rc = reduce<2>(ctx, 1, act, s.ring, RING_AFTER_PRED, RING_AFTER_CORR);
+    if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, act, f32);
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  double sum = 0.0;
+  for (int r = 0; r < reps && rc == EFFPOT_OK; r++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]));
+    sum += t;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  *ms = sum / reps;
+  return rc;
+}
+
 template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int which, int reps, double* ms,
                                                bool f32 = false) {
+  if (which == 3) return time_corrector_in_steps(ctx, s, reps, ms, f32);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));

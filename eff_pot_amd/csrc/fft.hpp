@@ -148,16 +148,21 @@ inline bool fft_size_supported(int n) {
 }
 
 // Accessor describing where line c, element i lives in global memory.
-// Lines c = cx + ncx*co; address = base + cx*1 + co*s_outer + i*s_elem.
+// Lines c = cx + ncx*co; address = base + cx*1 + co*s_outer + i*s_elem.  A run
+// is a padded row of the half spectrum: ncx (a multiple of 16) lines of which
+// the first nvalid exist, so groups of 16 consecutive lines start on 128-byte
+// boundaries; the padding lines are neither read nor written.
 struct LineMap {
-  int ncx;          // lines per contiguous run (nxh)
+  int ncx;          // lines per run (padded row length)
   long s_outer;     // stride between runs
   long s_elem;      // stride between elements of a line
-  int nlines;       // lines per field and slot
+  int nlines;       // lines per field and slot (padding included)
+  int nvalid;       // lines of a run that exist (nx/2+1)
   EP_DEV long addr(int c, int i) const {
     int co = c / ncx, ci = c - co * ncx;
     return (long)ci + (long)co * s_outer + (long)i * s_elem;
   }
+  EP_DEV bool valid(int c) const { return c < nlines && c - (c / ncx) * ncx < nvalid; }
 };
 
 // ---- the two-level transform, step by step -------------------------------------
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs<T> a) {
   const int c0 = blockIdx.x * C;
   auto load = [&](int c, int i) -> cx<T> {
     int cc = c0 + c;
-    return cc < lm.nlines ? d[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
+    return lm.valid(cc) ? d[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
   };
   fft_step1<T, N>(lds, a.twN, a.sign, load);
   __syncthreads();
@@ -257,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs<T> a) {
   fft_step3<T, N>(lds, b, a.sign);
   const int c = threadIdx.x % C, k1 = threadIdx.x / C;
   const int cc = c0 + c;
-  if (cc < lm.nlines) {
+  if (lm.valid(cc)) {
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) d[lm.addr(cc, k1 + N1 * k2)] = b[k2];
   }
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
   const int c0 = blockIdx.x * C;
   auto load = [&](int c, int i) -> cx<T> {
     int cc = c0 + c;
-    return cc < lm.nlines ? in[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
+    return lm.valid(cc) ? in[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
   };
   fft_step1<T, N>(lds, a.twN, -1, load);
   __syncthreads();
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
   fft_step3<T, N>(lds, spec, -1);
   const int c = threadIdx.x % C, kk1 = threadIdx.x / C;
   const int cc = c0 + c;
-  const bool valid = cc < lm.nlines;
+  const bool valid = lm.valid(cc);
   const int nout = a.nout[g];
   for (int j = 0; j < nout; j++) {
     const T* K = a.kern[g][j];
@@ -311,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
     cx<T>* out = a.out[g][j] + so;
     auto store = [&](int cl, int i, cx<T> val) {
       int c2 = c0 + cl;
-      if (c2 < lm.nlines) out[lm.addr(c2, i)] = val;
+      if (lm.valid(c2)) out[lm.addr(c2, i)] = val;
     };
     fft_from_regs<T, N>(v, lds, a.twN, +1, store);
   }
@@ -325,6 +330,7 @@ template <typename T> struct XArgs {
   cx<T>* c[4];
   long rslot, cslot;
   int nrows;  // ny*nz (even)
+  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
   T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
   T aff0[4], aff1[4];
   const cx<T>* twN;
@@ -390,9 +396,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
     cx<T> z = lds[k * P + c], zm = lds[((N - k) % N) * P + c];
     cx<T> s = cx<T>{z.x + zm.x, z.y - zm.y};  // z + conj(zm)
     cx<T> d = cx<T>{z.x - zm.x, z.y + zm.y};  // z - conj(zm)
-    cx<T>* oa = dst + (long)(2 * pr) * NH;
+    cx<T>* oa = dst + (long)(2 * pr) * rm.pitch;
     oa[k] = cx<T>{T(0.5) * s.x, T(0.5) * s.y};
-    oa[NH + k] = cx<T>{T(0.5) * d.y, T(-0.5) * d.x};
+    oa[rm.pitch + k] = cx<T>{T(0.5) * d.y, T(-0.5) * d.x};
   }
 }
 
@@ -417,8 +423,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
     const int pr = pair0 + c;
     cx<T> A{T(0), T(0)}, B{T(0), T(0)};
     if (pr < npairs) {
-      const cx<T>* ra = src + (long)(2 * pr) * NH;
-      A = ra[k]; B = ra[NH + k];
+      const cx<T>* ra = src + (long)(2 * pr) * rm.pitch;
+      A = ra[k]; B = ra[rm.pitch + k];
     }
     if (k == 0 || k == N / 2) { A.y = T(0); B.y = T(0); }
     lds[k * P + c] = cx<T>{A.x - B.y, A.y + B.x};

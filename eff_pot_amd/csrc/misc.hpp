@@ -83,13 +83,18 @@ EP_DEV double pfreq_dev(int i, int n, double h) {
 struct KernParams {
   int nx, ny, nz, nxh;
   double hx, hy, hz, eps, sigma, hc, l;
+  int nxp;  // padded row length of the half grid
 };
 // fvlj4 (fforma), wcgk (initcg), kalfs (DFT4He3d.f90:396-400) on the half grid.
 __global__ void k_kspace(KernParams kp, double* wcgk, double* fvlj, double* kalfs) {
-  long M = (long)kp.nxh * kp.ny * kp.nz;
+  long M = (long)kp.nxp * kp.ny * kp.nz;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < M; k += (long)gridDim.x * blockDim.x) {
-    int ix = (int)(k % kp.nxh);
-    long r = k / kp.nxh;
+    int ix = (int)(k % kp.nxp);
+    long r = k / kp.nxp;
+    if (ix >= kp.nxh) {  // row padding
+      fvlj[k] = 0.0; wcgk[k] = 0.0; kalfs[k] = 0.0;
+      continue;
+    }
     int iy = (int)(r % kp.ny), iz = (int)(r / kp.ny);
     double px = (1.0 / (kp.nx * kp.hx)) * ix;
     double py = pfreq_dev(iy, kp.ny, kp.hy), pz = pfreq_dev(iz, kp.nz, kp.hz);

@@ -218,8 +218,9 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
  * launch of a hot-path kernel on the context's stream, measured with HIP
  * events over `reps` back-to-back launches on the current (live-precision)
  * state: 0 = H psi + predictor/modifier, 1 = H psi + corrector, 2 = the whole
- * poten() kernel sequence.  The launches overwrite the integrator state:
- * call it after the solves that matter. */
+ * poten() kernel sequence, 3 = the corrector timed inside `reps` whole
+ * predictor-corrector steps (caches as in a relaxation).  The launches
+ * overwrite the integrator state: call it after the solves that matter. */
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
