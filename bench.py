@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--tol", type=float, default=1e-9)
-    ap.add_argument("--tol-fp32", type=float, default=1e-9)
+    ap.add_argument("--tol-fp32", type=float, default=None,
+                    help="|dE/E| per step that ends the FP32 phase of a mixed solve (default 1.02 tol)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="PC steps timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -62,7 +63,10 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.5)
     ap.add_argument("--stress-steps", type=int, default=300)
     ap.add_argument("--seed", type=int, default=20261203, help="noise seed of the initial droplets")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.tol_fp32 is None:
+        a.tol_fp32 = 1.02 * a.tol
+    return a
 
 
 def workload(n, mode, batch):

@@ -175,6 +175,7 @@ struct effpot_ctx {
   int* d_mask = nullptr;      // [5][B] per-purpose activity masks of effpot_tick
   int* h_mask = nullptr;      // pinned staging for d_mask
   std::vector<int> mask_dev;  // what d_mask holds (-1: unknown)
+  std::vector<double> h_relprev;  // |dE/E| per step at each slot's previous check
   bool corr_den = true;       // corrector stores the density (false when a renormalisation follows)
   double* h_pin = nullptr;    // pinned host buffer
   int tx = 64, zc = 16;
@@ -760,6 +761,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_dt, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_dt.assign(ctx->B, 0.0);
   ctx->mask_dev.assign((size_t)ctx->B * 4, -1);
+  ctx->h_relprev.assign(ctx->B, 0.0);
   ctx->dt_dev.assign(ctx->B, -1.0);
   ctx->h_n4.assign(ctx->B, p->n4);
   if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
@@ -1164,8 +1166,17 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
   const int* dP = ctx->d_mask + 2 * B;
   const int* dE = ctx->d_mask + 3 * B;
   std::vector<effpot_energies> e(B);
-  const double tol = f32 ? opt->tol_fp32 : opt->tol;
-  const int cap = f32 || !ctx->mixed ? opt->max_steps : opt->max_steps + std::max(opt->polish_steps, 0);
+  // Every energy is FP64 (FP64 state, FP64 poten at the check), but the rate
+  // at which it falls depends on the fields that advanced the state, and near
+  // the end of a relaxation |dE/E| per step decreases so slowly that a 0.2 %
+  // bias moves its crossing of tol by several checks.  So convergence is
+  // decided in the FP64 phase only: an FP32 slot switches to FP64 fields (the
+  // polish) when its |dE/E| per step is below tol_fp32, or when the linear
+  // extrapolation of the last two checks crosses tol at the next check, or at
+  // max_steps; it is DONE at the first FP64 check below tol, or after
+  // max_steps + polish_steps.
+  const int cap_f32 = opt->max_steps;
+  const int cap = ctx->mixed ? opt->max_steps + std::max(opt->polish_steps, 0) : opt->max_steps;
   // START: fields of the new states; they take their first RK step below.
   if (any_s) {
     RC(push_mask(ctx, 0, ms));
@@ -1179,6 +1190,7 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
       if (ms[b]) {  // a new solve: counters start from zero
         sl[b].nrk = 0; sl[b].since = 0; sl[b].steps = 0; sl[b].converged = 0;
         sl[b].eprev = 0.0; sl[b].rel_de = 0.0;
+        ctx->h_relprev[b] = 0.0;
         if (opt->n_rk > 0) { sl[b].phase = EFFPOT_SLOT_RK; mr[b] = 1; any_r = true; }
         else { sl[b].phase = EFFPOT_SLOT_PC; }
       }
@@ -1242,12 +1254,12 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
       }
       double rel = std::fabs(e[b].etot - q.eprev) / std::fabs(e[b].etot) / (double)std::max(q.since, 1);
       q.rel_de = rel; q.eprev = e[b].etot; q.e = e[b]; q.since = 0;
-      if (rel < tol || q.steps >= cap) {
-        if (f32) {
-          q.fp32 = 0;  // FP64 polish: same state and history, FP64 fields from here on
-        } else {
-          q.phase = EFFPOT_SLOT_DONE; q.converged = rel < tol;
-        }
+      const double prev = ctx->h_relprev[b];
+      ctx->h_relprev[b] = rel;
+      if (!f32) {
+        if (rel < opt->tol || q.steps >= cap) { q.phase = EFFPOT_SLOT_DONE; q.converged = rel < opt->tol; }
+      } else if (rel < opt->tol_fp32 || (prev > 0.0 && rel + (rel - prev) < opt->tol) || q.steps >= cap_f32) {
+        q.fp32 = 0;  // FP64 polish: same state and history, FP64 fields from here on
       }
     }
   }

@@ -118,8 +118,9 @@ typedef struct effpot_solve_opts {
   int32_t n_rk;     /* Runge-Kutta-Gill starter steps (3)                  */
   int32_t pener;    /* steps between energy evaluations (DFT4He3d pener)   */
   int32_t max_steps;/* hard cap on steps (not counting polish)             */
-  int32_t polish_steps; /* MIXED only: FP64 steps after the FP32 phase      */
-  double tol_fp32;  /* MIXED only: per-step |dE/E| that ends the FP32 phase */
+  int32_t polish_steps; /* MIXED only: cap on FP64 steps after the FP32 phase */
+  double tol_fp32;  /* MIXED only: per-step |dE/E| that ends the FP32 phase
+                       (>= tol, e.g. 1.02 tol)                             */
   int32_t check_finite; /* 1: scan the density for NaN/<=0 at every energy */
   int32_t pad_;
 } effpot_solve_opts;
@@ -224,10 +225,14 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
- * one step of its own phase (a START slot gets poten + its first RK step;
- * a MIXED slot switches to FP64 polish when its FP32 |dE/E| per step drops
- * below tol_fp32; a slot is DONE when its FP64 |dE/E| per step < tol or it
- * has taken max_steps).  `slots[batch]` is host memory owned by the caller;
+ * one step of its own phase (a START slot gets poten + its first RK step).
+ * Energies are FP64 in every mode; a slot is DONE at the first energy check
+ * with FP64 fields where |dE/E| per step < tol, or after max_steps (plus
+ * polish_steps for MIXED).  A MIXED slot advances with FP32 mean fields
+ * (defect-corrected from the FP64 field at every check) and switches to FP64
+ * fields (the polish) when |dE/E| per step < tol_fp32, when the trend of the
+ * last two checks reaches tol at the next one, or after max_steps.
+ * `slots[batch]` is host memory owned by the caller;
  * set a slot to START after effpot_init_fermi / effpot_upload_psi and
  * effpot_set_impurity.  No reference counterpart (the reference has no
  * driver loop); the solve of one slot is the same whatever the other slots do. */
