@@ -175,7 +175,8 @@ struct effpot_ctx {
   int* d_mask = nullptr;      // [5][B] per-purpose activity masks of effpot_tick
   int* h_mask = nullptr;      // pinned staging for d_mask
   std::vector<int> mask_dev;  // what d_mask holds (-1: unknown)
-  std::vector<double> h_relprev;  // |dE/E| per step at each slot's previous check
+  std::vector<double> h_relprev;   // |dE/E| per step at each slot's previous check
+  std::vector<double> h_reltrend;  // its smoothed change per check
   bool corr_den = true;       // corrector stores the density (false when a renormalisation follows)
   double* h_pin = nullptr;    // pinned host buffer
   int tx = 64, zc = 16;
@@ -494,32 +495,36 @@ template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring
 #define RC(x) do { int rc_ = (x); if (rc_ != EFFPOT_OK) return rc_; } while (0)
 
 // ---- poten (poten.f90:5-131 + term_alfa.f90:5-109) ----------------------------
-template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) {
+// `only` >= 0 runs just launch number `only` of the sequence (benchmark hook:
+// 0-4 stage A x r2c, y, z convolution, y, x c2r; 5 k_grad; 6-10 stage B;
+// 11 k_div; 12-16 stage C; 17 k_pot4).
+template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1) {
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
+  auto sel = [only](int k) { return only < 0 || only == k; };
   // stage A: FFT(den) -> dencg, delj4 [, denalf]
   {
     T* in[1] = {s.den};
     cx<T>* sp[1] = {s.spec[0]};
-    RC(fft_x_r2c(ctx, s, in, sp, 1, act));
-    RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    if (sel(0)) RC(fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(1)) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[0];
     ca.nout[0] = otc ? 3 : 2;
     ca.out[0][0] = s.spec[1]; ca.kern[0][0] = s.kern[0];
     ca.out[0][1] = s.spec[2]; ca.kern[0][1] = s.kern[1];
     ca.out[0][2] = s.spec[3]; ca.kern[0][2] = s.kern[2];
-    RC(fft_zconv(ctx, s, ca, 1, act));
+    if (sel(2)) RC(fft_zconv(ctx, s, ca, 1, act));
     cx<T>* o[3] = {s.spec[1], s.spec[2], s.spec[3]};
-    RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
+    if (sel(3)) RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
     // the alpha_s convolution is stored as falfs = 1 - denalf/den0s (term_alfa.f90:157-159),
     // the only form its consumers use
     T* r[3] = {s.dencg, s.delj4, s.denalf};
     const T a0[3] = {T(0), T(0), T(1)};
     const T a1[3] = {T(1), T(1), T(-1.0 / ctx->p.den0s)};
-    RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
+    if (sel(4)) RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
   }
   // derden + FFT sources
-  {
+  if (sel(5)) {
     PotenArgs<T> a{};
     a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.otc = otc;
     a.cp4 = (T)ctx->p.cp4; a.cpp4 = (T)ctx->p.cpp4; a.den0s = (T)ctx->p.den0s;
@@ -536,8 +541,8 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
     const int nf = otc ? 4 : 1;
     T* in[4] = {s.src[0], s.src[1], s.src[2], s.src[3]};
     cx<T>* sp[4] = {s.spec[4], s.spec[5], s.spec[6], s.spec[7]};
-    RC(fft_x_r2c(ctx, s, in, sp, nf, act));
-    RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
+    if (sel(6)) RC(fft_x_r2c(ctx, s, in, sp, nf, act));
+    if (sel(7)) RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
     ConvArgs<T> ca{};
     for (int g = 0; g < nf; g++) {
       ca.in[g] = sp[g];
@@ -545,11 +550,11 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
       ca.out[g][0] = s.spec[g];
       ca.kern[g][0] = g == 0 ? s.kern[0] : s.kern[2];
     }
-    RC(fft_zconv(ctx, s, ca, nf, act));
+    if (sel(8)) RC(fft_zconv(ctx, s, ca, nf, act));
     cx<T>* o[4] = {s.spec[0], s.spec[1], s.spec[2], s.spec[3]};
-    RC(fft_strided(ctx, s, 1, o, nf, +1, act));
+    if (sel(9)) RC(fft_strided(ctx, s, 1, o, nf, +1, act));
     T* r[4] = {s.sto1, s.intx, s.inty, s.intz};
-    RC(fft_x_c2r(ctx, s, o, r, nf, T(1), act));
+    if (sel(10)) RC(fft_x_c2r(ctx, s, o, r, nf, T(1), act));
   }
   if (otc) {
     DivArgs<T> a{};
@@ -558,23 +563,25 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act) 
     a.src = s.src[0]; a.div = s.div;
     dim3 g = sgrid(ctx);
 #define M_(TXX) hipLaunchKernelGGL((k_div<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
-    TX_SWITCH(ctx->tx, M_)
+    if (sel(11)) {
+      TX_SWITCH(ctx->tx, M_)
+      LAUNCHCHK();
+    }
 #undef M_
-    LAUNCHCHK();
     // stage C: FFT(sum_i d_i den int_i) * kalfs
     T* in[1] = {s.src[0]};
     cx<T>* sp[1] = {s.spec[4]};
-    RC(fft_x_r2c(ctx, s, in, sp, 1, act));
-    RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    if (sel(12)) RC(fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(13)) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[4]; ca.nout[0] = 1; ca.out[0][0] = s.spec[0]; ca.kern[0][0] = s.kern[2];
-    RC(fft_zconv(ctx, s, ca, 1, act));
+    if (sel(14)) RC(fft_zconv(ctx, s, ca, 1, act));
     cx<T>* o[1] = {s.spec[0]};
-    RC(fft_strided(ctx, s, 1, o, 1, +1, act));
+    if (sel(15)) RC(fft_strided(ctx, s, 1, o, 1, +1, act));
     T* r[1] = {s.ua};
-    RC(fft_x_c2r(ctx, s, o, r, 1, T(1), act));
+    if (sel(16)) RC(fft_x_c2r(ctx, s, o, r, 1, T(1), act));
   }
-  {
+  if (sel(17)) {
     Pot4Args<T> a{};
     a.N = ctx->N; a.slot = ctx->N; a.active = act; a.otc = otc;
     a.a0 = (T)(ctx->p.cp4 / 2.0); a.a1 = (T)(ctx->p.cpp4 / 3.0); a.den0s = (T)ctx->p.den0s;
@@ -762,6 +769,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_dt.assign(ctx->B, 0.0);
   ctx->mask_dev.assign((size_t)ctx->B * 4, -1);
   ctx->h_relprev.assign(ctx->B, 0.0);
+  ctx->h_reltrend.assign(ctx->B, 0.0);
   ctx->dt_dev.assign(ctx->B, -1.0);
   ctx->h_n4.assign(ctx->B, p->n4);
   if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
@@ -1150,6 +1158,11 @@ static int pc_step_graph(effpot_ctx* ctx, State<T>& s, bool f32, const int* dP) 
 // Advance the slots whose field precision matches f32 (FP32 mean fields of a
 // MIXED context, else FP64) by one step of their own phase.  The wavefunction,
 // its history and the integrator always live in the FP64 state.
+// A mixed slot hands over to FP64 fields this many energy checks before its
+// |dE/E| trend reaches tol: the FP32-phase rate of decrease is biased by a few
+// checks' worth near the end of a relaxation.
+constexpr double kSwitchChecks = 10.0;
+
 static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, effpot_slot* sl) {
   State<double>& s = ctx->s64;
   const int B = ctx->B;
@@ -1171,10 +1184,11 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
   // the end of a relaxation |dE/E| per step decreases so slowly that a 0.2 %
   // bias moves its crossing of tol by several checks.  So convergence is
   // decided in the FP64 phase only: an FP32 slot switches to FP64 fields (the
-  // polish) when its |dE/E| per step is below tol_fp32, or when the linear
-  // extrapolation of the last two checks crosses tol at the next check, or at
-  // max_steps; it is DONE at the first FP64 check below tol, or after
-  // max_steps + polish_steps.
+  // polish) when its |dE/E| per step is below tol_fp32, or when it is below
+  // 2 tol and its trend (smoothed change per check) would reach tol within
+  // kSwitchChecks checks, or at max_steps; it is DONE at the first FP64 check
+  // below tol, or after max_steps + polish_steps.  (The early, fast decay is
+  // not extrapolated: it would hand almost the whole relaxation to FP64.)
   const int cap_f32 = opt->max_steps;
   const int cap = ctx->mixed ? opt->max_steps + std::max(opt->polish_steps, 0) : opt->max_steps;
   // START: fields of the new states; they take their first RK step below.
@@ -1191,6 +1205,7 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
         sl[b].nrk = 0; sl[b].since = 0; sl[b].steps = 0; sl[b].converged = 0;
         sl[b].eprev = 0.0; sl[b].rel_de = 0.0;
         ctx->h_relprev[b] = 0.0;
+        ctx->h_reltrend[b] = 0.0;
         if (opt->n_rk > 0) { sl[b].phase = EFFPOT_SLOT_RK; mr[b] = 1; any_r = true; }
         else { sl[b].phase = EFFPOT_SLOT_PC; }
       }
@@ -1256,9 +1271,15 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
       q.rel_de = rel; q.eprev = e[b].etot; q.e = e[b]; q.since = 0;
       const double prev = ctx->h_relprev[b];
       ctx->h_relprev[b] = rel;
+      if (prev > 0.0) {
+        const double d = rel - prev;
+        ctx->h_reltrend[b] = ctx->h_reltrend[b] == 0.0 ? d : 0.7 * ctx->h_reltrend[b] + 0.3 * d;
+      }
+      const double trend = ctx->h_reltrend[b];
       if (!f32) {
         if (rel < opt->tol || q.steps >= cap) { q.phase = EFFPOT_SLOT_DONE; q.converged = rel < opt->tol; }
-      } else if (rel < opt->tol_fp32 || (prev > 0.0 && rel + (rel - prev) < opt->tol) || q.steps >= cap_f32) {
+      } else if (rel < opt->tol_fp32 || (rel < 2.0 * opt->tol && trend < 0.0 && rel + kSwitchChecks * trend < opt->tol) ||
+                 q.steps >= cap_f32) {
         q.fp32 = 0;  // FP64 polish: same state and history, FP64 fields from here on
       }
     }
@@ -1337,7 +1358,10 @@ template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int
         return rc;
       }
       case 2: return f32 ? poten_t(ctx, ctx->s32, ctx->d_active) : poten_t(ctx, s, ctx->d_active);
-      default: return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
+      default:
+        if (which >= 100 && which < 118)
+          return f32 ? poten_t(ctx, ctx->s32, ctx->d_active, which - 100) : poten_t(ctx, s, ctx->d_active, which - 100);
+        return ctx->fail(EFFPOT_ERR_ARG, "unknown kernel id");
     }
   };
   RC(once());

@@ -120,7 +120,7 @@ typedef struct effpot_solve_opts {
   int32_t max_steps;/* hard cap on steps (not counting polish)             */
   int32_t polish_steps; /* MIXED only: cap on FP64 steps after the FP32 phase */
   double tol_fp32;  /* MIXED only: per-step |dE/E| that ends the FP32 phase
-                       (>= tol, e.g. 1.02 tol)                             */
+                       (>= tol)                                            */
   int32_t check_finite; /* 1: scan the density for NaN/<=0 at every energy */
   int32_t pad_;
 } effpot_solve_opts;
@@ -220,7 +220,10 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
  * events over `reps` back-to-back launches on the current (live-precision)
  * state: 0 = H psi + predictor/modifier, 1 = H psi + corrector, 2 = the whole
  * poten() kernel sequence, 3 = the corrector timed inside `reps` whole
- * predictor-corrector steps (caches as in a relaxation).  The launches
+ * predictor-corrector steps (caches as in a relaxation), 100 + k = launch k
+ * of the poten() sequence alone (0-4 stage A: x r2c, y, z convolution, y,
+ * x c2r; 5 derden; 6-10 stage B; 11 alpha_s divergence; 12-16 stage C;
+ * 17 field assembly).  The launches
  * overwrite the integrator state: call it after the solves that matter. */
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
 
@@ -230,8 +233,9 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
  * with FP64 fields where |dE/E| per step < tol, or after max_steps (plus
  * polish_steps for MIXED).  A MIXED slot advances with FP32 mean fields
  * (defect-corrected from the FP64 field at every check) and switches to FP64
- * fields (the polish) when |dE/E| per step < tol_fp32, when the trend of the
- * last two checks reaches tol at the next one, or after max_steps.
+ * fields (the polish) when |dE/E| per step < tol_fp32, when it is below
+ * 2 tol and its trend would reach tol within 10 energy checks, or after
+ * max_steps.
  * `slots[batch]` is host memory owned by the caller;
  * set a slot to START after effpot_init_fermi / effpot_upload_psi and
  * effpot_set_impurity.  No reference counterpart (the reference has no
