@@ -296,22 +296,29 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
     int cc = c0 + c;
     return lm.valid(cc) ? in[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
   };
-  fft_step1<T, N>(lds, a.twN, -1, load);
-  __syncthreads();
-  cx<T> spec[N2];
-  fft_step3<T, N>(lds, spec, -1);
   const int c = threadIdx.x % C, kk1 = threadIdx.x / C;
   const int cc = c0 + c;
   const bool valid = lm.valid(cc);
   const int nout = a.nout[g];
-  for (int j = 0; j < nout; j++) {
+  // k-space factors of this thread's outputs: the first output's are loaded
+  // before the forward transform, each next output's during the previous
+  // inverse transform, so their latency overlaps arithmetic
+  T kn[N2];
+  auto load_k = [&](int j) {
     const T* K = a.kern[g][j];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[lm.addr(cc, kk1 + N1 * k2)] : T(0);
+  };
+  load_k(0);
+  fft_step1<T, N>(lds, a.twN, -1, load);
+  __syncthreads();
+  cx<T> spec[N2];
+  fft_step3<T, N>(lds, spec, -1);
+  for (int j = 0; j < nout; j++) {
     cx<T> v[N2];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) {
-      T kv = valid ? K[lm.addr(cc, kk1 + N1 * k2)] * a.scale : T(0);
-      v[k2] = kv * spec[k2];
-    }
+    for (int k2 = 0; k2 < N2; k2++) v[k2] = (kn[k2] * a.scale) * spec[k2];
+    if (j + 1 < nout) load_k(j + 1);
     __syncthreads();  // LDS reuse
     cx<T>* out = a.out[g][j] + so;
     auto store = [&](int cl, int i, cx<T> val) {
