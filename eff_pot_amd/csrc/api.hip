@@ -1411,6 +1411,7 @@ int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc) {
   int tx0 = ctx->tx, zc0 = ctx->zc;
   ctx->tx = tx; ctx->zc = zc;
   if (nblk(ctx) > ctx->nblk_max) { ctx->tx = tx0; ctx->zc = zc0; return ctx->fail(EFFPOT_ERR_ARG, "tile exceeds partial buffer"); }
+  HIPCHK(hipStreamSynchronize(ctx->st));  // a replay may still be running
   for (int k = 0; k < 2; k++)
     if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
   return EFFPOT_OK;

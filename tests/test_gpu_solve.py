@@ -79,6 +79,21 @@ def test_converged_mixed_vs_oracle_and_fp64(gpu_available):
     assert l2 < 1e-4
 
 
+def test_mixed_stops_where_fp64_stops(gpu_available):
+    """With tol_fp32 = tol (the benchmark's setting) the FP32 phase hands over
+    to FP64 fields on the |dE/E| trend, so the mixed solve stops at the same
+    energy check as the FP64 one and E_imp agrees far below 1e-3 K."""
+    print("[conv32] mixed, tol_fp32 = tol", flush=True)
+    g, c64, r64 = _conv32(0)
+    c64.close()
+    g, c32, r32 = _conv32(PREC_MIXED, tol_fp32=g["config"]["tol"])
+    c32.close()
+    print(f"steps mixed {r32.steps} fp64 {r64.steps}, dE_imp {r32.e.eimpu - r64.e.eimpu:.2e} K")
+    assert r32.converged and r64.converged
+    assert r32.steps == r64.steps
+    assert abs(r32.e.eimpu - r64.e.eimpu) < 1e-5
+
+
 # requests (z0, seed, Morse D) and the slot each takes in the mix: request 3
 # reuses slot 0 as soon as request 0 is done, while requests 1 and 2 run
 REQS = [(0.5, 1, 25.0), (1.5, 2, 30.0), (-2.0, 3, 35.0), (1.0, 4, 28.0)]
