@@ -172,7 +172,7 @@ struct effpot_ctx {
   std::vector<double> dt_dev; // what d_dt holds
   std::vector<double> h_n4;
   int* d_bad = nullptr;
-  int* d_mask = nullptr;      // [5][B] per-purpose activity masks of effpot_tick
+  int* d_mask = nullptr;      // [7][B] per-purpose activity masks of effpot_tick
   int* h_mask = nullptr;      // pinned staging for d_mask
   std::vector<int> mask_dev;  // what d_mask holds (-1: unknown)
   std::vector<double> h_relprev;   // |dE/E| per step at each slot's previous check
@@ -183,6 +183,10 @@ struct effpot_ctx {
   bool graphs = true;               // replay captured PC steps (EFFPOT_GRAPHS=0 disables)
   bool tune_log = false;            // EFFPOT_TUNE_LOG=1 prints the autotuner's timings
   hipGraphExec_t pc_exec[2] = {nullptr, nullptr};  // [0] FP64, [1] FP32
+  hipGraphExec_t pc_exec2[2] = {nullptr, nullptr}; // the same as two concurrent slot groups
+  hipStream_t st2 = nullptr;                       // second stream of the two-group graph
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool split = true;                               // two slot groups when B >= 2 (EFFPOT_SPLIT=0 disables)
   std::vector<void*> allocs;
   std::string err;
 
@@ -723,6 +727,12 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->dev = device;
   if (hipSetDevice(device) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) { delete ctx; return EFFPOT_ERR_DEVICE; }
+  if (hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+    effpot_ctx_destroy(ctx);
+    return EFFPOT_ERR_DEVICE;
+  }
   ctx->nx = p->nx; ctx->ny = p->ny; ctx->nz = p->nz; ctx->nxh = p->nx / 2 + 1; ctx->B = p->batch;
   ctx->nxp = (ctx->nxh + 15) / 16 * 16;
   ctx->N = (long)p->nx * p->ny * p->nz; ctx->M = (long)ctx->nxp * p->ny * p->nz;
@@ -739,6 +749,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->mixed = p->precision == EFFPOT_PREC_MIXED;
   ctx->cplx = p->psi_complex != 0;
   if (const char* g = std::getenv("EFFPOT_GRAPHS")) ctx->graphs = std::atoi(g) != 0;
+  if (const char* g = std::getenv("EFFPOT_SPLIT")) ctx->split = std::atoi(g) != 0;
   if (const char* g = std::getenv("EFFPOT_TUNE_LOG")) ctx->tune_log = std::atoi(g) != 0;
   ctx->h_active.assign(ctx->B, 1);
   ctx->have_imp.assign(ctx->B, 0);
@@ -767,7 +778,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_n4, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_dt, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_dt.assign(ctx->B, 0.0);
-  ctx->mask_dev.assign((size_t)ctx->B * 4, -1);
+  ctx->mask_dev.assign((size_t)ctx->B * 7, -1);
   ctx->h_relprev.assign(ctx->B, 0.0);
   ctx->h_reltrend.assign(ctx->B, 0.0);
   ctx->dt_dev.assign(ctx->B, -1.0);
@@ -775,8 +786,8 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_bad, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
-  if ((rc = dalloc(ctx, &ctx->d_mask, (size_t)ctx->B * 5)) != EFFPOT_OK) return bail(rc);
-  if (hipHostMalloc((void**)&ctx->h_mask, sizeof(int) * ctx->B * 5, hipHostMallocDefault) != hipSuccess)
+  if ((rc = dalloc(ctx, &ctx->d_mask, (size_t)ctx->B * 7)) != EFFPOT_OK) return bail(rc);
+  if (hipHostMalloc((void**)&ctx->h_mask, sizeof(int) * ctx->B * 7, hipHostMallocDefault) != hipSuccess)
     return bail(EFFPOT_ERR_NOMEM);
   if (hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * ctx->B * 16 + 64, hipHostMallocDefault) != hipSuccess)
     return bail(EFFPOT_ERR_NOMEM);
@@ -804,11 +815,16 @@ int effpot_ctx_destroy(effpot_ctx* ctx) {
   if (!ctx) return EFFPOT_ERR_ARG;
   hipSetDevice(ctx->dev);
   if (ctx->st) hipStreamSynchronize(ctx->st);
-  for (int k = 0; k < 2; k++)
+  for (int k = 0; k < 2; k++) {
     if (ctx->pc_exec[k]) hipGraphExecDestroy(ctx->pc_exec[k]);
+    if (ctx->pc_exec2[k]) hipGraphExecDestroy(ctx->pc_exec2[k]);
+  }
   for (void* v : ctx->allocs) hipFree(v);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->h_mask) hipHostFree(ctx->h_mask);
+  if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+  if (ctx->st2) hipStreamDestroy(ctx->st2);
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
   return EFFPOT_OK;
@@ -1123,6 +1139,48 @@ static int push_mask(effpot_ctx* ctx, int k, const std::vector<int>& m) {
   return EFFPOT_OK;
 }
 
+static void drop_graphs(effpot_ctx* ctx) {
+  for (int k = 0; k < 2; k++) {
+    if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
+    if (ctx->pc_exec2[k]) { hipGraphExecDestroy(ctx->pc_exec2[k]); ctx->pc_exec2[k] = nullptr; }
+  }
+}
+
+// The PC step as two independent chains, one per slot group (masks dA, dB),
+// on two streams inside one graph: the groups touch disjoint slot data, and
+// each chain's launch tails and barriers overlap the other's work.
+template <typename T>
+static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA, const int* dB) {
+  const int which = f32 ? 1 : 0;
+  if (!ctx->pc_exec2[which]) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = hipEventRecord(ctx->ev_fork, ctx->st);
+    hipError_t e2 = hipStreamWaitEvent(ctx->st2, ctx->ev_fork, 0);
+    int rc = (e1 == hipSuccess && e2 == hipSuccess) ? EFFPOT_OK : EFFPOT_ERR_DEVICE;
+    ctx->corr_den = false;
+    for (int part = 0; part < 2 && rc == EFFPOT_OK; part++) {
+      if (part == 1) std::swap(ctx->st, ctx->st2);  // launches of group B go to the second stream
+      const int* d = part == 0 ? dA : dB;
+      rc = step_pc_t(ctx, s, d, f32);
+      if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, d, f32);
+      if (part == 1) std::swap(ctx->st, ctx->st2);
+    }
+    ctx->corr_den = true;
+    if (rc == EFFPOT_OK && (hipEventRecord(ctx->ev_join, ctx->st2) != hipSuccess ||
+                            hipStreamWaitEvent(ctx->st, ctx->ev_join, 0) != hipSuccess))
+      rc = EFFPOT_ERR_DEVICE;
+    hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    if (rc != EFFPOT_OK) return ctx->fail(rc, "two-group graph capture failed");
+    if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->pc_exec2[which], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  HIPCHK(hipGraphLaunch(ctx->pc_exec2[which], ctx->st));
+  return EFFPOT_OK;
+}
+
 // One predictor-corrector step + renormalisation of the slots in mask dP.  The
 // launch sequence only references device memory (rings, masks, fields), so it
 // is captured once per precision and replayed as a HIP graph (the time step of
@@ -1224,7 +1282,15 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
   }
   if (any_p) {
     RC(push_mask(ctx, 2, mp));
-    RC(pc_step_graph(ctx, s, f32, dP));
+    if (ctx->graphs && ctx->split && B >= 2) {
+      std::vector<int> ma(B, 0), mb(B, 0);
+      for (int b = 0; b < B; b++) (b < B / 2 ? ma : mb)[b] = mp[b];
+      RC(push_mask(ctx, 5, ma));
+      RC(push_mask(ctx, 6, mb));
+      RC(pc_step_graph2(ctx, s, f32, ctx->d_mask + 5 * B, ctx->d_mask + 6 * B));
+    } else {
+      RC(pc_step_graph(ctx, s, f32, dP));
+    }
     for (int b = 0; b < B; b++)
       if (mp[b]) {
         sl[b].steps++; sl[b].since++;
@@ -1397,8 +1463,7 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     }
   }
   ctx->tx = best_tx; ctx->zc = best_zc;
-  for (int k = 0; k < 2; k++)  // captured graphs hold the old launch shapes
-    if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
+  drop_graphs(ctx);  // captured graphs hold the old launch shapes
   if (tx_out) *tx_out = best_tx;
   if (zc_out) *zc_out = best_zc;
   return EFFPOT_OK;
@@ -1412,8 +1477,7 @@ int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc) {
   ctx->tx = tx; ctx->zc = zc;
   if (nblk(ctx) > ctx->nblk_max) { ctx->tx = tx0; ctx->zc = zc0; return ctx->fail(EFFPOT_ERR_ARG, "tile exceeds partial buffer"); }
   HIPCHK(hipStreamSynchronize(ctx->st));  // a replay may still be running
-  for (int k = 0; k < 2; k++)
-    if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
+  drop_graphs(ctx);
   return EFFPOT_OK;
 }
 
