@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .abi import (Params, Impurity, Energies, SolveOpts, Result, Slot, OK, FIELDS,  # noqa: F401
+from .abi import (Params, Impurity, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
                   SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
                   ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, H4OT)
@@ -68,8 +68,36 @@ def lib():
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
         L.effpot_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
+        L.effpot_snapshot_read.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p]
+        L.effpot_snapshot_write.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p, C.c_int]
         _lib = L
     return _lib
+
+
+def read_snapshot(path, header_only=False):
+    """Read a wavefunction snapshot (readenc mode 3, formatted or unformatted).
+    Returns (Snapshot header, psi complex array of shape (nz, ny, nx) = Fortran
+    psi(nx,ny,nz), or None when header_only)."""
+    h = Snapshot()
+    rc = lib().effpot_snapshot_read(os.fsencode(path), C.byref(h), None)
+    if rc != OK:
+        raise EffpotError(rc, lib().effpot_last_error(None).decode())
+    if header_only:
+        return h, None
+    buf = np.zeros((h.nz, h.ny, h.nx, 2))
+    rc = lib().effpot_snapshot_read(os.fsencode(path), C.byref(h), buf.ctypes.data)
+    if rc != OK:
+        raise EffpotError(rc, lib().effpot_last_error(None).decode())
+    return h, buf[..., 0] + 1j * buf[..., 1]
+
+
+def write_snapshot(path, header: Snapshot, psi, binary=False):
+    """Write a snapshot in printoutc's format (formatted, or unformatted when binary)."""
+    psi = np.asarray(psi).reshape(header.nz, header.ny, header.nx)
+    buf = np.ascontiguousarray(np.stack([psi.real, psi.imag], axis=-1), dtype=np.float64)
+    rc = lib().effpot_snapshot_write(os.fsencode(path), C.byref(header), buf.ctypes.data, 1 if binary else 0)
+    if rc != OK:
+        raise EffpotError(rc, lib().effpot_last_error(None).decode())
 
 
 def footprint_bytes(params: Params) -> int:

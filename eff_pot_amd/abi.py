@@ -62,6 +62,18 @@ class Slot(C.Structure):
                 ("rel_de", C.c_double), ("e", Energies)]
 
 
+SNAPSHOT_MAX_INVAR = 16
+
+
+class Snapshot(C.Structure):
+    """Header of a wavefunction snapshot (printoutc.f90:69-114 / readenc.f90:151-168)."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("ninvar", C.c_int32),
+                ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+                ("rimp", C.c_double * 3), ("vimp", C.c_double * 3),
+                ("invar", C.c_double * (2 * SNAPSHOT_MAX_INVAR)), ("binary", C.c_int32), ("reserved_", C.c_int32)]
+
+
 class Result(C.Structure):
     _fields_ = [("e", Energies), ("steps", C.c_int32), ("converged", C.c_int32),
                 ("rel_de", C.c_double), ("err_he", C.c_double)]

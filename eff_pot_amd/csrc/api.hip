@@ -19,6 +19,7 @@
 #include "../../include/effpot.h"
 #include "common.hpp"
 #include "fft.hpp"
+#include "snapshot.hpp"
 #include "misc.hpp"
 #include "stencil.hpp"
 
@@ -830,7 +831,13 @@ int effpot_ctx_destroy(effpot_ctx* ctx) {
   return EFFPOT_OK;
 }
 
-const char* effpot_last_error(const effpot_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* effpot_last_error(const effpot_ctx* ctx) { return ctx ? ctx->err.c_str() : snap::g_err.c_str(); }
+
+int effpot_snapshot_read(const char* path, effpot_snapshot* hdr, double* psi) { return snap::read(path, hdr, psi); }
+
+int effpot_snapshot_write(const char* path, const effpot_snapshot* hdr, const double* psi, int binary) {
+  return snap::write(path, hdr, psi, binary);
+}
 
 int effpot_sync(effpot_ctx* ctx) {
   RC(check_ctx(ctx));

@@ -36,6 +36,7 @@ extern "C" {
 #define EFFPOT_ERR_STATE 7     /* call out of order (e.g. step before impurity)   */
 #define EFFPOT_ERR_NOMEM 8     /* device allocation failed                        */
 #define EFFPOT_ERR_DT 9        /* inconsistent / non-positive time step           */
+#define EFFPOT_ERR_IO 10       /* snapshot file missing, truncated or malformed   */
 
 /* ---- enumerations ------------------------------------------------------ */
 #define EFFPOT_CORE_OT 0  /* core4='OT '  (poten.f90:27-92, no alpha_s)      */
@@ -152,6 +153,19 @@ typedef struct effpot_slot {
   effpot_energies e; /* energies at the last evaluation                       */
 } effpot_slot;
 
+/* Header of a wavefunction snapshot (printoutc.f90:69-114 / readenc.f90:151-168):
+ * the grid record, the impurity position and velocity, the impurity's internal
+ * variables invar(ninvar) (complex) and, after it, psi(nx,ny,nz) complex(8). */
+#define EFFPOT_SNAPSHOT_MAX_INVAR 16
+typedef struct effpot_snapshot {
+  int32_t nx, ny, nz, ninvar;
+  double xmax, ymax, zmax, hx, hy, hz;
+  double rimp[3], vimp[3];
+  double invar[2 * EFFPOT_SNAPSHOT_MAX_INVAR]; /* interleaved complex, ninvar used */
+  int32_t binary;   /* read: 1 if the file was unformatted (printoutc iprint < 0) */
+  int32_t reserved_;
+} effpot_snapshot;
+
 typedef struct effpot_ctx effpot_ctx;
 
 /* ---- context ----------------------------------------------------------- */
@@ -161,7 +175,8 @@ typedef struct effpot_ctx effpot_ctx;
  * device `device`. */
 int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out);
 int effpot_ctx_destroy(effpot_ctx* ctx);
-const char* effpot_last_error(const effpot_ctx* ctx);
+const char* effpot_last_error(const effpot_ctx* ctx); /* NULL: the calling thread's
+                                                        last snapshot error */
 /* Device bytes held by a context with these parameters (admission control). */
 int64_t effpot_footprint_bytes(const effpot_params* p);
 
@@ -254,6 +269,18 @@ int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc);
 int effpot_sync(effpot_ctx* ctx);
 
 #ifdef __cplusplus
+/* ---- wavefunction snapshots (host I/O, no context) ------------------------
+ * effpot_snapshot_read replaces readenc mode 3 (readenc.f90:151-168 with
+ * titols.f90:12-37): it reads the formatted file (leading `#` lines skipped,
+ * list-directed values: "(re,im)" complex, "k*" repeat counts, D exponents) or
+ * the unformatted one (printoutc.f90:69-77, sequential records with 4-byte
+ * markers), detected from the content.  psi (interleaved complex, nx*ny*nz)
+ * may be NULL to read the header only.  effpot_snapshot_write replaces
+ * printoutc's write (binary = 0 formatted, 1 unformatted); formatted values
+ * are written with 17 significant digits, so a round trip is exact. */
+int effpot_snapshot_read(const char* path, effpot_snapshot* hdr, double* psi);
+int effpot_snapshot_write(const char* path, const effpot_snapshot* hdr, const double* psi, int binary);
+
 }
 #endif
 #endif /* EFFPOT_H */

@@ -84,3 +84,41 @@ def test_server_results_match_alone(gpu_available):
                                   polish_steps=30000, tol_fp32=1e-7, check_finite=0))[0]
         assert x.steps == r.steps
         assert abs(x.e.eimpu - r.eimp) < 1e-9 and abs(x.e.etot - r.etot) < 1e-9
+
+
+def test_oneshot_batch_matches_oracle_and_single(gpu_available, tmp_path):
+    """Five snapshots (formatted and unformatted) evaluated two at a time as
+    slots of one context: each E_imp and E_tot equal the one-slot evaluation
+    bit for bit and the FP64 oracle's poten + energy of the same psi."""
+    from eff_pot_amd import oneshot
+    from oracle.oracle import Oracle
+    print("[oneshot] 5 snapshots, batch 2", flush=True)
+    p = ep.make_params(32, xmax=12.0, n4=100.0, core=ep.CORE_OTC, h_core=2.377)
+    paths, psis, poss = [], [], []
+    with ep.Context(p) as c:
+        for k in range(5):
+            c.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 300 + k)
+            psi = c.psi()
+            h = ep.Snapshot()
+            h.nx = h.ny = h.nz = 32
+            h.xmax = h.ymax = h.zmax = 12.0
+            h.hx = h.hy = h.hz = 0.75
+            pos = (0.0, 0.3 * k, 1.0 - 0.5 * k)
+            h.rimp[:] = pos
+            path = str(tmp_path / f"den{k}.dat")
+            ep.write_snapshot(path, h, psi, binary=(k % 2 == 1))
+            paths.append(path); psis.append(psi); poss.append(pos)
+    imp = make_morse(D=30.0, re=4.0, a=1.0)
+    many = oneshot.evaluate(paths, imp, batch=2)
+    single = oneshot.evaluate(paths[:1], imp, batch=1)
+    assert many[0]["eimpu"] == single[0]["eimpu"] and many[0]["etot"] == single[0]["etot"]
+    for k in range(5):
+        o = Oracle(p)
+        o.set_psi(psis[k])
+        o.set_impurity(make_morse(rimp=poss[k], D=30.0, re=4.0, a=1.0))
+        o.poten()
+        e = o.energy()
+        o.close()
+        assert many[k]["rimp"] == poss[k]
+        assert abs(many[k]["eimpu"] - e["eimpu"]) <= 1e-10 * abs(e["eimpu"])
+        assert abs(many[k]["etot"] - e["etot"]) <= 1e-10 * abs(e["etot"])
