@@ -34,6 +34,14 @@ module effpot_c
     integer(c_int32_t) :: check_finite, pad_
   end type
 
+  type, bind(c) :: effpot_snapshot
+    integer(c_int32_t) :: nx, ny, nz, ninvar
+    real(c_double) :: xmax, ymax, zmax, hx, hy, hz
+    real(c_double) :: rimp(3), vimp(3)
+    real(c_double) :: invar(32)
+    integer(c_int32_t) :: binary, reserved_
+  end type
+
   type, bind(c) :: effpot_result
     type(effpot_energies) :: e
     integer(c_int32_t) :: steps, converged
@@ -131,6 +139,13 @@ module effpot_c
       integer(c_int), value :: slot
       real(c_double), intent(out) :: den(*)
     end function
+    ! readenc mode 3 (formatted or unformatted file); psi = c_null_ptr reads the header only
+    integer(c_int) function effpot_snapshot_read(path, hdr, psi) bind(c, name="effpot_snapshot_read")
+      import :: c_int, c_char, c_ptr, effpot_snapshot
+      character(kind=c_char), intent(in) :: path(*)
+      type(effpot_snapshot), intent(out) :: hdr
+      type(c_ptr), value :: psi
+    end function
     ! derivnd(id, nn, h, iv, f, df, icon) on a real (nx,ny,nz) array, grid and icon of the context
     integer(c_int) function effpot_derivnd(ctx, id, iv, f, df) bind(c, name="effpot_derivnd")
       import :: c_int, c_ptr, c_double
@@ -154,13 +169,12 @@ contains
     integer :: i
     if (rc == EFFPOT_OK) return
     text = ''
-    if (c_associated(ctx)) then
-      call c_f_pointer(effpot_last_error(ctx), msg, [512])
-      do i = 1, 512
-        if (msg(i) == c_null_char) exit
-        text(i:i) = msg(i)
-      end do
-    end if
+    ! a null context gives the thread's last snapshot (file) error
+    call c_f_pointer(effpot_last_error(ctx), msg, [512])
+    do i = 1, 512
+      if (msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
     write(0, '(a,": status ",i0,": ",a)') where, rc, trim(text)
     error stop 'effpot: severe error'
   end subroutine check

@@ -24,11 +24,13 @@ program effpot_driver
   real(8) :: morse_d = 30.d0, morse_re = 4.d0, morse_a = 1.d0
   real(8) :: r_cutoff_pi = 0.d0, umax_pi = 0.d0, afermi = 0.6d0, rho0 = 0.0218d0, noise = 1.d-6
   real(8) :: tol = 1.d-9
+  logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
   real(8), parameter :: ktops = 7.63822291d0
   namelist /input/ title, filedenin, core4, selec_pi, n4, mode, nx, ny, nz, npd, icon, niter, pener, &
                    nthread, precision, seed, xmax, ymax, zmax, xc, yc, zc, eps4, sigma4, h_core, l, &
                    cp4, cpp4, den4c, alphas, den0s, h2o2m4, deltat, deltatps, ximp, yimp, zimp, &
-                   morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar
+                   morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar, &
+                   fast_io
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
@@ -36,7 +38,8 @@ program effpot_driver
   type(effpot_solve_opts) :: opt
   type(effpot_result) :: res
   type(c_ptr) :: ctx
-  complex(8), allocatable :: psi(:,:,:)
+  complex(8), allocatable, target :: psi(:,:,:)
+  type(effpot_snapshot) :: snap
   real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, rimp(3), vimp(3), hx, hy, hz
   integer :: nxp, nyp, nzp, nninvar0, isalto
   complex(8), allocatable :: invar0(:)
@@ -82,6 +85,15 @@ program effpot_driver
                                       seed, (/xc, yc, zc/)), 'effpot_init_fermi')
   case (3)   ! wavefunction snapshot (readenc.f90:151-168)
     allocate(psi(nx, ny, nz))
+    if (fast_io) then   ! one pass over the file in C: formatted or unformatted
+      call check(c_null_ptr, effpot_snapshot_read(trim(filedenin) // c_null_char, snap, c_null_ptr), &
+                 'effpot_snapshot_read')
+      if (snap%nx /= nx .or. snap%ny /= ny .or. snap%nz /= nz) stop 'readenc: grid of the file differs from the namelist'
+      call check(c_null_ptr, effpot_snapshot_read(trim(filedenin) // c_null_char, snap, c_loc(psi)), &
+                 'effpot_snapshot_read')
+      rimp = snap%rimp
+      call check(ctx, effpot_upload_psi(ctx, 0_c_int, psi), 'effpot_upload_psi')
+    else
     open(unit=1, file=filedenin, status='old')
     call titols(1, '#', isalto)
     read(1, *) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp
@@ -94,6 +106,7 @@ program effpot_driver
     read(1, *) psi
     close(1)
     call check(ctx, effpot_upload_psi(ctx, 0_c_int, psi), 'effpot_upload_psi')
+    end if
   case default
     stop 'ERROR READEN: This mode is still not programmed'
   end select

@@ -204,6 +204,43 @@ def test_fortran_driver_labels(gpu_available, tmp_path):
     assert float(line.split()[4]) == eint
 
 
+def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
+    """readenc mode 3 in the Fortran driver: the library's reader (fast_io,
+    formatted and unformatted files) and the driver's own list-directed READ
+    give the energies of the same psi that the Python binding computes."""
+    exe = os.path.join(ROOT, "driver", "effpot_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    print("[driver] mode 3", flush=True)
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    rimp = (0.25, -0.5, 1.75)
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 4242)
+        psi = ctx.psi()
+        ctx.set_impurity(make_morse(rimp=rimp))
+        ctx.poten()
+        e = ctx.energy()[0]
+    h = ep.Snapshot()
+    h.nx = h.ny = h.nz = 32
+    h.xmax = h.ymax = h.zmax = 12.0
+    h.hx = h.hy = h.hz = 0.75
+    h.rimp[:] = rimp
+    h.ninvar = 1
+    got = []
+    for binary, fast in ((False, True), (True, True), (False, False)):
+        path = str(tmp_path / f"den_{int(binary)}.dat")
+        ep.write_snapshot(path, h, psi, binary=binary)
+        nml = (f"&input\n nx=32, ny=32, nz=32, xmax=12.d0, ymax=12.d0, zmax=12.d0, n4=100, mode=3,\n"
+               f" filedenin='{path}', fast_io={'.true.' if fast else '.false.'},\n"
+               " core4='OTC', h_core=2.377d0, selec_pi='Morse', niter=0\n/\n")
+        out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+        assert out.returncode == 0, out.stderr
+        eint = float(re.search(r"Interaction energy \(X\*-He\) \.*: +(\S+)", out.stdout).group(1))
+        got.append(eint)
+        assert abs(eint - e["eHeX"]) < 1e-5
+    assert got[0] == got[1] == got[2]
+
+
 @pytest.mark.parametrize("precision,cplx", [(0, 1), (PREC_MIXED, 0)])
 def test_graph_replay_matches_eager(precision, cplx, gpu_available, monkeypatch):
     """Every replay of the captured predictor-corrector step, with eager energy
