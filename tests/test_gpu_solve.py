@@ -228,10 +228,10 @@ def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
     h.ninvar = 1
     got = []
     for binary, fast in ((False, True), (True, True), (False, False)):
-        path = str(tmp_path / f"den_{int(binary)}.dat")
-        ep.write_snapshot(path, h, psi, binary=binary)
+        name = f"den_{int(binary)}.dat"   # filedenin holds 60 characters: a name in the run's cwd
+        ep.write_snapshot(str(tmp_path / name), h, psi, binary=binary)
         nml = (f"&input\n nx=32, ny=32, nz=32, xmax=12.d0, ymax=12.d0, zmax=12.d0, n4=100, mode=3,\n"
-               f" filedenin='{path}', fast_io={'.true.' if fast else '.false.'},\n"
+               f" filedenin='{name}', fast_io={'.true.' if fast else '.false.'},\n"
                " core4='OTC', h_core=2.377d0, selec_pi='Morse', niter=0\n/\n")
         out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
         assert out.returncode == 0, out.stderr
