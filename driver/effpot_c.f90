@@ -42,6 +42,12 @@ module effpot_c
     integer(c_int32_t) :: binary, reserved_
   end type
 
+  type, bind(c) :: effpot_plus
+    integer(c_int32_t) :: kind, nterms, k0, pad_
+    real(c_double) :: r_cutoff, umax, rcutoff, fcutoff, r0, aa, bb, cc
+    real(c_double) :: pg(16), pg_lo(16)
+  end type
+
   type, bind(c) :: effpot_result
     type(effpot_energies) :: e
     integer(c_int32_t) :: steps, converged
@@ -138,6 +144,19 @@ module effpot_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: slot
       real(c_double), intent(out) :: den(*)
+    end function
+    ! Select_Pot(selec_plus, ...) of the uplus term: preset by name, then per slot
+    integer(c_int) function effpot_plus_preset(selec, r_cutoff, umax, plus) bind(c, name="effpot_plus_preset")
+      import :: c_int, c_char, c_double, effpot_plus
+      character(kind=c_char), intent(in) :: selec(*)
+      real(c_double), value :: r_cutoff, umax
+      type(effpot_plus), intent(out) :: plus
+    end function
+    integer(c_int) function effpot_set_plus(ctx, slot, plus) bind(c, name="effpot_set_plus")
+      import :: c_int, c_ptr, effpot_plus
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      type(effpot_plus), intent(in) :: plus
     end function
     ! readenc mode 3 (formatted or unformatted file); psi = c_null_ptr reads the header only
     integer(c_int) function effpot_snapshot_read(path, hdr, psi) bind(c, name="effpot_snapshot_read")

@@ -13,6 +13,7 @@ program effpot_driver
   character(len=60) :: filedenin = 'he4.input.density'
   character(len=3) :: core4 = 'OTC'
   character(len=80) :: selec_pi = 'Morse'
+  character(len=80) :: selec_plus = 'Ba_plus_2D2'   ! X+-He potential of uplus (V_impur.f90:14-16)
   integer :: n4 = 300, mode = 0, nx = 64, ny = 64, nz = 64, npd = 13, icon = 13
   integer :: niter = 0, pener = 50, nthread = 1, precision = 0, ninvar = 10
   integer(c_int64_t) :: seed = 20261203_c_int64_t
@@ -23,6 +24,7 @@ program effpot_driver
   real(8) :: ximp = 0.d0, yimp = 0.d0, zimp = 0.d0
   real(8) :: morse_d = 30.d0, morse_re = 4.d0, morse_a = 1.d0
   real(8) :: r_cutoff_pi = 0.d0, umax_pi = 0.d0, afermi = 0.6d0, rho0 = 0.0218d0, noise = 1.d-6
+  real(8) :: r_cutoff_plus = 1.05d0, umax_plus = 6.260015d4
   real(8) :: tol = 1.d-9
   logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
   real(8), parameter :: ktops = 7.63822291d0
@@ -30,10 +32,11 @@ program effpot_driver
                    nthread, precision, seed, xmax, ymax, zmax, xc, yc, zc, eps4, sigma4, h_core, l, &
                    cp4, cpp4, den4c, alphas, den0s, h2o2m4, deltat, deltatps, ximp, yimp, zimp, &
                    morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar, &
-                   fast_io
+                   fast_io, selec_plus, r_cutoff_plus, umax_plus
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
+  type(effpot_plus) :: plus
   type(effpot_energies) :: e
   type(effpot_solve_opts) :: opt
   type(effpot_result) :: res
@@ -123,6 +126,11 @@ program effpot_driver
   imp%d = morse_d; imp%re = morse_re; imp%a = morse_a
   imp%r_cutoff = r_cutoff_pi; imp%umax = umax_pi; imp%cap = 1000.d0; imp%rimp = rimp
   call check(ctx, effpot_set_impurity(ctx, 0_c_int, imp), 'effpot_set_impurity')
+  if (effpot_plus_preset(trim(selec_plus) // c_null_char, r_cutoff_plus, umax_plus, plus) /= EFFPOT_OK) then
+    write(6, '(A,": aquest potencial no est definit, de part de: Selec_Pot")') trim(selec_plus)
+    stop 'From V_impur: 0001'
+  end if
+  call check(ctx, effpot_set_plus(ctx, 0_c_int, plus), 'effpot_set_plus')
 
   if (niter > 0) then
     opt%dt = deltat; opt%tol = tol; opt%n_rk = 3; opt%pener = pener; opt%max_steps = niter

@@ -12,10 +12,10 @@ import os
 
 import numpy as np
 
-from .abi import (Params, Impurity, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
+from .abi import (Params, Impurity, Plus, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
                   SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
-                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, H4OT)
+                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, H4OT)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeffpot.so")
@@ -70,6 +70,10 @@ def lib():
         L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
         L.effpot_snapshot_read.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p]
         L.effpot_snapshot_write.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p, C.c_int]
+        L.effpot_set_plus.argtypes = [vp, C.c_int, C.POINTER(Plus)]
+        L.effpot_plus_preset.argtypes = [C.c_char_p, C.c_double, C.c_double, C.POINTER(Plus)]
+        L.effpot_plus_eval.argtypes = [C.POINTER(Plus), C.c_double]
+        L.effpot_plus_eval.restype = C.c_double
         _lib = L
     return _lib
 
@@ -98,6 +102,25 @@ def write_snapshot(path, header: Snapshot, psi, binary=False):
     rc = lib().effpot_snapshot_write(os.fsencode(path), C.byref(header), buf.ctypes.data, 1 if binary else 0)
     if rc != OK:
         raise EffpotError(rc, lib().effpot_last_error(None).decode())
+
+
+# the reference's selec_plus / r_cutoff_plus / umax_plus defaults (V_impur.f90:14-16)
+PLUS_DEFAULT = ("Ba_plus_2D2", 1.05, 6.260015e4)
+
+
+def make_plus(selec=PLUS_DEFAULT[0], r_cutoff=PLUS_DEFAULT[1], umax=PLUS_DEFAULT[2]) -> Plus:
+    """X+-He potential by Select_Pot name ('null', 'Ba_plus_gs', 'Ba_plus_2D0',
+    'Ba_plus_2D1', 'Ba_plus_2D2'); an unknown name raises (Select_Pot stops)."""
+    p = Plus()
+    rc = lib().effpot_plus_preset(selec.encode(), r_cutoff, umax, C.byref(p))
+    if rc != OK:
+        raise EffpotError(rc, f"{selec}: this potential is not defined (Select_Pot)")
+    return p
+
+
+def plus_eval(plus: Plus, r) -> float:
+    """The X+ potential at distance r (A), in K, as the device evaluates it."""
+    return lib().effpot_plus_eval(C.byref(plus), float(r))
 
 
 def footprint_bytes(params: Params) -> int:
@@ -177,6 +200,10 @@ class Context:
 
     def set_impurity(self, imp: Impurity, slot=0):
         self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))
+
+    def set_plus(self, plus, slot=0):
+        """X+ potential of the uplus energy term for one slot (None: none)."""
+        self._chk(lib().effpot_set_plus(self.h, slot, C.byref(plus) if plus is not None else None))
 
     def set_active(self, slot, active):
         self._chk(lib().effpot_set_active(self.h, slot, 1 if active else 0))

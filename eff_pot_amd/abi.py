@@ -9,7 +9,8 @@ OK = 0
 ERR_ARG, ERR_GRID, ERR_CORE, ERR_POT, ERR_DEVICE, ERR_NONFINITE, ERR_STATE, ERR_NOMEM, ERR_DT = range(1, 10)
 CORE_OT, CORE_OTC = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-POT_NULL, POT_MORSE = 0, 1
+POT_NULL, POT_MORSE, POT_SERIES = 0, 1, 2
+SERIES_MAX = 16
 ICON_PERIODIC, ICON_ZERO = 8, 13
 
 FIELDS = {"pot4": 0, "dencg": 1, "delj4": 2, "denalf": 3, "ualphas": 4, "uimp": 5,
@@ -36,6 +37,14 @@ class Impurity(C.Structure):
                 ("D", C.c_double), ("re", C.c_double), ("a", C.c_double),
                 ("r_cutoff", C.c_double), ("umax", C.c_double), ("cap", C.c_double),
                 ("rimp", C.c_double * 3)]
+
+
+class Plus(C.Structure):
+    """X+-He potential of the uplus energy term (effpot_plus)."""
+    _fields_ = [("kind", C.c_int32), ("nterms", C.c_int32), ("k0", C.c_int32), ("pad_", C.c_int32),
+                ("r_cutoff", C.c_double), ("umax", C.c_double), ("rcutoff", C.c_double), ("fcutoff", C.c_double),
+                ("r0", C.c_double), ("aa", C.c_double), ("bb", C.c_double), ("cc", C.c_double),
+                ("pg", C.c_double * SERIES_MAX), ("pg_lo", C.c_double * SERIES_MAX)]
 
 
 class Energies(C.Structure):

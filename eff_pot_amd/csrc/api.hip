@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 #include "snapshot.hpp"
+#include "plus.hpp"
 #include "misc.hpp"
 #include "stencil.hpp"
 
@@ -163,6 +164,8 @@ struct effpot_ctx {
   int* d_mask_tmp = nullptr;
   std::vector<int> h_active;
   std::vector<int> have_imp;
+  PlusSlot* d_plus = nullptr;     // [B] X+ potential of the uplus term
+  std::vector<PlusSlot> h_plus;
   double* d_partial = nullptr;
   int nblk_max = 0;
   double* d_red = nullptr;    // [B][16]
@@ -657,7 +660,7 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
   a.den = s.den; a.dencg = s.dencg; a.delj4 = s.delj4; a.denalf = s.denalf;
   a.dx = s.dx; a.dy = s.dy; a.dz = s.dz; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz;
   a.uimp = s.uimp; a.pot4 = s.pot4; a.partial = ctx->d_partial; a.nblk = nblk(ctx);
-  a.psi_all = s.psi; a.ring = s.ring;
+  a.psi_all = s.psi; a.ring = s.ring; a.plus = ctx->d_plus;
   dim3 g = sgrid(ctx);
 #define M_(TXX)                                                                                   \
   if (ctx->cplx) hipLaunchKernelGGL((k_energy<T, cx<T>, TXX>), g, dim3(kThreads), 0, ctx->st, a); \
@@ -684,6 +687,7 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
       e.etot4 += e.ealphas;
     }
     e.eHeX = S[5] * dv;
+    e.uplus = S[11] * dv;
     e.eimpu = e.ekinx + e.eHeX + e.eso;
     e.etot = e.etot4 + e.eimpu;
     e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
@@ -760,6 +764,10 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (ctx->mixed && (rc = alloc_state(ctx, ctx->s32, true)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->imptab, (size_t)ctx->B * (ctx->npot + 2))) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_active, ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->h_plus.assign(ctx->B, PlusSlot{});
+  if (hipMemcpy(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_mask_tmp, ctx->B)) != EFFPOT_OK) return bail(rc);
   {  // partial sums sized for the largest block count any tile choice gives
@@ -1018,10 +1026,43 @@ int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
                        (const double*)(ctx->s64.uimp + (long)slot * ctx->N), ctx->s32.uimp + (long)slot * ctx->N, ctx->N);
     LAUNCHCHK();
   }
+  PlusSlot& ps = ctx->h_plus[slot];
+  for (int i = 0; i < 3; i++) ps.rimp[i] = imp->rimp[i];
+  HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->have_imp[slot] = 1;
   return EFFPOT_OK;
 }
+
+int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  PlusSlot& ps = ctx->h_plus[slot];
+  if (plus) {
+    if (plus->kind != EFFPOT_POT_NULL && plus->kind != EFFPOT_POT_SERIES)
+      return ctx->fail(EFFPOT_ERR_POT, "unknown X+ potential kind");
+    if (plus->kind == EFFPOT_POT_SERIES &&
+        (plus->nterms < 1 || plus->nterms > EFFPOT_SERIES_MAX || plus->k0 < 0 || plus->k0 > 64))
+      return ctx->fail(EFFPOT_ERR_ARG, "X+ series: nterms in 1..16 and k0 in 0..64");
+    ps.p = *plus;
+    ps.on = plus->kind == EFFPOT_POT_SERIES;
+  } else {
+    std::memset(&ps.p, 0, sizeof(ps.p));
+    ps.on = 0;
+  }
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));  // an energy launch may still read the slot's entry
+  HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_plus* out) {
+  if (!selec || !out) return EFFPOT_ERR_ARG;
+  return plus_preset(selec, r_cutoff, umax, out) ? EFFPOT_OK : EFFPOT_ERR_POT;
+}
+
+double effpot_plus_eval(const effpot_plus* plus, double r) { return plus ? plus_pot(*plus, r) : 0.0; }
 
 int effpot_poten(effpot_ctx* ctx) {
   RC(check_ctx(ctx));

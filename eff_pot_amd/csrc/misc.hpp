@@ -114,8 +114,8 @@ struct ImpParams {
   double rx, ry, rz;
 };
 EP_DEV double select_pot_dev(const ImpParams& ip, double r) {
+  if (ip.kind == 0) return 0.0;  // 'null' is 0 on both sides of r_cutoff (V_impur.f90:171-176)
   if (r < ip.r_cutoff) return ip.umax;
-  if (ip.kind == 0) return 0.0;
   double e = exp(-ip.a * (r - ip.re));
   return ip.D * (e * e - 2.0 * e);
 }

@@ -20,6 +20,7 @@
 // LDS plane is double-buffered so one barrier per plane suffices.
 #pragma once
 #include "common.hpp"
+#include "plus.hpp"
 
 namespace ep {
 
@@ -491,9 +492,10 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// energy() densities (energy.f90:47-165), the chemical potential and the
-// helium centre of mass (r_cm.f90:38-54): 11 block partial sums.
-constexpr int kEnergySums = 11;
+// energy() densities (energy.f90:47-165), the chemical potential, the
+// helium centre of mass (r_cm.f90:38-54) and the X+-He term: 12 block
+// partial sums.
+constexpr int kEnergySums = 12;
 template <typename T> struct EnergyArgs {
   Grid g;
   StenCoef<T> sc;
@@ -506,6 +508,7 @@ template <typename T> struct EnergyArgs {
   const cx<T>* psi_all;  // [slot][5][N]; the live buffer is ring[8*slot]
   const int* ring;
   const T *den, *dencg, *delj4, *denalf, *dx, *dy, *dz, *ix, *iy, *iz, *uimp, *pot4;
+  const PlusSlot* plus;  // [slot] X+ potential of uplus
   double* partial;
   int nblk;
 };
@@ -517,6 +520,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
   __shared__ double red[4 * kEnergySums];
   const long so = (long)s * a.slot;
   const V* psi = reinterpret_cast<const V*>(a.psi_all + ((long)s * 5 + a.ring[8 * s]) * a.slot);
+  const PlusSlot& pl = a.plus[s];
+  const bool plus_on = pl.on != 0;
   double v[kEnergySums];
 #pragma unroll
   for (int k = 0; k < kEnergySums; k++) v[k] = 0.0;
@@ -541,6 +546,11 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
       v[9] += (double)d * (a.z0 + a.hz * z);
       V hp = hpsi_of(p, d2, a.h2o2m4, a.pot4[i], a.uimp[i]);
       v[10] -= (double)dotre(p, hp);
+      if (plus_on) {  // uplus: Select_Pot(selec_plus, |rimp - r|) (energy.f90:149-157)
+        const double rx = pl.rimp[0] - (a.x0 + a.hx * x), ry = pl.rimp[1] - (a.y0 + a.hy * y),
+                     rz = pl.rimp[2] - (a.z0 + a.hz * z);
+        v[11] += (double)d * plus_pot(pl.p, sqrt((rx * rx + ry * ry) + rz * rz));
+      }
     });
   block_sum<kEnergySums>(v, red);
   if (threadIdx.x == 0) {

@@ -4,35 +4,45 @@ The reference computes the impurity energy of a stored density one file per
 process: loop.sh:9-13 and get_impurity_energy.sh:7-16 run the program once per
 snapshot, which reads it (readenc mode 3, readenc.f90:151-168), builds the
 impurity field at the snapshot's rimp (potenimp), calls poten and energy
-(DFT4He3d.f90:437-439) and prints the energy balance.  Here the snapshots are
+(DFT4He3d.f90:437-439) and prints the energy balance, U* (eHeX), U+ (uplus
+with Select_Pot(selec_plus)) and E_SO among it.  Here the snapshots are
 read with effpot_snapshot_read and evaluated `batch` at a time as the slots of
 one context: one potenimp per slot, one poten and one energy launch sequence
 for the whole batch.  Snapshots on different grids go to different contexts.
 
     python -m eff_pot_amd.oneshot --morse 30 4 1 den-*.dat
 
-prints, per file, the reference's "FIRST ENERGY BALANCE" and impurity-energy
-labels (formats 6050 / 6060 of the driver), so get_impurity_energy.sh's parsing
-applies unchanged, followed by a one-line summary.
+prints, per file, the reference's "FIRST ENERGY BALANCE", impurity-energy
+labels and X-He CoM distance (formats 6050 / 6060, DFT4He3d.f90:460-463), so
+get_impurity_energy.sh's parsing applies unchanged, followed by a one-line
+summary.  The X+ potential defaults to the reference's selec_plus
+('Ba_plus_2D2', r_cutoff_plus 1.05, umax_plus 6.260015e4).
 """
 import argparse
 import sys
 from collections import OrderedDict
 
-from . import Context, make_params, make_morse, read_snapshot, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO, H4OT
+import math
+
+from . import (Context, make_params, make_morse, make_plus, read_snapshot, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO,
+               H4OT, PLUS_DEFAULT)
 
 
 def _grid_key(h):
     return (h.nx, h.ny, h.nz, round(h.xmax, 12), round(h.ymax, 12), round(h.zmax, 12))
 
 
-def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch=8, rimp=None):
+def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch=8, rimp=None, plus="default"):
     """E_imp and the energy balance of each snapshot in `paths`.
 
     impurity: an effpot Impurity (its rimp is replaced by the snapshot's own
-    rimp unless `rimp` is given).  Returns one dict per path, in order, with
-    the energies (effpot_energies fields) and the grid/impurity position used.
+    rimp unless `rimp` is given).  plus: the X+ potential of uplus (an effpot
+    Plus, None for none, "default" for the reference's selec_plus).  Returns
+    one dict per path, in order, with the energies (effpot_energies fields),
+    the impurity position used and the X-He centre-of-mass distance.
     """
+    if isinstance(plus, str) and plus == "default":
+        plus = make_plus(*PLUS_DEFAULT)
     headers = [read_snapshot(p, header_only=True)[0] for p in paths]
     groups = OrderedDict()
     for i, h in enumerate(headers):
@@ -56,12 +66,18 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
                                      r_cutoff=impurity.r_cutoff, umax=impurity.umax, cap=impurity.cap)
                     imp.kind = impurity.kind
                     ctx.set_impurity(imp, slot=s)
+                    ctx.set_plus(plus, slot=s)
                 ctx.poten()
                 es = ctx.energy()
                 for s, i in enumerate(chunk):
                     d = dict(es[s])
                     d["path"] = paths[i]
                     d["rimp"] = tuple(rimp) if rimp is not None else tuple(headers[i].rimp)
+                    # r_cm divides by the atom number (the context's n4 is 1 here)
+                    for c in ("xcm", "ycm", "zcm"):
+                        d[c] /= d["natoms"]
+                    d["dist"] = math.sqrt((d["rimp"][0] - d["xcm"]) ** 2 + (d["rimp"][1] - d["ycm"]) ** 2 +
+                                          (d["rimp"][2] - d["zcm"]) ** 2)
                     out[i] = d
     return out
 
@@ -84,6 +100,7 @@ def print_balance(d, f=sys.stdout):
     f.write(f"    Interaction energy (X+-He) ...: {d['uplus']:18.6f} K\n")
     f.write(f"    Spin-Orbit energy (X) ........: {d['eso']:18.6f} K\n")
     f.write(f"    TOTAL ENERGY (He+X) ..........: {d['etot']:18.6f} K\n\n")
+    f.write(f" X-HE CoM distance:    {d['dist']:.15g}     \n")
 
 
 def main(argv=None):
@@ -91,19 +108,24 @@ def main(argv=None):
     ap.add_argument("files", nargs="+")
     ap.add_argument("--morse", type=float, nargs=3, metavar=("D", "RE", "A"), default=(30.0, 4.0, 1.0))
     ap.add_argument("--rimp", type=float, nargs=3, default=None, help="impurity position (default: each file's)")
+    ap.add_argument("--selec-plus", default=PLUS_DEFAULT[0], help="X+-He potential of uplus (Select_Pot name)")
+    ap.add_argument("--r-cutoff-plus", type=float, default=PLUS_DEFAULT[1])
+    ap.add_argument("--umax-plus", type=float, default=PLUS_DEFAULT[2])
     ap.add_argument("--core", choices=["OT", "OTC"], default="OTC")
     ap.add_argument("--h-core", type=float, default=2.377)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--quiet", action="store_true", help="only the summary lines")
     a = ap.parse_args(argv)
     imp = make_morse(D=a.morse[0], re=a.morse[1], a=a.morse[2])
+    plus = make_plus(a.selec_plus, a.r_cutoff_plus, a.umax_plus)
     res = evaluate(a.files, imp, core=CORE_OTC if a.core == "OTC" else CORE_OT,
-                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp)
+                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus)
     for d in res:
         if not a.quiet:
             print(f"# {d['path']}")
             print_balance(d)
-        print(f"{d['path']}  E_imp {d['eimpu']:.10f} K  E_tot {d['etot']:.10f} K  N {d['natoms']:.6f}")
+        print(f"{d['path']}  E_imp {d['eimpu']:.10f} K  U+ {d['uplus']:.10f} K  E_tot {d['etot']:.10f} K  "
+              f"N {d['natoms']:.6f}  d_CoM {d['dist']:.6f} A")
     return 0
 
 

@@ -45,8 +45,9 @@ extern "C" {
 #define EFFPOT_PREC_FP64 0  /* every array and operation in FP64                 */
 #define EFFPOT_PREC_MIXED 1 /* FP32 bulk fields/FFTs/stencils, FP64 reductions   */
 
-#define EFFPOT_POT_NULL 0  /* Select_Pot 'null' (V_impur.f90): U = 0            */
+#define EFFPOT_POT_NULL 0  /* Select_Pot 'null' (V_impur.f90:171-176): U = 0, r_cutoff ignored */
 #define EFFPOT_POT_MORSE 1 /* isotropic Morse D[e^{-2a(r-re)} - 2e^{-a(r-re)}] */
+#define EFFPOT_POT_SERIES 2 /* inverse-power fit (X+-He potentials, see effpot_plus) */
 
 #define EFFPOT_ICON_PERIODIC 8 /* derivnd icon=8  (init_deriv_parallel.f90:529-634) */
 #define EFFPOT_ICON_ZERO 13    /* derivnd icon=13 (redef.f:133-170): f=0 outside  */
@@ -90,6 +91,31 @@ typedef struct effpot_impurity {
   double cap;        /* uimp = min(uimp, cap)                             */
   double rimp[3];    /* impurity position (A)                             */
 } effpot_impurity;
+
+/* X+-He potential of the uplus term (energy.f90:146-163): the radial
+ * potential Select_Pot(selec_plus, r, r_cutoff_plus, umax_plus) evaluated at
+ * |rimp - r| (V_impur.f90:87).  kind EFFPOT_POT_NULL is 0 everywhere, as
+ * 'null' is (V_impur.f90:171-176); kind EFFPOT_POT_SERIES is the
+ * form of the reference's inverse-power fits (V_Ba_plus_*, V_impur.f90:1218-1396):
+ *   r <  r_cutoff            umax
+ *   r <= rcutoff             fcutoff
+ *   r <= r0                  |aa r^2 + bb r + cc|
+ *   otherwise                sum_{k=1..nterms} pg[k-1] r^-(k+k0)
+ * The fits' coefficients carry 33 digits; pg + pg_lo holds them as a
+ * double-double pair and the device sums the series in double-double, so the
+ * cancellation between terms of 1e10 K near r0 costs no precision. */
+#define EFFPOT_SERIES_MAX 16
+typedef struct effpot_plus {
+  int32_t kind;       /* EFFPOT_POT_NULL | EFFPOT_POT_SERIES               */
+  int32_t nterms;     /* series terms, 1..EFFPOT_SERIES_MAX                */
+  int32_t k0;         /* power offset, >= 0                                */
+  int32_t pad_;
+  double r_cutoff, umax;   /* Select_Pot's cutoff                          */
+  double rcutoff, fcutoff; /* inner plateau                                */
+  double r0, aa, bb, cc;   /* parabola between rcutoff and r0              */
+  double pg[EFFPOT_SERIES_MAX];    /* coefficients (high parts)            */
+  double pg_lo[EFFPOT_SERIES_MAX]; /* low parts (0 for plain doubles)      */
+} effpot_plus;
 
 /* Energies as printed by DFT4He3d.f90:460-463 (formats 6050/6060) and
  * computed by energy.f90:12-202, plus the chemical potential and the
@@ -207,6 +233,23 @@ int effpot_set_dt(effpot_ctx* ctx, int slot, double dt);
 
 /* ---- impurity field (potenimpini.f90 / updatepoten+update+potenimp) ---- */
 int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp);
+
+/* X+-He potential of one slot for the uplus energy term, evaluated at the
+ * slot's impurity position (energy.f90:146-163).  A context starts with no
+ * X+ potential (uplus = 0); NULL removes it again. */
+int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus);
+
+/* Fills `out` for a Select_Pot name (trailing blanks ignored): 'null',
+ * 'Ba_plus_gs', 'Ba_plus_2D0', 'Ba_plus_2D1', 'Ba_plus_2D2' (the fits of
+ * V_impur.f90:1218-1396; the reference's default selec_plus is 'Ba_plus_2D2'
+ * with r_cutoff_plus 1.05, umax_plus 6.260015e4, V_impur.f90:14-16).  An
+ * unknown name gives EFFPOT_ERR_POT, where Select_Pot stops
+ * (V_impur.f90:927-931).  Host only; needs no context. */
+int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_plus* out);
+
+/* The potential itself at distance r (A), in K, as the device evaluates it
+ * (Select_Pot's role when Lcontrol prints it).  Host only. */
+double effpot_plus_eval(const effpot_plus* plus, double r);
 
 /* ---- operators on all active slots ------------------------------------ */
 int effpot_poten(effpot_ctx* ctx);                         /* poten.f90:5   */

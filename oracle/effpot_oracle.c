@@ -127,6 +127,7 @@ typedef struct oracle {
   /* impurity table (potenimpini.f90:11-50) */
   int npot; double delinter; double* vtab; /* vtab[0..npot+1] */
   effpot_impurity imp; int have_imp;
+  effpot_plus plus; /* X+ potential of uplus (kind NULL: none) */
 } oracle_t;
 
 static int tstgrid(int n) { /* tstgrid.f90:8-12: n = 2^a 3^b 5^c <= 256 */
@@ -486,8 +487,8 @@ void oracle_derivnd_c(oracle_t* o, int id, int iv, const double* f, double* df) 
 /* Impurity: Select_Pot, potenimpini, updatepoten/update, potenimp.          */
 /* ------------------------------------------------------------------------ */
 double oracle_select_pot(const effpot_impurity* imp, double r) {
+  if (imp->kind == EFFPOT_POT_NULL) return 0.0; /* V_null on both sides of r_cutoff, V_impur.f90:171-176 */
   if (r < imp->r_cutoff) return imp->umax;
-  if (imp->kind == EFFPOT_POT_NULL) return 0.0;
   double e = exp(-imp->a * (r - imp->re));
   return imp->D * (e * e - 2.0 * e);
 }
@@ -533,6 +534,30 @@ int oracle_set_impurity(oracle_t* o, const effpot_impurity* imp) {
     }
   }
   return EFFPOT_OK;
+}
+
+/* Select_Pot(selec_plus, r, r_cutoff_plus, umax_plus) for the uplus term
+ * (V_impur.f90:87): 'null' (:171-176) or an inverse-power fit of the
+ * V_Ba_plus_* form (:1238-1261): plateau, parabola, then the series summed
+ * as the reference does, Aux = xq^(k0+1), Sto += pg(k)*Aux, Aux *= xq, in
+ * REAL*16 (__float128 here; the coefficient is pg + pg_lo). */
+double oracle_plus_pot(const effpot_plus* p, double r) {
+  if (p->kind != EFFPOT_POT_SERIES) return 0.0;
+  if (r < p->r_cutoff) return p->umax;
+  if (r <= p->rcutoff) return p->fcutoff;
+  if (r <= p->r0) return fabs(p->aa * (r * r) + p->bb * r + p->cc); /* aa*x**2+bb*x+cc */
+  __float128 xq = (__float128)1.0 / (__float128)r, aux = 1, sto = 0;
+  for (int k = 0; k <= p->k0; k++) aux *= xq;
+  for (int k = 0; k < p->nterms; k++) {
+    sto += ((__float128)p->pg[k] + (__float128)p->pg_lo[k]) * aux;
+    aux *= xq;
+  }
+  return (double)sto;
+}
+
+void oracle_set_plus(oracle_t* o, const effpot_plus* p) {
+  if (p) o->plus = *p;
+  else memset(&o->plus, 0, sizeof(o->plus));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -610,7 +635,8 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
   deriv(o, 1, 2, o->psi, g[1], 2);
   deriv(o, 1, 3, o->psi, g[2], 2);
   const double aux1 = 0.5 * p->cp4, aux2 = (1.0 / 3.0) * p->cpp4;
-  double ekin = 0.0, elj = 0.0, ecor = 0.0, eal = 0.0, esol = 0.0, ehex = 0.0, nat = 0.0;
+  double ekin = 0.0, elj = 0.0, ecor = 0.0, eal = 0.0, esol = 0.0, ehex = 0.0, nat = 0.0, upl = 0.0;
+  const double* ri = o->imp.rimp;
   double xc = 0.0, yc = 0.0, zc = 0.0;
   for (int iz = 0; iz < o->nz; iz++)
     for (int iy = 0; iy < o->ny; iy++)
@@ -627,6 +653,10 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
           eal += o->falfs[i] * (o->dxden[i] * o->intx[i] + o->dyden[i] * o->inty[i] + o->dzden[i] * o->intz[i]);
         esol += a3 * (1.0 + tanh(p->beta * (a3 - p->den_m)));
         ehex += a3 * o->uimp[i];
+        if (o->plus.kind == EFFPOT_POT_SERIES) { /* energy.f90:149-157, rXHe = rimp - r */
+          double rx = ri[0] - o->x[ix], ry = ri[1] - o->y[iy], rz = ri[2] - o->z[iz];
+          upl += a3 * oracle_plus_pot(&o->plus, sqrt(rx * rx + ry * ry + rz * rz));
+        }
         nat += a3;
         xc += a3 * o->x[ix]; yc += a3 * o->y[iy]; zc += a3 * o->z[iz];
       }
@@ -643,7 +673,8 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
     e->etot4 += e->ealphas;
   }
   e->eHeX = ehex * o->dxyz;
-  e->ekinx = 0.0; e->eso = 0.0; e->uplus = 0.0;
+  e->ekinx = 0.0; e->eso = 0.0;
+  e->uplus = upl * o->dxyz;
   e->eimpu = e->ekinx + e->eHeX + e->eso;
   e->etot = e->etot4 + e->eimpu;
   e->xcm = xc * o->dxyz / p->n4; e->ycm = yc * o->dxyz / p->n4; e->zcm = zc * o->dxyz / p->n4;

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from eff_pot_amd.abi import Params, Impurity, Energies, SolveOpts, Result, OK
+from eff_pot_amd.abi import Params, Impurity, Plus, Energies, SolveOpts, Result, OK
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -49,6 +49,9 @@ def lib():
         L.oracle_set_impurity.argtypes = [C.c_void_p, C.POINTER(Impurity)]
         L.oracle_select_pot.argtypes = [C.POINTER(Impurity), C.c_double]
         L.oracle_select_pot.restype = C.c_double
+        L.oracle_plus_pot.argtypes = [C.POINTER(Plus), C.c_double]
+        L.oracle_plus_pot.restype = C.c_double
+        L.oracle_set_plus.argtypes = [C.c_void_p, C.POINTER(Plus)]
         L.oracle_update.argtypes = [C.c_double, C.c_int, C.c_double, _dp]
         L.oracle_update.restype = C.c_double
         L.oracle_poten.argtypes = [C.c_void_p]
@@ -70,6 +73,11 @@ def lib():
         L.oracle_solve.argtypes = [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(Result)]
         _lib = L
     return _lib
+
+
+def plus_pot(plus, r):
+    """Select_Pot of an X+ potential at r, summed in quad precision."""
+    return lib().oracle_plus_pot(C.byref(plus), r)
 
 
 def fd_coeffs(id_):
@@ -178,6 +186,9 @@ class Oracle:
         rc = lib().oracle_set_impurity(self.h, C.byref(imp))
         if rc != OK:
             raise ValueError(f"oracle_set_impurity failed: {rc}")
+
+    def set_plus(self, plus):
+        lib().oracle_set_plus(self.h, C.byref(plus) if plus is not None else None)
 
     # --- operators ---
     def poten(self):

@@ -89,7 +89,8 @@ def test_server_results_match_alone(gpu_available):
 def test_oneshot_batch_matches_oracle_and_single(gpu_available, tmp_path):
     """Five snapshots (formatted and unformatted) evaluated two at a time as
     slots of one context: each E_imp and E_tot equal the one-slot evaluation
-    bit for bit and the FP64 oracle's poten + energy of the same psi."""
+    bit for bit and the FP64 oracle's poten + energy of the same psi; uplus
+    (default Ba_plus_2D2) and the X-He centre-of-mass distance as well."""
     from eff_pot_amd import oneshot
     from oracle.oracle import Oracle
     print("[oneshot] 5 snapshots, batch 2", flush=True)
@@ -116,9 +117,14 @@ def test_oneshot_batch_matches_oracle_and_single(gpu_available, tmp_path):
         o = Oracle(p)
         o.set_psi(psis[k])
         o.set_impurity(make_morse(rimp=poss[k], D=30.0, re=4.0, a=1.0))
+        o.set_plus(ep.make_plus())  # the reference's default selec_plus
         o.poten()
         e = o.energy()
         o.close()
         assert many[k]["rimp"] == poss[k]
         assert abs(many[k]["eimpu"] - e["eimpu"]) <= 1e-10 * abs(e["eimpu"])
         assert abs(many[k]["etot"] - e["etot"]) <= 1e-10 * abs(e["etot"])
+        assert abs(many[k]["uplus"] - e["uplus"]) <= 1e-10 * abs(e["uplus"])
+        # r_cm over the snapshot's own atom number (the oracle divides by its n4 = 100)
+        cm = np.array([e["xcm"], e["ycm"], e["zcm"]]) * 100.0 / e["natoms"]
+        assert abs(many[k]["dist"] - np.linalg.norm(np.array(poss[k]) - cm)) <= 1e-9

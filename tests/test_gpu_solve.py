@@ -207,7 +207,8 @@ def test_fortran_driver_labels(gpu_available, tmp_path):
 def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
     """readenc mode 3 in the Fortran driver: the library's reader (fast_io,
     formatted and unformatted files) and the driver's own list-directed READ
-    give the energies of the same psi that the Python binding computes."""
+    give the energies of the same psi that the Python binding computes,
+    including uplus with the default X+ potential (Ba_plus_2D2)."""
     exe = os.path.join(ROOT, "driver", "effpot_driver")
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built")
@@ -218,6 +219,7 @@ def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
         ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 4242)
         psi = ctx.psi()
         ctx.set_impurity(make_morse(rimp=rimp))
+        ctx.set_plus(ep.make_plus())  # the namelist default selec_plus = 'Ba_plus_2D2'
         ctx.poten()
         e = ctx.energy()[0]
     h = ep.Snapshot()
@@ -236,8 +238,9 @@ def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
         out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
         assert out.returncode == 0, out.stderr
         eint = float(re.search(r"Interaction energy \(X\*-He\) \.*: +(\S+)", out.stdout).group(1))
-        got.append(eint)
-        assert abs(eint - e["eHeX"]) < 1e-5
+        eplus = float(re.search(r"Interaction energy \(X\+-He\) \.*: +(\S+)", out.stdout).group(1))
+        got.append((eint, eplus))
+        assert abs(eint - e["eHeX"]) < 1e-5 and abs(eplus - e["uplus"]) < 1e-5 and e["uplus"] != 0.0
     assert got[0] == got[1] == got[2]
 
 

@@ -1,0 +1,94 @@
+"""X+-He potential of the uplus energy term (energy.f90:146-163), host only.
+
+Pins: the reference's Ba+ - He fits evaluated exactly from their Block Data
+(tests/golden/plus_potentials.json, written by make_plus_vectors.py) against
+both the library's double-double evaluation (effpot_plus_eval, the function
+the energy kernel runs) and the oracle's REAL*16-style sum; Select_Pot's
+r < r_cutoff -> umax rule and 'null' (0 on both sides); and the oracle's
+uplus on a small grid against a brute-force sum in exact rationals."""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import eff_pot_amd as ep
+from eff_pot_amd.abi import make_params, make_morse, CORE_OT
+from oracle import oracle as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "plus_potentials.json")))
+FITS = ["Ba_plus_gs", "Ba_plus_2D0", "Ba_plus_2D1", "Ba_plus_2D2"]
+
+
+@pytest.mark.parametrize("name", FITS)
+def test_fits_match_exact_values(name):
+    p = ep.make_plus(name, 0.0, 0.0)
+    for r, v in zip(GOLD["radii"], GOLD[name]):
+        for got in (ep.plus_eval(p, r), O.plus_pot(p, r)):
+            # the series branch to the last ulp; the parabola is the
+            # reference's own FP64 expression
+            assert abs(got - v) <= 4e-16 * abs(v), (name, r, got, v)
+
+
+def test_select_pot_cutoff_and_null():
+    p = ep.make_plus("Ba_plus_2D2", 1.05, 6.260015e4)
+    for r in (0.0, 0.5, 1.0499999999999998):
+        assert ep.plus_eval(p, r) == 6.260015e4 and O.plus_pot(p, r) == 6.260015e4
+    # r == r_cutoff is outside (Select_Pot tests r < r_cutoff)
+    assert ep.plus_eval(p, 1.05) == ep.plus_eval(ep.make_plus("Ba_plus_2D2", 0.0, 0.0), 1.05)
+    n = ep.make_plus("null", 1.05, 6.260015e4)
+    assert n.kind == ep.POT_NULL
+    for r in (0.0, 1.0, 2.0, 10.0):
+        assert ep.plus_eval(n, r) == 0.0 and O.plus_pot(n, r) == 0.0
+    assert ep.make_plus("Ba_plus_2D2   ").kind == ep.POT_SERIES  # Fortran Trim
+    with pytest.raises(ep.EffpotError) as e:
+        ep.make_plus("Ba_plus_2D3")
+    assert e.value.code == 4  # EFFPOT_ERR_POT
+
+
+def _exact_v(p, r):
+    x = Fraction(r)
+    if r < p.r_cutoff:
+        return Fraction(p.umax)
+    if x <= Fraction(p.rcutoff):
+        return Fraction(p.fcutoff)
+    if x <= Fraction(p.r0):
+        return abs(Fraction(p.aa) * x * x + Fraction(p.bb) * x + Fraction(p.cc))
+    return sum((Fraction(p.pg[k]) + Fraction(p.pg_lo[k])) / x ** (k + 1 + p.k0) for k in range(p.nterms))
+
+
+def test_oracle_uplus_brute_force():
+    """16^3 grid, h = 0.5, impurity on a node off the centre so that points
+    sit exactly at r = 2 (the plateau/series boundary) and a sign slip in
+    rimp - r would move the sum."""
+    n = 16
+    prm = make_params(n, xmax=4.0, n4=20.0, core=CORE_OT, h_core=2.377)
+    o = O.Oracle(prm)
+    rng = np.random.default_rng(5)
+    psi = rng.uniform(0.05, 0.2, (n, n, n)) + 0j
+    o.set_psi(psi)
+    rimp = (0.5, -1.0, 1.5)
+    o.set_impurity(make_morse(rimp=rimp))
+    plus = ep.make_plus("Ba_plus_2D2", 1.05, 6.260015e4)
+    o.set_plus(plus)
+    o.poten()
+    e = o.energy()
+    x = -4.0 + 0.5 * np.arange(n)
+    den = np.abs(psi) ** 2
+    tot = Fraction(0)
+    hits = 0
+    for iz in range(n):
+        for iy in range(n):
+            for ix in range(n):
+                r = float(np.sqrt((rimp[0] - x[ix]) ** 2 + (rimp[1] - x[iy]) ** 2 + (rimp[2] - x[iz]) ** 2))
+                hits += r == 2.0
+                tot += Fraction(float(den[iz, iy, ix])) * _exact_v(plus, r)
+    assert hits > 0
+    want = float(tot * Fraction(0.125))
+    assert abs(e["uplus"] - want) <= 1e-13 * abs(want), (e["uplus"], want)
+    # no X+ potential: uplus = 0 and nothing else moves
+    o.set_plus(None)
+    e0 = o.energy()
+    assert e0["uplus"] == 0.0 and e0["etot"] == e["etot"] and e0["eHeX"] == e["eHeX"]
+    o.close()
