@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .abi import (Params, Impurity, Plus, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
+from .abi import (Params, Impurity, Plus, StateImp, STATE_P, STATE_D, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
                   SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
                   ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, H4OT)
@@ -71,6 +71,8 @@ def lib():
         L.effpot_snapshot_read.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p]
         L.effpot_snapshot_write.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p, C.c_int]
         L.effpot_set_plus.argtypes = [vp, C.c_int, C.POINTER(Plus)]
+        L.effpot_set_impurity_state.argtypes = [vp, C.c_int, C.POINTER(StateImp)]
+        L.effpot_instates.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.effpot_plus_preset.argtypes = [C.c_char_p, C.c_double, C.c_double, C.POINTER(Plus)]
         L.effpot_plus_eval.argtypes = [C.POINTER(Plus), C.c_double]
         L.effpot_plus_eval.restype = C.c_double
@@ -121,6 +123,35 @@ def make_plus(selec=PLUS_DEFAULT[0], r_cutoff=PLUS_DEFAULT[1], umax=PLUS_DEFAULT
 def plus_eval(plus: Plus, r) -> float:
     """The X+ potential at distance r (A), in K, as the device evaluates it."""
     return lib().effpot_plus_eval(C.byref(plus), float(r))
+
+
+def make_state_imp(state, rimp=(0.0, 0.0, 0.0), instate=0, als=0.0, v_pi=None, v_sigma=None, v_delta=None,
+                   invar=None) -> StateImp:
+    """A P or D internal-state impurity; the potentials default to the
+    reference's selec_pi / selec_sigma / selec_delta (V_impur.f90:5-13)."""
+    imp = StateImp(state=state, instate=instate, als=als)
+    for i in range(3):
+        imp.rimp[i] = rimp[i]
+    imp.v_pi = v_pi if v_pi is not None else make_plus("Ba_plus_2D1", 1.05, 5.216077e4)
+    imp.v_sigma = v_sigma if v_sigma is not None else make_plus("Ba_plus_2D0", 1.05, 9.674836e4)
+    imp.v_delta = v_delta if v_delta is not None else make_plus("Ba_plus_2D2", 1.05, 6.260015e4)
+    if invar is not None:
+        for i, c in enumerate(invar):
+            imp.invar[2 * i], imp.invar[2 * i + 1] = c.real, c.imag
+    return imp
+
+
+def instates(state, moments, als, instate=0):
+    """Host half of instates: (eigenvalues, invar, eso) of V + (Als/2) SO
+    built from the density moments (effpot_instates)."""
+    n = 6 if state == STATE_P else 10
+    mom = np.zeros(31)
+    mom[:len(moments)] = moments
+    inv, eig, eso = np.zeros(20), np.zeros(10), C.c_double()
+    rc = lib().effpot_instates(state, mom, als, instate, inv, eig, C.byref(eso))
+    if rc != OK:
+        raise EffpotError(rc, "effpot_instates: bad arguments")
+    return eig[:n], inv[:2 * n:2] + 1j * inv[1:2 * n:2], eso.value
 
 
 def footprint_bytes(params: Params) -> int:
@@ -200,6 +231,12 @@ class Context:
 
     def set_impurity(self, imp: Impurity, slot=0):
         self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))
+
+    def set_impurity_state(self, imp: StateImp, slot=0) -> StateImp:
+        """P/D-state impurity (instates on the slot's current psi when
+        imp.instate >= 0); fills imp.invar, eigen, moments and eso."""
+        self._chk(lib().effpot_set_impurity_state(self.h, slot, C.byref(imp)))
+        return imp
 
     def set_plus(self, plus, slot=0):
         """X+ potential of the uplus energy term for one slot (None: none)."""

@@ -47,6 +47,20 @@ class Plus(C.Structure):
                 ("pg", C.c_double * SERIES_MAX), ("pg_lo", C.c_double * SERIES_MAX)]
 
 
+STATE_P, STATE_D = 1, 2
+
+
+class StateImp(C.Structure):
+    """Impurity in a P or D internal state (effpot_state_imp)."""
+    _fields_ = [("state", C.c_int32), ("instate", C.c_int32), ("als", C.c_double), ("rimp", C.c_double * 3),
+                ("v_pi", Plus), ("v_sigma", Plus), ("v_delta", Plus), ("invar", C.c_double * 20),
+                ("eigen", C.c_double * 10), ("moments", C.c_double * 31), ("eso", C.c_double)]
+
+    def invar_c(self):
+        n = 6 if self.state == STATE_P else 10
+        return [complex(self.invar[2 * i], self.invar[2 * i + 1]) for i in range(n)]
+
+
 class Energies(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("natoms", "etot4", "ekin4", "elj4", "ealphas", "esolid", "ecor4", "eimpu",

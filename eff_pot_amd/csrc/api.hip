@@ -23,6 +23,7 @@
 #include "plus.hpp"
 #include "misc.hpp"
 #include "stencil.hpp"
+#include "state.hpp"
 
 using namespace ep;
 
@@ -166,6 +167,9 @@ struct effpot_ctx {
   std::vector<int> have_imp;
   PlusSlot* d_plus = nullptr;     // [B] X+ potential of the uplus term
   std::vector<PlusSlot> h_plus;
+  std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
+  double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
+  double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
   double* d_partial = nullptr;
   int nblk_max = 0;
   double* d_red = nullptr;    // [B][16]
@@ -688,6 +692,7 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
     }
     e.eHeX = S[5] * dv;
     e.uplus = S[11] * dv;
+    e.eso = ctx->h_eso[b];
     e.eimpu = e.ekinx + e.eHeX + e.eso;
     e.etot = e.etot4 + e.eimpu;
     e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
@@ -766,6 +771,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_active, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_plus.assign(ctx->B, PlusSlot{});
+  ctx->h_eso.assign(ctx->B, 0.0);
   if (hipMemcpy(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -1030,6 +1036,108 @@ int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
   for (int i = 0; i < 3; i++) ps.rimp[i] = imp->rimp[i];
   HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->have_imp[slot] = 1;
+  ctx->h_eso[slot] = 0.0;
+  return EFFPOT_OK;
+}
+
+int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
+                    double* eso) {
+  if (state != EFFPOT_STATE_P && state != EFFPOT_STATE_D) return EFFPOT_ERR_ARG;
+  const int n = state == EFFPOT_STATE_P ? 6 : 10;
+  if (!moments || instate >= n / 2 || instate < -1) return EFFPOT_ERR_ARG;
+  zd H[100], V[100];
+  double w[10];
+  state_matrix(state, moments, als, H);
+  herm_eigen(n, H, w, V);
+  if (eigen) std::memcpy(eigen, w, sizeof(double) * n);
+  if (instate >= 0 && invar) state_select(n, V, instate, invar);
+  if (eso && invar) *eso = state_eso(state, invar, als);
+  return EFFPOT_OK;
+}
+
+int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) {
+  RC(check_ctx(ctx));
+  if (!imp || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  const int st = imp->state;
+  if (st != EFFPOT_STATE_P && st != EFFPOT_STATE_D) return ctx->fail(EFFPOT_ERR_ARG, "state must be P or D");
+  const int n = st == EFFPOT_STATE_P ? 6 : 10;
+  if (imp->instate < -1 || imp->instate >= n / 2)
+    return ctx->fail(EFFPOT_ERR_ARG, "instate out of range (P: 0..2, D: 0..4, or -1 for a given invar)");
+  for (const effpot_plus* v : {&imp->v_pi, &imp->v_sigma, &imp->v_delta})
+    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES) ||
+        (v->kind == EFFPOT_POT_SERIES && (v->nterms < 1 || v->nterms > EFFPOT_SERIES_MAX || v->k0 < 0)))
+      return ctx->fail(EFFPOT_ERR_POT, "bad state potential");
+  hipSetDevice(ctx->dev);
+  const long L = ctx->npot + 2;
+  if (!ctx->d_stab) {
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_stab, (size_t)ctx->B * 3 * L)) != EFFPOT_OK) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_smom, (size_t)kStateBlocks * kMomD)) != EFFPOT_OK) return rc;
+  }
+  double* tab = ctx->d_stab + (long)slot * 3 * L;
+  const dim3 tg((unsigned)((L + 255) / 256)), tb(256);
+  const double dl = ctx->delinter;
+  if (st == EFFPOT_STATE_P) {  // potpi, potdel = V_sigma - V_pi
+    hipLaunchKernelGGL(k_state_table, tg, tb, 0, ctx->st, imp->v_pi, imp->v_pi, 0, ctx->npot, dl, tab);
+    hipLaunchKernelGGL(k_state_table, tg, tb, 0, ctx->st, imp->v_pi, imp->v_sigma, 1, ctx->npot, dl, tab + L);
+  } else {  // potdel = V_delta, potpidel = V_pi - V_delta, potsigdel = V_sigma - V_delta
+    hipLaunchKernelGGL(k_state_table, tg, tb, 0, ctx->st, imp->v_delta, imp->v_delta, 0, ctx->npot, dl, tab);
+    hipLaunchKernelGGL(k_state_table, tg, tb, 0, ctx->st, imp->v_delta, imp->v_pi, 1, ctx->npot, dl, tab + L);
+    hipLaunchKernelGGL(k_state_table, tg, tb, 0, ctx->st, imp->v_delta, imp->v_sigma, 1, ctx->npot, dl, tab + 2 * L);
+  }
+  LAUNCHCHK();
+  StateArgs a{};
+  a.state = st; a.nx = ctx->nx; a.ny = ctx->ny; a.nz = ctx->nz;
+  a.x0 = -ctx->p.xmax; a.y0 = -ctx->p.ymax; a.z0 = -ctx->p.zmax; a.hx = ctx->hx; a.hy = ctx->hy; a.hz = ctx->hz;
+  a.npot = ctx->npot;
+  a.h = ((double)(ctx->npot - 1) * dl) / (double)(ctx->npot - 1);
+  a.c1oh = 1.0 / a.h; a.c1o2h2 = a.c1oh * a.c1oh * 0.5;
+  a.tab = tab; a.umax_pi = imp->v_pi.umax; a.umax_sigma = imp->v_sigma.umax;
+  a.rx = imp->rimp[0]; a.ry = imp->rimp[1]; a.rz = imp->rimp[2];
+  a.cap = 1000.0;  // potenimp.f90:184-186
+  std::memset(imp->moments, 0, sizeof(imp->moments));
+  if (imp->instate >= 0) {  // instates on the slot's current density
+    const int K = st == EFFPOT_STATE_P ? kMomP : kMomD;
+    // the slot's current psi buffer (ring roles live on the device)
+    std::vector<int> ring(8);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipMemcpy(ring.data(), ctx->s64.ring + 8 * slot, sizeof(int) * 8, hipMemcpyDeviceToHost));
+    const long off = ((long)slot * 5 + ring[R_CUR]) * ctx->N;  // in complex elements, as Ring::p
+#define M_(KK)                                                                                                 \
+  if (ctx->cplx)                                                                                             \
+    hipLaunchKernelGGL((k_state_moments<cx<double>, KK>), dim3(kStateBlocks), dim3(256), 0, ctx->st,           \
+                       (const cx<double>*)(ctx->s64.psi + off), a, ctx->d_smom);                              \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_state_moments<double, KK>), dim3(kStateBlocks), dim3(256), 0, ctx->st,              \
+                       (const double*)(ctx->s64.psi + off), a, ctx->d_smom)
+    if (K == kMomP) { M_(kMomP); } else { M_(kMomD); }
+#undef M_
+    LAUNCHCHK();
+    std::vector<double> part((size_t)kStateBlocks * K);
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->d_smom, sizeof(double) * part.size(), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int k = 0; k < K; k++) {
+      double acc = 0.0;
+      for (int b = 0; b < kStateBlocks; b++) acc += part[(size_t)b * K + k];
+      imp->moments[k] = acc * ctx->dxyz;
+    }
+  }
+  RC(effpot_instates(st, imp->moments, imp->als, imp->instate, imp->invar, imp->eigen, &imp->eso));
+  if (imp->instate < 0) imp->eso = state_eso(st, imp->invar, imp->als);
+  std::memcpy(a.inv, imp->invar, sizeof(a.inv));
+  hipLaunchKernelGGL((k_state_uimp<double>), dim3(1024), dim3(256), 0, ctx->st, a, ctx->s64.uimp + (long)slot * ctx->N);
+  LAUNCHCHK();
+  if (ctx->mixed) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, ctx->st,
+                       (const double*)(ctx->s64.uimp + (long)slot * ctx->N), ctx->s32.uimp + (long)slot * ctx->N, ctx->N);
+    LAUNCHCHK();
+  }
+  PlusSlot& ps = ctx->h_plus[slot];
+  for (int i = 0; i < 3; i++) ps.rimp[i] = imp->rimp[i];
+  HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->h_eso[slot] = imp->eso;
   ctx->have_imp[slot] = 1;
   return EFFPOT_OK;
 }

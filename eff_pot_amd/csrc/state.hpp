@@ -1,0 +1,336 @@
+// Anisotropic impurity in a P or D internal state (Lstate 'P' / 'D').
+//
+// potenimpini.f90:11-72 tabulates the radial potentials (P: V_pi and
+// V_sigma - V_pi; D: V_delta, V_pi - V_delta, V_sigma - V_delta) on the
+// npot-point grid; updatepoten (potenimp.f90:515-575) interpolates them at
+// |r - rimp| with the parabola of update() and divides the anisotropic parts
+// by r^2 (P) or r^4 (D); instates (instates.f90:1-800) builds the
+// density-weighted potential matrix plus the spin-orbit matrix, diagonalises
+// it and picks invar; potenimp (potenimp.f90:34-180) evaluates the He field of
+// that internal state, capped at 1000 K.
+//
+// Here one kernel reduces the density-weighted moments of every slot (the
+// grid half of instates), the host builds and diagonalises the 6x6 / 10x10
+// Hermitian matrix (Jacobi rotations), and one kernel writes uimp.  The
+// fields of updatepoten are recomputed from the tables where they are needed
+// instead of being stored (three more N-sized arrays per slot otherwise).
+#pragma once
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "common.hpp"
+#include "plus.hpp"
+#include "stencil.hpp"
+
+namespace ep {
+
+// update() with Lparab (potenimp.f90:588-632) on a table f[0..npot+1]; the
+// index is clamped at npot (the reference reads past its table beyond rmax).
+EP_DEV double update_dev(double r, int npot, double h, double c1oh, double c1o2h2, const double* f) {
+  int i0 = (int)(1.5 + r / h);
+  if (i0 > npot) i0 = npot;
+  const double r0 = (double)(i0 - 1) * h;
+  const double f0 = f[i0], fm = f[i0 - 1], fp = f[i0 + 1];
+  const double aa = r < r0 ? (fp - 2.0 * f0 + fm) * c1o2h2 : (fm - 2.0 * f0 + fp) * c1o2h2;
+  const double bb = (fp - f0) * c1oh - aa * (2.0 * r0 + h);
+  const double cc = f0 - r0 * (bb + r0 * aa);
+  return r * (aa * r + bb) + cc;
+}
+
+// Moments of instates: P (instates.f90:136-157) A, Vx, Vy, Vz, Vxy, Vxz, Vyz;
+// D (:455-519) MP(i<=j), MS(i<=j) row-major over the 5x5 upper triangles, then
+// the diagonal sum of Delta.
+constexpr int kMomP = 7, kMomD = 31, kStateBlocks = 256;
+constexpr double kZeroR = 1e-3;  // updatepoten's zeror
+
+struct StateArgs {
+  int state;  // EFFPOT_STATE_P | EFFPOT_STATE_D
+  int nx, ny, nz;
+  double x0, y0, z0, hx, hy, hz;
+  int npot;
+  double h, c1oh, c1o2h2;
+  const double* tab;  // [3][npot+2] of this slot
+  double umax_pi, umax_sigma;
+  double rx, ry, rz;
+  double inv[20];  // invar (complex, interleaved)
+  double cap;
+};
+
+// updatepoten fields at one point: P (t0, t1) = (VPi, Delta/r^2);
+// D (t0, t1, t2) = (Delta, Pi_Del/r^4, Sig_Del/r^4) (potenimp.f90:531-565).
+EP_DEV void state_fields(const StateArgs& a, double r2, double& t0, double& t1, double& t2) {
+  const double r = sqrt(r2);
+  const int L = a.npot + 2;
+  if (a.state == EFFPOT_STATE_P) {
+    t0 = update_dev(r, a.npot, a.h, a.c1oh, a.c1o2h2, a.tab);
+    t1 = r > kZeroR ? update_dev(r, a.npot, a.h, a.c1oh, a.c1o2h2, a.tab + L) / r2 : a.umax_sigma;
+    t2 = 0.0;
+  } else {
+    const double r4 = r2 * r2;
+    t0 = update_dev(r, a.npot, a.h, a.c1oh, a.c1o2h2, a.tab);
+    t1 = r > kZeroR ? update_dev(r, a.npot, a.h, a.c1oh, a.c1o2h2, a.tab + L) / r4 : a.umax_pi;
+    t2 = r > kZeroR ? update_dev(r, a.npot, a.h, a.c1oh, a.c1o2h2, a.tab + 2 * L) / r4 : a.umax_sigma;
+  }
+}
+
+// The five d-orbital quadratic forms of instates / potenimp (:466-499):
+// armonicd d1..d5 and the upper triangle of MP (the Sig_Del-weighted matrix).
+struct DForms {
+  double d[6];   // d1..d5, d0
+  double mp[15]; // MP(i<=j) row-major, without the radial factor
+};
+EP_DEV void d_forms(double xx, double yy, double zz, DForms& f) {
+  const double sqr3 = 1.7320508075688772, sqr3o2 = 0.8660254037844386;
+  const double insqr3 = 0.5773502691896258, in3 = 1.0 / 3.0;
+  const double d0 = xx * xx + yy * yy + zz * zz, d1 = sqr3 * xx * yy, d2 = sqr3 * yy * zz;
+  const double d3 = (-xx * xx - yy * yy + 2.0 * zz * zz) / 2.0, d4 = sqr3 * xx * zz, d5 = sqr3o2 * (xx * xx - yy * yy);
+  const double sd1 = d1 * d1, sd2 = d2 * d2, sd4 = d4 * d4, sd5 = d5 * d5;
+  const double s35 = d3 + insqr3 * d5, f35 = d3 - insqr3 * d5;
+  f.d[0] = d1; f.d[1] = d2; f.d[2] = d3; f.d[3] = d4; f.d[4] = d5; f.d[5] = d0;
+  f.mp[0] = in3 * (sd2 + sd4 + 4.0 * sd5);               // (1,1)
+  f.mp[1] = -4.0 * in3 * d1 * d2 + insqr3 * d4 * d0;     // (1,2)
+  f.mp[2] = -2.0 * insqr3 * d2 * d4;                     // (1,3)
+  f.mp[3] = -4.0 * in3 * d1 * d4 + insqr3 * d2 * d0;     // (1,4)
+  f.mp[4] = -4.0 * in3 * d1 * d5;                        // (1,5)
+  f.mp[5] = in3 * (sd1 + sd4) + s35 * s35;               // (2,2)
+  f.mp[6] = insqr3 * d1 * d4 - d2 * s35;                 // (2,3)
+  f.mp[7] = -4.0 * in3 * d2 * d4 + insqr3 * d1 * d0;     // (2,4)
+  f.mp[8] = -d1 * d4 - insqr3 * d2 * s35;                // (2,5)
+  f.mp[9] = sd4 + sd2;                                   // (3,3)
+  f.mp[10] = insqr3 * d1 * d2 - d4 * f35;                // (3,4)
+  f.mp[11] = insqr3 * (sd2 - sd4);                       // (3,5)
+  f.mp[12] = in3 * (sd1 + sd2) + f35 * f35;              // (4,4)
+  f.mp[13] = d1 * d2 + insqr3 * d4 * f35;                // (4,5)
+  f.mp[14] = in3 * (sd2 + sd4 + 4.0 * sd1);              // (5,5)
+}
+
+// Table of one radial potential (mode 0: va) or of a difference (mode 1:
+// vb - va), potenimpini.f90:35-58, with the end copies of :60-66.
+__global__ void k_state_table(effpot_plus va, effpot_plus vb, int mode, int npot, double delinter, double* tab) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= npot + 1; i += gridDim.x * blockDim.x) {
+    const int j = i == 0 ? 1 : (i == npot + 1 ? npot : i);
+    const double r = (double)(j - 1) * delinter;
+    const double a = plus_pot(va, r);
+    tab[i] = mode == 0 ? a : plus_pot(vb, r) - a;
+  }
+}
+
+// Density-weighted moments of one slot: block partials [kStateBlocks][K].
+template <typename V, int K>
+__global__ __launch_bounds__(256) void k_state_moments(const V* psi, StateArgs a, double* partial) {
+  __shared__ double red[4 * K];
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  const long N = (long)a.nx * a.ny * a.nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % a.nx);
+    const long rr = i / a.nx;
+    const int iy = (int)(rr % a.ny), iz = (int)(rr / a.ny);
+    const double xx = (a.x0 + a.hx * ix) - a.rx, yy = (a.y0 + a.hy * iy) - a.ry, zz = (a.z0 + a.hz * iz) - a.rz;
+    const double den = (double)norm2(psi[i]);
+    double t0, t1, t2;
+    state_fields(a, (xx * xx + yy * yy) + zz * zz, t0, t1, t2);
+    if constexpr (K == kMomP) {  // instates.f90:145-153
+      v[0] += den;
+      v[1] += den * (t0 + t1 * xx * xx);
+      v[2] += den * (t0 + t1 * yy * yy);
+      v[3] += den * (t0 + t1 * zz * zz);
+      v[4] += den * t1 * xx * yy;
+      v[5] += den * t1 * xx * zz;
+      v[6] += den * t1 * yy * zz;
+    } else {  // instates.f90:466-517: MP weighted by Sig_Del, MS by Pi_Del
+      DForms f;
+      d_forms(xx, yy, zz, f);
+      const double dv1 = t1 * den, dv2 = t2 * den;
+#pragma unroll
+      for (int k = 0; k < 15; k++) v[k] += f.mp[k] * dv2;
+      int k = 15;
+#pragma unroll
+      for (int p = 0; p < 5; p++)
+#pragma unroll
+        for (int q = p; q < 5; q++) v[k++] += (f.d[p] * f.d[q]) * dv1;
+      v[30] += t0 * den;
+    }
+  }
+  block_sum<K>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partial[(long)blockIdx.x * K + k] = v[k];
+  }
+}
+
+// potenimp (:34-45 P, :54-158 D) and the cap (:184-186) for one slot.
+template <typename T>
+__global__ __launch_bounds__(256) void k_state_uimp(StateArgs a, T* uimp) {
+  const long N = (long)a.nx * a.ny * a.nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % a.nx);
+    const long rr = i / a.nx;
+    const int iy = (int)(rr % a.ny), iz = (int)(rr / a.ny);
+    const double xx = (a.x0 + a.hx * ix) - a.rx, yy = (a.y0 + a.hy * iy) - a.ry, zz = (a.z0 + a.hz * iz) - a.rz;
+    double t0, t1, t2;
+    state_fields(a, (xx * xx + yy * yy) + zz * zz, t0, t1, t2);
+    double u;
+    if (a.state == EFFPOT_STATE_P) {
+      // odd (spin up) and even (spin down) components of iv . r
+      const double ore = a.inv[0] * xx + a.inv[4] * yy + a.inv[8] * zz;
+      const double oim = a.inv[1] * xx + a.inv[5] * yy + a.inv[9] * zz;
+      const double ere = a.inv[2] * xx + a.inv[6] * yy + a.inv[10] * zz;
+      const double eim = a.inv[3] * xx + a.inv[7] * yy + a.inv[11] * zz;
+      u = t0 + t1 * ((ore * ore + oim * oim) + (ere * ere + eim * eim));
+    } else {
+      DForms f;
+      d_forms(xx, yy, zz, f);
+      // M = Delta*I + Pi_Del*(d d^T) + Sig_Del*MP; u = sum_ij M_ij Re(conj(c_i) c_j)
+      // over both spin components, c_i = iv(2i-1) (up) and iv(2i) (down)
+      u = 0.0;
+      int k = 0;
+#pragma unroll
+      for (int p = 0; p < 5; p++)
+#pragma unroll
+        for (int q = p; q < 5; q++, k++) {
+          double m = f.mp[k] * t2 + (f.d[p] * f.d[q]) * t1;
+          if (p == q) m += t0;
+          const double* cp = a.inv + 4 * p;
+          const double* cq = a.inv + 4 * q;
+          const double g = (cp[0] * cq[0] + cp[1] * cq[1]) + (cp[2] * cq[2] + cp[3] * cq[3]);
+          u += (p == q ? 1.0 : 2.0) * m * g;
+        }
+    }
+    uimp[i] = (T)(u < a.cap ? u : a.cap);
+  }
+}
+
+// ---- host half of instates -------------------------------------------------
+using zd = std::complex<double>;
+
+// V + (Als/2) SO of the moments (instates.f90:159-195 P, :520-565 D), n x n
+// row-major.
+inline int state_matrix(int state, const double* m, double als, zd* H) {
+  const int n = state == EFFPOT_STATE_P ? 6 : 10;
+  for (int i = 0; i < n * n; i++) H[i] = 0.0;
+  auto at = [&](int i, int j) -> zd& { return H[(i - 1) * n + (j - 1)]; };  // 1-based
+  const zd I(0.0, 1.0);
+  const double also2 = 0.5 * als;
+  zd so[10][10] = {};
+  auto S = [&](int i, int j) -> zd& { return so[i - 1][j - 1]; };
+  if (state == EFFPOT_STATE_P) {
+    const double Vx = m[1], Vy = m[2], Vz = m[3], Vxy = m[4], Vxz = m[5], Vyz = m[6];
+    at(1, 1) = at(2, 2) = Vx; at(3, 3) = at(4, 4) = Vy; at(5, 5) = at(6, 6) = Vz;
+    at(1, 3) = at(2, 4) = Vxy; at(1, 5) = at(2, 6) = Vxz; at(3, 5) = at(4, 6) = Vyz;
+    S(1, 3) = -I; S(2, 4) = I; S(2, 5) = -1.0; S(1, 6) = 1.0; S(4, 5) = -I; S(3, 6) = -I;
+  } else {
+    double mp[5][5], ms[5][5];
+    int k = 0;
+    for (int p = 0; p < 5; p++)
+      for (int q = p; q < 5; q++, k++) { mp[p][q] = mp[q][p] = m[k]; }
+    for (int p = 0; p < 5; p++)
+      for (int q = p; q < 5; q++, k++) { ms[p][q] = ms[q][p] = m[k]; }
+    for (int p = 0; p < 5; p++)
+      for (int q = 0; q < 5; q++) {
+        const double v = (p == q ? m[30] : 0.0) + ms[p][q] + mp[p][q];
+        H[(2 * p) * n + 2 * q] = v;          // MatD(2i-1, 2j-1)
+        H[(2 * p + 1) * n + 2 * q + 1] = v;  // MatD(2i, 2j)
+      }
+    const double s3 = std::sqrt(3.0);
+    S(1, 4) = 1.0; S(1, 8) = -I; S(1, 9) = 2.0 * I; S(2, 3) = -1.0; S(2, 7) = -I; S(2, 10) = -2.0 * I;
+    S(3, 6) = -I * s3; S(3, 7) = I; S(3, 10) = -I; S(4, 5) = -I * s3; S(4, 8) = -I; S(4, 9) = -I;
+    S(5, 8) = -s3; S(6, 7) = s3; S(7, 10) = -1.0; S(8, 9) = 1.0;
+  }
+  for (int i = 1; i <= n; i++)
+    for (int j = i + 1; j <= n; j++) {
+      S(j, i) = std::conj(S(i, j));
+      if (state == EFFPOT_STATE_P) at(j, i) = at(i, j);  // Mat(j,i) = Mat(i,j)
+    }
+  for (int i = 1; i <= n; i++)
+    for (int j = 1; j <= n; j++) at(i, j) += also2 * S(i, j);
+  return n;
+}
+
+// (Als/2) <v|SO|v>: energy.f90:170-192 (P and D).
+inline double state_eso(int state, const double* inv, double als) {
+  const int n = state == EFFPOT_STATE_P ? 6 : 10;
+  zd H[100];
+  double zero[kMomD] = {};
+  state_matrix(state, zero, als, H);  // with zero moments H = (Als/2) SO
+  zd acc = 0.0;
+  for (int i = 0; i < n; i++) {
+    zd hv = 0.0;
+    for (int j = 0; j < n; j++) hv += H[i * n + j] * zd(inv[2 * j], inv[2 * j + 1]);
+    acc += std::conj(zd(inv[2 * i], inv[2 * i + 1])) * hv;
+  }
+  return acc.real();
+}
+
+// Eigenvalues (ascending) and eigenvectors (columns of V) of a Hermitian
+// n x n matrix by cyclic complex Jacobi rotations: each rotation first
+// removes the phase of a_pq, then is the real symmetric Jacobi rotation of
+// the 2x2 block (Numerical Recipes' tan formula).
+inline void herm_eigen(int n, zd* A, double* w, zd* V) {
+  for (int i = 0; i < n * n; i++) V[i] = (i / n == i % n) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < n; p++) {
+      diag += std::norm(A[p * n + p]);
+      for (int q = p + 1; q < n; q++) off += std::norm(A[p * n + q]);
+    }
+    if (off == 0.0 || off <= 1e-33 * (diag + 2.0 * off)) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const zd apq = A[p * n + q];
+        const double g = std::abs(apq);
+        if (g == 0.0) continue;
+        const zd e = apq / g;  // e^{i phi}
+        const double app = A[p * n + p].real(), aqq = A[q * n + q].real();
+        const double th = (aqq - app) / (2.0 * g);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        const zd ec = std::conj(e);
+        // U = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] on (p, q); A <- U^H A U, V <- V U
+        for (int k = 0; k < n; k++) {  // columns
+          const zd akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * ec * akq;
+          A[k * n + q] = s * akp + c * ec * akq;
+          const zd vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * ec * vkq;
+          V[k * n + q] = s * vkp + c * ec * vkq;
+        }
+        for (int k = 0; k < n; k++) {  // rows
+          const zd apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * e * aqk;
+          A[q * n + k] = s * apk + c * e * aqk;
+        }
+        A[p * n + q] = A[q * n + p] = 0.0;
+        A[p * n + p] = A[p * n + p].real();
+        A[q * n + q] = A[q * n + q].real();
+      }
+  }
+  int idx[10];
+  for (int i = 0; i < n; i++) { idx[i] = i; w[i] = A[i * n + i].real(); }
+  for (int i = 1; i < n; i++)  // insertion sort, ascending
+    for (int j = i; j > 0 && w[idx[j]] < w[idx[j - 1]]; j--) std::swap(idx[j], idx[j - 1]);
+  double ws[10];
+  zd Vs[100];
+  for (int k = 0; k < n; k++) {
+    ws[k] = w[idx[k]];
+    for (int i = 0; i < n; i++) Vs[i * n + k] = V[i * n + idx[k]];
+  }
+  std::memcpy(w, ws, sizeof(double) * n);
+  for (int i = 0; i < n * n; i++) V[i] = Vs[i];
+}
+
+// instates' default selection (:360-364 P, :722-726 D): the equal-weight
+// sum of the two vectors of Kramers pair `instate` (0 = lowest).
+inline void state_select(int n, const zd* V, int instate, double* inv) {
+  const double c = 1.0 / std::sqrt(2.0);
+  std::memset(inv, 0, sizeof(double) * 20);
+  for (int i = 0; i < n; i++) {
+    const zd v = c * V[i * n + 2 * instate] + c * V[i * n + 2 * instate + 1];
+    inv[2 * i] = v.real();
+    inv[2 * i + 1] = v.imag();
+  }
+}
+
+}  // namespace ep

@@ -117,6 +117,30 @@ typedef struct effpot_plus {
   double pg_lo[EFFPOT_SERIES_MAX]; /* low parts (0 for plain doubles)      */
 } effpot_plus;
 
+/* Impurity in a P or D internal state (Lstate 'P' / 'D', modules.f90:303):
+ * the He field of potenimp (potenimp.f90:34-180) for the internal state
+ * invar, chosen by instates (instates.f90:1-800) from the current helium
+ * density.  The radial potentials are Select_Pot entries (V_pi, V_sigma,
+ * V_delta, V_impur.f90:66-85); their umax also gives updatepoten's values at
+ * r <= 1e-3 A (potenimp.f90:536-563).  Fields marked "out" are written. */
+#define EFFPOT_STATE_P 1   /* 6 components: (x,y,z) orbital x (up,down) spin  */
+#define EFFPOT_STATE_D 2   /* 10 components: d1..d5 x (up,down)               */
+typedef struct effpot_state_imp {
+  int32_t state;       /* EFFPOT_STATE_P | EFFPOT_STATE_D                   */
+  int32_t instate;     /* >= 0: the equal-weight sum of Kramers pair `instate`
+                          (0 = lowest) of V + SO (instates.f90:360-364, :722-726);
+                          -1: invar as given (readenc instate 5)          */
+  double als;          /* spin-orbit splitting Als_P / Als_D (K)            */
+  double rimp[3];      /* impurity position (A)                            */
+  effpot_plus v_pi, v_sigma, v_delta; /* v_delta is unused for P           */
+  double invar[20];    /* in (instate -1) / out: complex invar(1..6|10)     */
+  double eigen[10];    /* out: eigenvalues of V + SO, ascending (K)         */
+  double moments[31];  /* out: density moments of instates (P: A, Vx, Vy, Vz,
+                          Vxy, Vxz, Vyz; D: MP(i<=j), MS(i<=j) row-major,
+                          then the Delta sum), times dxyz                  */
+  double eso;          /* out: spin-orbit energy (energy.f90:170-192)       */
+} effpot_state_imp;
+
 /* Energies as printed by DFT4He3d.f90:460-463 (formats 6050/6060) and
  * computed by energy.f90:12-202, plus the chemical potential and the
  * helium centre of mass (r_cm.f90:22-57). */
@@ -233,6 +257,20 @@ int effpot_set_dt(effpot_ctx* ctx, int slot, double dt);
 
 /* ---- impurity field (potenimpini.f90 / updatepoten+update+potenimp) ---- */
 int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp);
+
+/* Impurity of one slot in a P or D internal state: tabulates the radial
+ * potentials, runs instates on the slot's current density (instate >= 0),
+ * writes the capped field uimp and keeps eso for the slot's energies
+ * (eimpu = ekinx + eHeX + eso).  Replaces potenimpini + updatepoten +
+ * instates + potenimp for Lstate 'P' / 'D'; effpot_set_impurity (isotropic)
+ * resets eso to 0. */
+int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp);
+
+/* The host half of instates: V + (Als/2) SO from the density moments (as in
+ * effpot_state_imp.moments), its eigenvalues (ascending) and, for
+ * instate >= 0, the selected invar and its eso.  Host only. */
+int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
+                    double* eso);
 
 /* X+-He potential of one slot for the uplus energy term, evaluated at the
  * slot's impurity position (energy.f90:146-163).  A context starts with no

@@ -128,6 +128,8 @@ typedef struct oracle {
   int npot; double delinter; double* vtab; /* vtab[0..npot+1] */
   effpot_impurity imp; int have_imp;
   effpot_plus plus; /* X+ potential of uplus (kind NULL: none) */
+  double eso;       /* spin-orbit energy of a P/D-state impurity */
+  double *stab, *sf[3]; /* P/D state: potential tables [3][npot+2], updatepoten fields */
 } oracle_t;
 
 static int tstgrid(int n) { /* tstgrid.f90:8-12: n = 2^a 3^b 5^c <= 256 */
@@ -347,7 +349,8 @@ void oracle_destroy(oracle_t* o) {
                     o->dzden, o->falfs, o->intx, o->inty, o->intz, o->sto1, o->rw1, o->rw2,
                     o->rw3, o->rw4, o->rw5, o->rw6, (double*)o->fden, (double*)o->wk1,
                     (double*)o->wk2, o->psiold[0], o->psiold[1], o->psiold[2],
-                    o->hpsiold[0], o->hpsiold[1], o->pc, o->q, o->lap, o->hpsi, o->vtab};
+                    o->hpsiold[0], o->hpsiold[1], o->pc, o->q, o->lap, o->hpsi, o->vtab,
+                    o->stab, o->sf[0], o->sf[1], o->sf[2]};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
   free(o->plx.tw); free(o->ply.tw); free(o->plz.tw);
   free(o);
@@ -513,7 +516,7 @@ double oracle_update(double r, int npot, double rmax, const double* f) {
 
 int oracle_set_impurity(oracle_t* o, const effpot_impurity* imp) {
   if (imp->kind != EFFPOT_POT_MORSE && imp->kind != EFFPOT_POT_NULL) return EFFPOT_ERR_POT;
-  o->imp = *imp; o->have_imp = 1;
+  o->imp = *imp; o->have_imp = 1; o->eso = 0.0;
   /* potenimpini.f90:35-58 (isotropic: V_pi = V, Delta = 0) */
   for (int i = 1; i <= o->npot; i++) o->vtab[i] = oracle_select_pot(imp, (double)(i - 1) * o->delinter);
   o->vtab[0] = o->vtab[1];
@@ -558,6 +561,222 @@ double oracle_plus_pot(const effpot_plus* p, double r) {
 void oracle_set_plus(oracle_t* o, const effpot_plus* p) {
   if (p) o->plus = *p;
   else memset(&o->plus, 0, sizeof(o->plus));
+}
+
+/* ------------------------------------------------------------------------ */
+/* P/D internal state: potenimpini (potenimpini.f90:14-66), updatepoten     */
+/* (potenimp.f90:531-565), the grid sums and matrices of instates           */
+/* (instates.f90:136-195 P, :448-565 D), potenimp (potenimp.f90:34-186) and */
+/* the spin-orbit energy (energy.f90:168-192).  The eigenvectors are left   */
+/* to the caller (the tests diagonalise the matrix with LAPACK).            */
+/* ------------------------------------------------------------------------ */
+static void state_prepare(oracle_t* o, const effpot_state_imp* s) {
+  const int L = o->npot + 2;
+  if (!o->stab) {
+    o->stab = dalloc(3 * (size_t)L);
+    for (int k = 0; k < 3; k++) o->sf[k] = dalloc(o->N);
+  }
+  double* t0 = o->stab; double* t1 = o->stab + L; double* t2 = o->stab + 2 * L;
+  for (int i = 1; i <= o->npot; i++) {
+    double r = (double)(i - 1) * o->delinter;
+    if (s->state == EFFPOT_STATE_P) {
+      double aux = oracle_plus_pot(&s->v_pi, r);
+      t0[i] = aux;                                   /* potpi  */
+      t1[i] = oracle_plus_pot(&s->v_sigma, r) - aux; /* potdel */
+    } else {
+      double aux = oracle_plus_pot(&s->v_delta, r);
+      t0[i] = aux;                                   /* potdel    */
+      t1[i] = oracle_plus_pot(&s->v_pi, r) - aux;    /* potpidel  */
+      t2[i] = oracle_plus_pot(&s->v_sigma, r) - aux; /* potsigdel */
+    }
+  }
+  double* ts[3] = {t0, t1, t2};
+  for (int k = 0; k < 3; k++) { ts[k][0] = ts[k][1]; ts[k][o->npot + 1] = ts[k][o->npot]; }
+  const double rmax = (double)(o->npot - 1) * o->delinter, zeror = 1e-3;
+  for (int iz = 0; iz < o->nz; iz++) {
+    double zt = (o->z[iz] - s->rimp[2]) * (o->z[iz] - s->rimp[2]);
+    for (int iy = 0; iy < o->ny; iy++) {
+      double yt = (o->y[iy] - s->rimp[1]) * (o->y[iy] - s->rimp[1]) + zt;
+      for (int ix = 0; ix < o->nx; ix++) {
+        size_t i = IDX(o, ix, iy, iz);
+        double r2 = (o->x[ix] - s->rimp[0]) * (o->x[ix] - s->rimp[0]) + yt, r = sqrt(r2);
+        if (s->state == EFFPOT_STATE_P) {
+          o->sf[0][i] = oracle_update(r, o->npot, rmax, t0);                                      /* Vpi   */
+          o->sf[1][i] = r > zeror ? oracle_update(r, o->npot, rmax, t1) / r2 : s->v_sigma.umax;   /* Delta */
+        } else {
+          double r4 = r2 * r2;
+          o->sf[0][i] = oracle_update(r, o->npot, rmax, t0);                                      /* Delta   */
+          o->sf[1][i] = r > zeror ? oracle_update(r, o->npot, rmax, t1) / r4 : s->v_pi.umax;      /* Pi_Del  */
+          o->sf[2][i] = r > zeror ? oracle_update(r, o->npot, rmax, t2) / r4 : s->v_sigma.umax;   /* Sig_Del */
+        }
+      }
+    }
+  }
+}
+
+/* armonicd and the MP entries of instates/potenimp for one point. */
+static void state_dforms(double xx, double yy, double zz, double* arm, double mp[5][5]) {
+  const double sqr3 = sqrt(3.0), sqr3o2 = sqrt(3.0) / 2.0, insqr3 = 1.0 / sqrt(3.0), in3 = 1.0 / 3.0;
+  double d0 = xx * xx + yy * yy + zz * zz, d1 = sqr3 * xx * yy, d2 = sqr3 * yy * zz;
+  double d3 = (-xx * xx - yy * yy + 2 * zz * zz) / 2.0, d4 = sqr3 * xx * zz, d5 = sqr3o2 * (xx * xx - yy * yy);
+  double sd1 = d1 * d1, sd2 = d2 * d2, sd4 = d4 * d4, sd5 = d5 * d5;
+  double sumd3d5 = d3 + insqr3 * d5, difd3d5 = d3 - insqr3 * d5;
+  arm[0] = d1; arm[1] = d2; arm[2] = d3; arm[3] = d4; arm[4] = d5;
+  mp[0][0] = in3 * (sd2 + sd4 + 4.0 * sd5);
+  mp[1][1] = in3 * (sd1 + sd4) + sumd3d5 * sumd3d5;
+  mp[2][2] = sd4 + sd2;
+  mp[3][3] = in3 * (sd1 + sd2) + difd3d5 * difd3d5;
+  mp[4][4] = in3 * (sd2 + sd4 + 4.0 * sd1);
+  mp[0][1] = -4.0 * in3 * d1 * d2 + insqr3 * d4 * d0;
+  mp[0][2] = -2.0 * insqr3 * d2 * d4;
+  mp[0][3] = -4.0 * in3 * d1 * d4 + insqr3 * d2 * d0;
+  mp[0][4] = -4.0 * in3 * d1 * d5;
+  mp[1][2] = insqr3 * d1 * d4 - d2 * sumd3d5;
+  mp[1][3] = -4.0 * in3 * d2 * d4 + insqr3 * d1 * d0;
+  mp[1][4] = -d1 * d4 - insqr3 * d2 * sumd3d5;
+  mp[2][3] = insqr3 * d1 * d2 - d4 * difd3d5;
+  mp[2][4] = insqr3 * (sd2 - sd4);
+  mp[3][4] = d1 * d2 + insqr3 * d4 * difd3d5;
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < i; j++) mp[i][j] = mp[j][i];
+}
+
+/* instates' grid sums (times dxyz) in the library's moment order. */
+int oracle_state_moments(oracle_t* o, const effpot_state_imp* s, double* mom) {
+  if (s->state != EFFPOT_STATE_P && s->state != EFFPOT_STATE_D) return EFFPOT_ERR_ARG;
+  state_prepare(o, s);
+  for (int k = 0; k < 31; k++) mom[k] = 0.0;
+  for (int iz = 0; iz < o->nz; iz++)
+    for (int iy = 0; iy < o->ny; iy++)
+      for (int ix = 0; ix < o->nx; ix++) {
+        size_t i = IDX(o, ix, iy, iz);
+        double xx = o->x[ix] - s->rimp[0], yy = o->y[iy] - s->rimp[1], zz = o->z[iz] - s->rimp[2];
+        double den = o->den[i];
+        if (s->state == EFFPOT_STATE_P) { /* instates.f90:145-153 */
+          double vpi = o->sf[0][i], dv = o->sf[1][i];
+          mom[0] += den;
+          mom[1] += den * (vpi + dv * xx * xx);
+          mom[2] += den * (vpi + dv * yy * yy);
+          mom[3] += den * (vpi + dv * zz * zz);
+          mom[4] += den * dv * xx * yy;
+          mom[5] += den * dv * xx * zz;
+          mom[6] += den * dv * yy * zz;
+        } else { /* instates.f90:460-517 */
+          double arm[5], mp[5][5];
+          state_dforms(xx, yy, zz, arm, mp);
+          double dvdel = o->sf[0][i] * den, dv1 = o->sf[1][i] * den, dv2 = o->sf[2][i] * den;
+          int k = 0;
+          for (int p = 0; p < 5; p++)
+            for (int q = p; q < 5; q++) mom[k++] += mp[p][q] * dv2;
+          for (int p = 0; p < 5; p++)
+            for (int q = p; q < 5; q++) mom[k++] += (arm[p] * arm[q]) * dv1;
+          mom[30] += dvdel;
+        }
+      }
+  for (int k = 0; k < 31; k++) mom[k] *= o->dxyz;
+  return EFFPOT_OK;
+}
+
+/* V + (Als/2) SO as instates builds it (P :159-195, D :520-565), n x n
+ * row-major, re/im interleaved. */
+int oracle_state_matrix(int state, const double* mom, double als, double* H) {
+  const int n = state == EFFPOT_STATE_P ? 6 : 10;
+  double vr[10][10] = {{0}}, sr[10][10] = {{0}}, si[10][10] = {{0}};
+  if (state == EFFPOT_STATE_P) {
+    double Vx = mom[1], Vy = mom[2], Vz = mom[3], Vxy = mom[4], Vxz = mom[5], Vyz = mom[6];
+    vr[0][0] = vr[1][1] = Vx; vr[2][2] = vr[3][3] = Vy; vr[4][4] = vr[5][5] = Vz;
+    vr[0][2] = vr[1][3] = Vxy; vr[0][4] = vr[1][5] = Vxz; vr[2][4] = vr[3][5] = Vyz;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) vr[i][j] = vr[j][i];
+    /* SO(1,3)=-i SO(2,4)=i SO(2,5)=-1 SO(1,6)=1 SO(4,5)=-i SO(3,6)=-i, Hermitian */
+    si[0][2] = -1; si[1][3] = 1; sr[1][4] = -1; sr[0][5] = 1; si[3][4] = -1; si[2][5] = -1;
+  } else {
+    double mp[5][5], ms[5][5];
+    int k = 0;
+    for (int p = 0; p < 5; p++)
+      for (int q = p; q < 5; q++) { mp[p][q] = mp[q][p] = mom[k++]; }
+    for (int p = 0; p < 5; p++)
+      for (int q = p; q < 5; q++) { ms[p][q] = ms[q][p] = mom[k++]; }
+    for (int p = 0; p < 5; p++)
+      for (int q = 0; q < 5; q++) {
+        double v = (p == q ? mom[30] : 0.0) + ms[p][q] + mp[p][q];
+        vr[2 * p][2 * q] = v; vr[2 * p + 1][2 * q + 1] = v;
+      }
+    const double s3 = sqrt(3.0);
+    sr[0][3] = 1; si[0][7] = -1; si[0][8] = 2; sr[1][2] = -1; si[1][6] = -1; si[1][9] = -2;
+    si[2][5] = -s3; si[2][6] = 1; si[2][9] = -1; si[3][4] = -s3; si[3][7] = -1; si[3][8] = -1;
+    sr[4][7] = -s3; sr[5][6] = s3; sr[6][9] = -1; sr[7][8] = 1;
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++) { sr[i][j] = sr[j][i]; si[i][j] = -si[j][i]; }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      H[2 * (i * n + j)] = vr[i][j] + 0.5 * als * sr[i][j];
+      H[2 * (i * n + j) + 1] = 0.5 * als * si[i][j];
+    }
+  return n;
+}
+
+/* eso of invar: energy.f90:170-172 (P, closed form) and :177-190 (D). */
+double oracle_state_eso(int state, const double* iv, double als) {
+  if (state == EFFPOT_STATE_P) {
+    /* c = conj(invar): 2 Re(c1 v6 - c2 v5) + 2 Im(c1 v3 + c4 v2 + c3 v6 + c4 v5) */
+#define CRE(a, b) (iv[2 * (a)] * iv[2 * (b)] + iv[2 * (a) + 1] * iv[2 * (b) + 1])
+#define CIM(a, b) (iv[2 * (a)] * iv[2 * (b) + 1] - iv[2 * (a) + 1] * iv[2 * (b)])
+    double t = 2.0 * (CRE(0, 5) - CRE(1, 4)) + 2.0 * (CIM(0, 2) + CIM(3, 1) + CIM(2, 5) + CIM(3, 4));
+#undef CRE
+#undef CIM
+    return 0.5 * als * t;
+  }
+  double zero[31] = {0}, H[200];
+  oracle_state_matrix(EFFPOT_STATE_D, zero, 2.0, H); /* H = SOD */
+  double re = 0.0;
+  for (int i = 0; i < 10; i++)
+    for (int j = 0; j < 10; j++) {
+      double hr = H[2 * (i * 10 + j)], hi = H[2 * (i * 10 + j) + 1];
+      double vr = iv[2 * j], vi = iv[2 * j + 1], cr = iv[2 * i], ci = -iv[2 * i + 1];
+      re += cr * (hr * vr - hi * vi) - ci * (hr * vi + hi * vr);
+    }
+  return 0.5 * re * als;
+}
+
+/* potenimp for a given invar (P :34-45, D :54-158), cap (:184-186); keeps
+ * eso for energy(). */
+int oracle_set_impurity_state(oracle_t* o, const effpot_state_imp* s) {
+  if (s->state != EFFPOT_STATE_P && s->state != EFFPOT_STATE_D) return EFFPOT_ERR_ARG;
+  state_prepare(o, s);
+  const double* iv = s->invar;
+  for (int iz = 0; iz < o->nz; iz++)
+    for (int iy = 0; iy < o->ny; iy++)
+      for (int ix = 0; ix < o->nx; ix++) {
+        size_t i = IDX(o, ix, iy, iz);
+        double xx = o->x[ix] - s->rimp[0], yy = o->y[iy] - s->rimp[1], zz = o->z[iz] - s->rimp[2];
+        double u;
+        if (s->state == EFFPOT_STATE_P) {
+          double odr = iv[0] * xx + iv[4] * yy + iv[8] * zz, odi = iv[1] * xx + iv[5] * yy + iv[9] * zz;
+          double evr = iv[2] * xx + iv[6] * yy + iv[10] * zz, evi = iv[3] * xx + iv[7] * yy + iv[11] * zz;
+          u = o->sf[0][i] + o->sf[1][i] * (odr * odr + odi * odi + evr * evr + evi * evi);
+        } else {
+          double arm[5], mp[5][5], m[5][5];
+          state_dforms(xx, yy, zz, arm, mp);
+          for (int p = 0; p < 5; p++)
+            for (int q = 0; q < 5; q++)
+              m[p][q] = (p == q ? o->sf[0][i] : 0.0) + arm[p] * arm[q] * o->sf[1][i] + mp[p][q] * o->sf[2][i];
+          /* sum_ij MatD_ij conj(iv_i) iv_j, MatD = m (x) 1_2 */
+          u = 0.0;
+          for (int a = 0; a < 10; a++)
+            for (int b = 0; b < 10; b++) {
+              if ((a & 1) != (b & 1)) continue;
+              double md = m[a / 2][b / 2];
+              u += md * (iv[2 * a] * iv[2 * b] + iv[2 * a + 1] * iv[2 * b + 1]);
+            }
+        }
+        o->uimp[i] = u < 1000.0 ? u : 1000.0;
+      }
+  o->eso = oracle_state_eso(s->state, iv, s->als);
+  memcpy(o->imp.rimp, s->rimp, sizeof(s->rimp));
+  o->have_imp = 1;
+  return EFFPOT_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -673,7 +892,7 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
     e->etot4 += e->ealphas;
   }
   e->eHeX = ehex * o->dxyz;
-  e->ekinx = 0.0; e->eso = 0.0;
+  e->ekinx = 0.0; e->eso = o->eso;
   e->uplus = upl * o->dxyz;
   e->eimpu = e->ekinx + e->eHeX + e->eso;
   e->etot = e->etot4 + e->eimpu;

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from eff_pot_amd.abi import Params, Impurity, Plus, Energies, SolveOpts, Result, OK
+from eff_pot_amd.abi import Params, Impurity, Plus, StateImp, Energies, SolveOpts, Result, OK
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -52,6 +52,11 @@ def lib():
         L.oracle_plus_pot.argtypes = [C.POINTER(Plus), C.c_double]
         L.oracle_plus_pot.restype = C.c_double
         L.oracle_set_plus.argtypes = [C.c_void_p, C.POINTER(Plus)]
+        L.oracle_state_moments.argtypes = [C.c_void_p, C.POINTER(StateImp), _dp]
+        L.oracle_state_matrix.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.oracle_state_eso.argtypes = [C.c_int, _dp, C.c_double]
+        L.oracle_state_eso.restype = C.c_double
+        L.oracle_set_impurity_state.argtypes = [C.c_void_p, C.POINTER(StateImp)]
         L.oracle_update.argtypes = [C.c_double, C.c_int, C.c_double, _dp]
         L.oracle_update.restype = C.c_double
         L.oracle_poten.argtypes = [C.c_void_p]
@@ -78,6 +83,23 @@ def lib():
 def plus_pot(plus, r):
     """Select_Pot of an X+ potential at r, summed in quad precision."""
     return lib().oracle_plus_pot(C.byref(plus), r)
+
+
+def state_matrix(state, moments, als):
+    """V + (Als/2) SO of instates from the density moments (complex n x n)."""
+    n = 6 if state == 1 else 10
+    m = np.zeros(31)
+    m[:len(moments)] = moments
+    H = np.zeros(2 * n * n)
+    lib().oracle_state_matrix(state, m, als, H)
+    return (H[0::2] + 1j * H[1::2]).reshape(n, n)
+
+
+def state_eso(state, invar, als):
+    v = np.zeros(20)
+    v[0:2 * len(invar):2] = np.real(invar)
+    v[1:2 * len(invar):2] = np.imag(invar)
+    return lib().oracle_state_eso(state, v, als)
 
 
 def fd_coeffs(id_):
@@ -186,6 +208,17 @@ class Oracle:
         rc = lib().oracle_set_impurity(self.h, C.byref(imp))
         if rc != OK:
             raise ValueError(f"oracle_set_impurity failed: {rc}")
+
+    def state_moments(self, imp: StateImp):
+        m = np.zeros(31)
+        rc = lib().oracle_state_moments(self.h, C.byref(imp), m)
+        assert rc == OK
+        return m
+
+    def set_impurity_state(self, imp: StateImp):
+        """potenimp for imp.invar (the caller chooses it)."""
+        rc = lib().oracle_set_impurity_state(self.h, C.byref(imp))
+        assert rc == OK
 
     def set_plus(self, plus):
         lib().oracle_set_plus(self.h, C.byref(plus) if plus is not None else None)

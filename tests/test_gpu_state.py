@@ -1,0 +1,94 @@
+"""P/D internal-state impurity on the GPU against the oracle.
+
+Two slots of one context (a P state and a D state at different positions,
+the reference's default Ba+ potentials or smooth attractive ones): the
+density moments of instates within 1e-12 of the oracle's, the eigenvalues
+within 1e-11 of LAPACK on the oracle's matrix, the field uimp within 1e-12 of
+the oracle's potenimp for the same invar, eso equal, and E_imp = eHeX + eso
+equal to the selected eigenvalue when the 1000 K cap does not act.  A short
+relaxation with a P-state impurity follows the oracle's to 1e-9."""
+import numpy as np
+import pytest
+
+import eff_pot_amd as ep
+from eff_pot_amd.abi import make_params, CORE_OTC, SolveOpts, PREC_MIXED
+from oracle import oracle as O
+from test_state_impurity import POTS
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (state, instate, rimp, smooth potentials)
+    (ep.STATE_P, 0, (0.0, 0.5, -1.0), False),
+    (ep.STATE_D, 1, (1.0, 0.0, 0.75), False),
+    (ep.STATE_P, 1, (-0.75, 0.25, 0.5), True),
+    (ep.STATE_D, 2, (0.5, -0.5, 0.0), True),
+]
+
+
+def _imp(state, instate, rimp, smooth, als=4.0):
+    return ep.make_state_imp(state, rimp=rimp, instate=instate, als=als, **(POTS if smooth else {}))
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_state_impurity_matches_oracle(gpu_available, precision):
+    print(f"[state] 32^3 x 4 slots precision={precision}", flush=True)
+    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=4, precision=precision)
+    with ep.Context(p) as ctx:
+        got, psis, fields = [], [], []
+        for s, case in enumerate(CASES):
+            ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 40 + s, slot=s)
+            got.append(ctx.set_impurity_state(_imp(*case), slot=s))
+            psis.append(ctx.psi(s))
+            fields.append(ctx.field("uimp", slot=s))
+        ctx.poten()
+        es = ctx.energy()
+    for s, (state, instate, rimp, smooth) in enumerate(CASES):
+        n = 6 if state == ep.STATE_P else 10
+        g = got[s]
+        o = O.Oracle(make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377))
+        o.set_psi(psis[s])
+        ref = _imp(state, instate, rimp, smooth)
+        m = o.state_moments(ref)
+        gm = np.array(g.moments)
+        assert np.max(np.abs(gm - m)) <= 1e-12 * np.abs(m).max(), (s, gm - m)
+        wl = np.linalg.eigvalsh(O.state_matrix(state, m, ref.als))
+        assert np.max(np.abs(np.array(g.eigen)[:n] - wl)) <= 1e-11 * np.abs(wl).max()
+        for i in range(20):
+            ref.invar[i] = g.invar[i]
+        o.set_impurity_state(ref)
+        u = o.field("uimp")
+        assert np.max(np.abs(fields[s] - u)) <= 1e-12 * np.abs(u).max(), s
+        o.poten()
+        e = o.energy()
+        o.close()
+        assert abs(g.eso - e["eso"]) <= 1e-12 * max(1.0, abs(e["eso"])) and es[s]["eso"] == g.eso
+        for k in ("eHeX", "eimpu", "etot"):
+            assert abs(es[s][k] - e[k]) <= 1e-10 * abs(e[k]), (s, k, es[s][k], e[k])
+        if smooth:  # no cap: E_imp is the eigenvalue of the selected pair
+            assert abs(es[s]["eimpu"] - g.eigen[2 * instate]) <= 1e-10 * abs(g.eigen[2 * instate])
+        print(s, g.eigen[2 * instate], es[s]["eimpu"], g.eso)
+
+
+def test_state_relaxation_matches_oracle(gpu_available):
+    print("[state] P-state relaxation 32^3, 300 steps", flush=True)
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=50, max_steps=300, polish_steps=0, tol_fp32=0.0,
+                     check_finite=1)
+    imp = _imp(ep.STATE_P, 0, (0.0, 0.0, 1.5), False)
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 5)
+        psi0 = ctx.psi()
+        ctx.set_impurity_state(imp)
+        r = ctx.solve(opts)[0]
+    o = O.Oracle(p)
+    o.set_psi(psi0)
+    ref = _imp(ep.STATE_P, -1, (0.0, 0.0, 1.5), False)
+    for i in range(20):
+        ref.invar[i] = imp.invar[i]
+    o.set_impurity_state(ref)
+    ro = o.solve(opts)
+    o.close()
+    assert r.steps == ro.steps == 300
+    for k in ("etot", "eimpu", "eHeX", "eso"):
+        a, b = getattr(r.e, k), getattr(ro.e, k)
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), (k, a, b)
