@@ -48,6 +48,14 @@ module effpot_c
     real(c_double) :: pg(16), pg_lo(16)
   end type
 
+  integer(c_int), parameter :: EFFPOT_STATE_P = 1, EFFPOT_STATE_D = 2
+  type, bind(c) :: effpot_state_imp
+    integer(c_int32_t) :: state, instate
+    real(c_double) :: als, rimp(3)
+    type(effpot_plus) :: v_pi, v_sigma, v_delta
+    real(c_double) :: invar(20), eigen(10), moments(31), eso
+  end type
+
   type, bind(c) :: effpot_result
     type(effpot_energies) :: e
     integer(c_int32_t) :: steps, converged
@@ -157,6 +165,13 @@ module effpot_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: slot
       type(effpot_plus), intent(in) :: plus
+    end function
+    ! P/D internal-state impurity: potenimpini + instates + potenimp
+    integer(c_int) function effpot_set_impurity_state(ctx, slot, imp) bind(c, name="effpot_set_impurity_state")
+      import :: c_int, c_ptr, effpot_state_imp
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      type(effpot_state_imp), intent(inout) :: imp
     end function
     ! readenc mode 3 (formatted or unformatted file); psi = c_null_ptr reads the header only
     integer(c_int) function effpot_snapshot_read(path, hdr, psi) bind(c, name="effpot_snapshot_read")

@@ -14,6 +14,10 @@ program effpot_driver
   character(len=3) :: core4 = 'OTC'
   character(len=80) :: selec_pi = 'Morse'
   character(len=80) :: selec_plus = 'Ba_plus_2D2'   ! X+-He potential of uplus (V_impur.f90:14-16)
+  ! P/D internal state (selec_pi other than 'Morse'/'null'): V_impur.f90:5-13, modules.f90:299-338
+  character(len=80) :: selec_sigma = 'Ba_plus_2D0', selec_delta = 'Ba_plus_2D2'
+  character(len=1) :: lstate = 'P'
+  integer :: instate = 0
   integer :: n4 = 300, mode = 0, nx = 64, ny = 64, nz = 64, npd = 13, icon = 13
   integer :: niter = 0, pener = 50, nthread = 1, precision = 0, ninvar = 10
   integer(c_int64_t) :: seed = 20261203_c_int64_t
@@ -25,6 +29,8 @@ program effpot_driver
   real(8) :: morse_d = 30.d0, morse_re = 4.d0, morse_a = 1.d0
   real(8) :: r_cutoff_pi = 0.d0, umax_pi = 0.d0, afermi = 0.6d0, rho0 = 0.0218d0, noise = 1.d-6
   real(8) :: r_cutoff_plus = 1.05d0, umax_plus = 6.260015d4
+  real(8) :: r_cutoff_sigma = 1.05d0, umax_sigma = 9.674836d4, r_cutoff_delta = 1.05d0, umax_delta = 6.260015d4
+  real(8) :: als_p = 0.d0, als_d = 0.d0
   real(8) :: tol = 1.d-9
   logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
   real(8), parameter :: ktops = 7.63822291d0
@@ -32,11 +38,13 @@ program effpot_driver
                    nthread, precision, seed, xmax, ymax, zmax, xc, yc, zc, eps4, sigma4, h_core, l, &
                    cp4, cpp4, den4c, alphas, den0s, h2o2m4, deltat, deltatps, ximp, yimp, zimp, &
                    morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar, &
-                   fast_io, selec_plus, r_cutoff_plus, umax_plus
+                   fast_io, selec_plus, r_cutoff_plus, umax_plus, selec_sigma, selec_delta, lstate, &
+                   instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
   type(effpot_plus) :: plus
+  type(effpot_state_imp) :: simp
   type(effpot_energies) :: e
   type(effpot_solve_opts) :: opt
   type(effpot_result) :: res
@@ -115,21 +123,28 @@ program effpot_driver
   end select
 
   imp%pad_ = 0
-  if (trim(selec_pi) == 'Morse') then
-    imp%kind = EFFPOT_POT_MORSE
-  else if (trim(selec_pi) == 'null') then
-    imp%kind = EFFPOT_POT_NULL
-  else
-    write(6, '(A,": aquest potencial no est definit, de part de: Selec_Pot")') trim(selec_pi)
-    stop 'From V_impur: 0001'
+  if (trim(selec_pi) == 'Morse' .or. trim(selec_pi) == 'null') then   ! isotropic impurity
+    imp%kind = merge(EFFPOT_POT_MORSE, EFFPOT_POT_NULL, trim(selec_pi) == 'Morse')
+    imp%d = morse_d; imp%re = morse_re; imp%a = morse_a
+    imp%r_cutoff = r_cutoff_pi; imp%umax = umax_pi; imp%cap = 1000.d0; imp%rimp = rimp
+    call check(ctx, effpot_set_impurity(ctx, 0_c_int, imp), 'effpot_set_impurity')
+  else   ! P or D internal state: potenimpini, instates (readenc.f90:176-221), potenimp
+    select case (lstate)
+    case ('P')
+      simp%state = EFFPOT_STATE_P; simp%als = als_p
+    case ('D')
+      simp%state = EFFPOT_STATE_D; simp%als = als_d
+    case default
+      stop 'Lstate must be P or D'
+    end select
+    if (instate < 0) stop 'instate less than 0, still not programmed'
+    simp%instate = instate; simp%rimp = rimp
+    call preset(selec_pi, r_cutoff_pi, umax_pi, simp%v_pi)
+    call preset(selec_sigma, r_cutoff_sigma, umax_sigma, simp%v_sigma)
+    call preset(selec_delta, r_cutoff_delta, umax_delta, simp%v_delta)
+    call check(ctx, effpot_set_impurity_state(ctx, 0_c_int, simp), 'effpot_set_impurity_state')
   end if
-  imp%d = morse_d; imp%re = morse_re; imp%a = morse_a
-  imp%r_cutoff = r_cutoff_pi; imp%umax = umax_pi; imp%cap = 1000.d0; imp%rimp = rimp
-  call check(ctx, effpot_set_impurity(ctx, 0_c_int, imp), 'effpot_set_impurity')
-  if (effpot_plus_preset(trim(selec_plus) // c_null_char, r_cutoff_plus, umax_plus, plus) /= EFFPOT_OK) then
-    write(6, '(A,": aquest potencial no est definit, de part de: Selec_Pot")') trim(selec_plus)
-    stop 'From V_impur: 0001'
-  end if
+  call preset(selec_plus, r_cutoff_plus, umax_plus, plus)
   call check(ctx, effpot_set_plus(ctx, 0_c_int, plus), 'effpot_set_plus')
 
   if (niter > 0) then
@@ -168,6 +183,16 @@ program effpot_driver
                T5,'TOTAL ENERGY (He+X) ..........: ',F18.6,' K',/)
 
 contains
+
+  subroutine preset(selec, r_cutoff, umax, v)   ! Select_Pot by name, or its stop
+    character(len=*), intent(in) :: selec
+    real(8), intent(in) :: r_cutoff, umax
+    type(effpot_plus), intent(out) :: v
+    if (effpot_plus_preset(trim(selec) // c_null_char, r_cutoff, umax, v) /= EFFPOT_OK) then
+      write(6, '(A,": aquest potencial no est definit, de part de: Selec_Pot")') trim(selec)
+      stop 'From V_impur: 0001'
+    end if
+  end subroutine preset
 
   integer function tstgrid(n)   ! n = 2^a 3^b 5^c <= 256 (tstgrid.f90:8-12)
     integer, intent(in) :: n

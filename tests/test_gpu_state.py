@@ -92,3 +92,35 @@ def test_state_relaxation_matches_oracle(gpu_available):
     for k in ("etot", "eimpu", "eHeX", "eso"):
         a, b = getattr(r.e, k), getattr(ro.e, k)
         assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), (k, a, b)
+
+
+def test_fortran_driver_p_state(gpu_available, tmp_path):
+    """The driver's Lstate/instate/selec_* keys reach the same P-state
+    impurity as the Python binding (Fermi start, one energy evaluation)."""
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "driver", "effpot_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    print("[state] driver P state", flush=True)
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    imp = ep.make_state_imp(ep.STATE_P, rimp=(0.0, 0.75, 1.5), instate=1, als=3.0)
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 20261203)
+        ctx.set_impurity_state(imp)
+        ctx.set_plus(ep.make_plus())
+        ctx.poten()
+        e = ctx.energy()[0]
+    nml = ("&input\n nx=32, ny=32, nz=32, xmax=12.d0, ymax=12.d0, zmax=12.d0, n4=100, mode=0,\n"
+           " core4='OTC', h_core=2.377d0, niter=0, ximp=0.d0, yimp=0.75d0, zimp=1.5d0,\n"
+           " lstate='P', instate=1, als_p=3.d0, selec_pi='Ba_plus_2D1', r_cutoff_pi=1.05d0,\n"
+           " umax_pi=5.216077d4\n/\n")
+    out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert out.returncode == 0, out.stderr
+
+    def val(label):
+        return float(re.search(label + r" \.*: +(\S+)", out.stdout).group(1))
+    assert abs(val(r"Impurity energy \(X\)") - e["eimpu"]) < 1e-5
+    assert abs(val(r"Spin-Orbit energy \(X\)") - e["eso"]) < 1e-5 and abs(e["eso"]) > 0.1
+    assert abs(val(r"TOTAL ENERGY \(He\+X\)") - e["etot"]) < 1e-5
