@@ -24,20 +24,24 @@ from collections import OrderedDict
 
 import math
 
-from . import (Context, make_params, make_morse, make_plus, read_snapshot, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO,
-               H4OT, PLUS_DEFAULT)
+from . import (Context, make_params, make_morse, make_plus, make_state_imp, read_snapshot, StateImp, STATE_P,
+               STATE_D, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO, H4OT, PLUS_DEFAULT)
 
 
 def _grid_key(h):
     return (h.nx, h.ny, h.nz, round(h.xmax, 12), round(h.ymax, 12), round(h.zmax, 12))
 
 
-def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch=8, rimp=None, plus="default"):
+def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch=8, rimp=None, plus="default",
+             state=None):
     """E_imp and the energy balance of each snapshot in `paths`.
 
     impurity: an effpot Impurity (its rimp is replaced by the snapshot's own
     rimp unless `rimp` is given).  plus: the X+ potential of uplus (an effpot
-    Plus, None for none, "default" for the reference's selec_plus).  Returns
+    Plus, None for none, "default" for the reference's selec_plus).  state: a
+    StateImp template for a P/D internal-state impurity instead of `impurity`
+    (instates runs on each snapshot's density, as readenc does for mode 3
+    with instate >= 0; its invar, eigenvalues and eso are returned).  Returns
     one dict per path, in order, with the energies (effpot_energies fields),
     the impurity position used and the X-He centre-of-mass distance.
     """
@@ -48,6 +52,7 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
     for i, h in enumerate(headers):
         groups.setdefault(_grid_key(h), []).append(i)
     out = [None] * len(paths)
+    extra = {}
     for key, idx in groups.items():
         h0 = headers[idx[0]]
         B = max(1, min(batch, len(idx)))
@@ -62,10 +67,15 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
                     h, psi = read_snapshot(paths[i])
                     ctx.upload_psi(psi, slot=s)
                     pos = tuple(rimp) if rimp is not None else tuple(h.rimp)
-                    imp = make_morse(rimp=pos, D=impurity.D, re=impurity.re, a=impurity.a,
-                                     r_cutoff=impurity.r_cutoff, umax=impurity.umax, cap=impurity.cap)
-                    imp.kind = impurity.kind
-                    ctx.set_impurity(imp, slot=s)
+                    if state is not None:
+                        st = StateImp.from_buffer_copy(state)
+                        st.rimp[:] = pos
+                        extra[i] = ctx.set_impurity_state(st, slot=s)
+                    else:
+                        imp = make_morse(rimp=pos, D=impurity.D, re=impurity.re, a=impurity.a,
+                                         r_cutoff=impurity.r_cutoff, umax=impurity.umax, cap=impurity.cap)
+                        imp.kind = impurity.kind
+                        ctx.set_impurity(imp, slot=s)
                     ctx.set_plus(plus, slot=s)
                 ctx.poten()
                 es = ctx.energy()
@@ -76,6 +86,10 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
                     # r_cm divides by the atom number (the context's n4 is 1 here)
                     for c in ("xcm", "ycm", "zcm"):
                         d[c] /= d["natoms"]
+                    if i in extra:
+                        n = 6 if extra[i].state == STATE_P else 10
+                        d["invar"] = extra[i].invar_c()
+                        d["eigen"] = list(extra[i].eigen)[:n]
                     d["dist"] = math.sqrt((d["rimp"][0] - d["xcm"]) ** 2 + (d["rimp"][1] - d["ycm"]) ** 2 +
                                           (d["rimp"][2] - d["zcm"]) ** 2)
                     out[i] = d
@@ -111,6 +125,10 @@ def main(argv=None):
     ap.add_argument("--selec-plus", default=PLUS_DEFAULT[0], help="X+-He potential of uplus (Select_Pot name)")
     ap.add_argument("--r-cutoff-plus", type=float, default=PLUS_DEFAULT[1])
     ap.add_argument("--umax-plus", type=float, default=PLUS_DEFAULT[2])
+    ap.add_argument("--lstate", choices=["P", "D"], default=None,
+                    help="P/D internal-state impurity (Ba+ fits of selec_pi/sigma/delta) instead of --morse")
+    ap.add_argument("--instate", type=int, default=0)
+    ap.add_argument("--als", type=float, default=0.0, help="spin-orbit splitting Als_P / Als_D (K)")
     ap.add_argument("--core", choices=["OT", "OTC"], default="OTC")
     ap.add_argument("--h-core", type=float, default=2.377)
     ap.add_argument("--batch", type=int, default=8)
@@ -118,8 +136,11 @@ def main(argv=None):
     a = ap.parse_args(argv)
     imp = make_morse(D=a.morse[0], re=a.morse[1], a=a.morse[2])
     plus = make_plus(a.selec_plus, a.r_cutoff_plus, a.umax_plus)
+    state = None
+    if a.lstate:
+        state = make_state_imp(STATE_P if a.lstate == "P" else STATE_D, instate=a.instate, als=a.als)
     res = evaluate(a.files, imp, core=CORE_OTC if a.core == "OTC" else CORE_OT,
-                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus)
+                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus, state=state)
     for d in res:
         if not a.quiet:
             print(f"# {d['path']}")

@@ -128,3 +128,42 @@ def test_oneshot_batch_matches_oracle_and_single(gpu_available, tmp_path):
         # r_cm over the snapshot's own atom number (the oracle divides by its n4 = 100)
         cm = np.array([e["xcm"], e["ycm"], e["zcm"]]) * 100.0 / e["natoms"]
         assert abs(many[k]["dist"] - np.linalg.norm(np.array(poss[k]) - cm)) <= 1e-9
+
+
+def test_oneshot_state_impurity(gpu_available, tmp_path):
+    """D-state impurity in the one-shot runner: instates on each snapshot's
+    density; batched results equal one-slot ones and the oracle's potenimp +
+    energy for the invar instates chose."""
+    from eff_pot_amd import oneshot
+    from oracle.oracle import Oracle
+    print("[oneshot] 3 snapshots, D state", flush=True)
+    p = ep.make_params(32, xmax=12.0, n4=100.0, core=ep.CORE_OTC, h_core=2.377)
+    paths, psis = [], []
+    with ep.Context(p) as c:
+        for k in range(3):
+            c.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 500 + k)
+            h = ep.Snapshot()
+            h.nx = h.ny = h.nz = 32
+            h.xmax = h.ymax = h.zmax = 12.0
+            h.hx = h.hy = h.hz = 0.75
+            h.rimp[:] = (0.75 * k, 0.0, -0.75)
+            path = str(tmp_path / f"d{k}.dat")
+            ep.write_snapshot(path, h, c.psi(), binary=True)
+            paths.append(path)
+            psis.append(c.psi())
+    st = ep.make_state_imp(ep.STATE_D, instate=1, als=2.0)
+    many = oneshot.evaluate(paths, None, batch=2, state=st)
+    one = oneshot.evaluate(paths[2:], None, batch=1, state=st)
+    assert many[2]["eimpu"] == one[0]["eimpu"] and many[2]["eso"] == one[0]["eso"]
+    for k in range(3):
+        imp = ep.make_state_imp(ep.STATE_D, rimp=many[k]["rimp"], instate=-1, als=2.0, invar=many[k]["invar"])
+        o = Oracle(p)
+        o.set_psi(psis[k])
+        o.set_impurity_state(imp)
+        o.set_plus(ep.make_plus())
+        o.poten()
+        e = o.energy()
+        o.close()
+        assert abs(many[k]["eso"] - e["eso"]) < 1e-12 * max(1.0, abs(e["eso"]))
+        for key in ("eimpu", "etot", "uplus"):
+            assert abs(many[k][key] - e[key]) <= 1e-10 * abs(e[key]), (k, key)
