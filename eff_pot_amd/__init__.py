@@ -71,6 +71,9 @@ def lib():
         L.effpot_snapshot_read.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p]
         L.effpot_snapshot_write.argtypes = [C.c_char_p, C.POINTER(Snapshot), C.c_void_p, C.c_int]
         L.effpot_set_plus.argtypes = [vp, C.c_int, C.POINTER(Plus)]
+        L.effpot_set_real_time.argtypes = [vp, C.c_int]
+        L.effpot_set_impurity_motion.argtypes = [vp, C.c_int, _dp, C.c_double]
+        L.effpot_get_impurity.argtypes = [vp, C.c_int, _dp, _dp, _dp]
         L.effpot_set_impurity_state.argtypes = [vp, C.c_int, C.POINTER(StateImp)]
         L.effpot_instates.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.effpot_plus_preset.argtypes = [C.c_char_p, C.c_double, C.c_double, C.POINTER(Plus)]
@@ -231,6 +234,20 @@ class Context:
 
     def set_impurity(self, imp: Impurity, slot=0):
         self._chk(lib().effpot_set_impurity(self.h, slot, C.byref(imp)))
+
+    def set_real_time(self, on=True):
+        """Real-time propagation for step_rk / step_pc (timec = i)."""
+        self._chk(lib().effpot_set_real_time(self.h, 1 if on else 0))
+
+    def set_impurity_motion(self, v, mass_u, slot=0):
+        """The slot's isotropic impurity moves (velocity v in A per K^-1, mass in u)."""
+        self._chk(lib().effpot_set_impurity_motion(self.h, slot, np.asarray(v, dtype=np.float64), mass_u))
+
+    def impurity(self, slot=0):
+        """(r, v, a) of the slot's impurity."""
+        r, v, a = np.zeros(3), np.zeros(3), np.zeros(3)
+        self._chk(lib().effpot_get_impurity(self.h, slot, r, v, a))
+        return r, v, a
 
     def set_impurity_state(self, imp: StateImp, slot=0) -> StateImp:
         """P/D-state impurity (instates on the slot's current psi when

@@ -24,6 +24,7 @@
 #include "misc.hpp"
 #include "stencil.hpp"
 #include "state.hpp"
+#include "motion.hpp"
 
 using namespace ep;
 
@@ -167,6 +168,12 @@ struct effpot_ctx {
   std::vector<int> have_imp;
   PlusSlot* d_plus = nullptr;     // [B] X+ potential of the uplus term
   std::vector<PlusSlot> h_plus;
+  std::vector<effpot_impurity> h_imp;  // [B] isotropic impurity of each slot
+  std::vector<int> h_is_state;    // [B] slot holds a P/D-state impurity
+  Motion* d_motion = nullptr;     // [B] moving impurities
+  std::vector<Motion> h_motion;
+  int any_motion = 0;             // some slot's impurity moves
+  int rt = 0;                     // real-time propagation
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
   double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
@@ -442,6 +449,7 @@ int launch_step_v(effpot_ctx* ctx, State<T>& s, State<P>& f, const int* act, Sta
   a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.arun = (T)c.arun; a.brun = (T)c.brun; a.crun = (T)c.crun; a.first_stage = c.first;
   a.write_den = ctx->corr_den ? 1 : 0;
+  a.rt = ctx->rt;
   Ring<T> rg = ring_of(ctx, s);
   dim3 g = sgrid(ctx);
 #define M_(TXX) hipLaunchKernelGGL((k_step<T, V, P, TXX, MODE>), g, dim3(kThreads), 0, ctx->st, a, rg)
@@ -624,6 +632,39 @@ template <typename T> int norm_t(effpot_ctx* ctx, State<T>& s, const int* act) {
   return reduce<2>(ctx, 1, act);
 }
 
+// The slot's impurity stops moving (a new impurity was set, or mass 0).
+int stop_motion(effpot_ctx* ctx, int slot) {
+  if (!ctx->h_motion[slot].on) return EFFPOT_OK;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->h_motion[slot] = Motion{};
+  HIPCHK(hipMemcpy(ctx->d_motion + slot, &ctx->h_motion[slot], sizeof(Motion), hipMemcpyHostToDevice));
+  ctx->any_motion = 0;
+  for (const Motion& q : ctx->h_motion) ctx->any_motion |= q.on;
+  return EFFPOT_OK;
+}
+
+// uimp of the moving slots at their current positions (FP64 fields).
+int move_field(effpot_ctx* ctx, const int* act) {
+  hipLaunchKernelGGL(k_imp_field_moving, dim3(256, ctx->B), dim3(256), 0, ctx->st, (const Motion*)ctx->d_motion, act,
+                     ctx->nx, ctx->ny, ctx->nz, -ctx->p.xmax, -ctx->p.ymax, -ctx->p.zmax, ctx->hx, ctx->hy, ctx->hz,
+                     ctx->npot, (double)(ctx->npot - 1) * ctx->delinter, (const double*)ctx->imptab, ctx->s64.uimp,
+                     ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+// forceimp (potenimp.f90:193-258) after poten: a = -sum grad(den) uimp dV / m.
+int force_t(effpot_ctx* ctx, const int* act) {
+  hipLaunchKernelGGL(k_force_partials, dim3(nblk(ctx), ctx->B), dim3(256), 0, ctx->st, (const Motion*)ctx->d_motion,
+                     act, (const double*)ctx->s64.dx, (const double*)ctx->s64.dy, (const double*)ctx->s64.dz,
+                     (const double*)ctx->s64.uimp, ctx->N, ctx->d_partial);
+  LAUNCHCHK();
+  RC(reduce<3>(ctx, 0, act));
+  hipLaunchKernelGGL(k_motion_force, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B,
+                     (const double*)ctx->d_red, ctx->dxyz);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
 // RK-Gill step (steprk.f90:286-405).  Leaves the norm reduction in d_red/d_scale.
 template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   const double r2 = std::sqrt(2.0);
@@ -636,7 +677,14 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
     RC(ring_op(ctx, s, j == 0 ? RING_BEFORE_RK0 : RING_BEFORE_RKJ, act));
     RC((launch_step<T, 2>(ctx, s, act, f32, c)));
     RC(ring_op(ctx, s, j == 0 ? RING_AFTER_RK0 : RING_AFTER_RKJ, act));
+    if (ctx->any_motion) {  // r, v stage; the field at the new position (steprk.f90:117-170)
+      hipLaunchKernelGGL(k_motion_rk, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B, j,
+                         (const double*)ctx->d_dt, c.arun, c.brun, c.crun);
+      LAUNCHCHK();
+      if (j <= 2) RC(move_field(ctx, act));
+    }
     if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
+    if (ctx->any_motion && j <= 2) RC(force_t(ctx, act));
   }
   RC(reduce<2>(ctx, 1, act));
   return EFFPOT_OK;
@@ -645,10 +693,22 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
 // Hamming predictor-modifier-corrector (steppc.f90:35-159).
 template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   RC((launch_step<T, 0>(ctx, s, act, f32)));
+  if (ctx->any_motion) {  // r, v predictor-modifier and the field there (steppc.f90:68-113)
+    hipLaunchKernelGGL(k_motion_pred, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B,
+                       (const double*)ctx->d_dt);
+    LAUNCHCHK();
+    RC(move_field(ctx, act));
+  }
   RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
+  if (ctx->any_motion) RC(force_t(ctx, act));
   RC((launch_step<T, 1>(ctx, s, act, f32)));  // roles as after RING_AFTER_PRED
   // norm sums, then both role rotations of the step in the same launch
   RC(reduce<2>(ctx, 1, act, s.ring, RING_AFTER_PRED, RING_AFTER_CORR));
+  if (ctx->any_motion) {
+    hipLaunchKernelGGL(k_motion_corr, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B,
+                       (const double*)ctx->d_dt);
+    LAUNCHCHK();
+  }
   return EFFPOT_OK;
 }
 
@@ -665,6 +725,14 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
   a.dx = s.dx; a.dy = s.dy; a.dz = s.dz; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz;
   a.uimp = s.uimp; a.pot4 = s.pot4; a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.psi_all = s.psi; a.ring = s.ring; a.plus = ctx->d_plus;
+  if (ctx->any_motion) {  // current positions for uplus, velocities for ekinx
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipMemcpy(ctx->h_motion.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
+    for (int b = 0; b < ctx->B; b++)
+      if (ctx->h_motion[b].on)
+        for (int c = 0; c < 3; c++) ctx->h_plus[b].rimp[c] = ctx->h_motion[b].r[c];
+    HIPCHK(hipMemcpyAsync(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  }
   dim3 g = sgrid(ctx);
 #define M_(TXX)                                                                                   \
   if (ctx->cplx) hipLaunchKernelGGL((k_energy<T, cx<T>, TXX>), g, dim3(kThreads), 0, ctx->st, a); \
@@ -693,6 +761,10 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
     e.eHeX = S[5] * dv;
     e.uplus = S[11] * dv;
     e.eso = ctx->h_eso[b];
+    if (ctx->any_motion && ctx->h_motion[b].on) {  // energy.f90:112
+      const Motion& q = ctx->h_motion[b];
+      e.ekinx = 0.5 * q.m * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
+    }
     e.eimpu = e.ekinx + e.eHeX + e.eso;
     e.etot = e.etot4 + e.eimpu;
     e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
@@ -772,6 +844,12 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_plus.assign(ctx->B, PlusSlot{});
   ctx->h_eso.assign(ctx->B, 0.0);
+  ctx->h_imp.assign(ctx->B, effpot_impurity{});
+  ctx->h_is_state.assign(ctx->B, 0);
+  if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->h_motion.assign(ctx->B, Motion{});
+  if (hipMemcpy(ctx->d_motion, ctx->h_motion.data(), sizeof(Motion) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(EFFPOT_ERR_DEVICE);
   if (hipMemcpy(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -1038,7 +1116,9 @@ int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->have_imp[slot] = 1;
   ctx->h_eso[slot] = 0.0;
-  return EFFPOT_OK;
+  ctx->h_imp[slot] = *imp;
+  ctx->h_is_state[slot] = 0;
+  return stop_motion(ctx, slot);
 }
 
 int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
@@ -1139,6 +1219,64 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->h_eso[slot] = imp->eso;
   ctx->have_imp[slot] = 1;
+  ctx->h_is_state[slot] = 1;
+  return stop_motion(ctx, slot);
+}
+
+int effpot_set_real_time(effpot_ctx* ctx, int on) {
+  RC(check_ctx(ctx));
+  if (on && (!ctx->cplx || ctx->mixed))
+    return ctx->fail(EFFPOT_ERR_ARG, "real time needs complex psi (psi_complex = 1) and FP64 precision");
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->rt = on ? 1 : 0;  // effpot_solve / effpot_tick (the captured graphs) refuse real time
+  return EFFPOT_OK;
+}
+
+int effpot_set_impurity_motion(effpot_ctx* ctx, int slot, const double* vimp, double mass_u) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  if (!(mass_u > 0)) return stop_motion(ctx, slot);
+  if (!vimp) return ctx->fail(EFFPOT_ERR_ARG, "null velocity");
+  if (ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "a moving impurity needs an FP64 context");
+  if (!ctx->have_imp[slot] || ctx->h_is_state[slot])
+    return ctx->fail(EFFPOT_ERR_STATE, "set an isotropic impurity (effpot_set_impurity) first");
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  Motion q{};
+  q.on = 1;
+  q.m = mass_u * 0.0207569277;  // mp_u, modules.f90:333
+  const effpot_impurity& im = ctx->h_imp[slot];
+  for (int c = 0; c < 3; c++) { q.r[c] = im.rimp[c]; q.v[c] = vimp[c]; }
+  for (int k = 0; k < 3; k++) { q.ir[k] = k; q.iv[k] = k; }
+  q.ia[0] = 0; q.ia[1] = 1;
+  q.ip = ImpParams{im.kind, im.D, im.re, im.a, im.r_cutoff, im.umax, im.cap, im.rimp[0], im.rimp[1], im.rimp[2]};
+  ctx->h_motion[slot] = q;
+  HIPCHK(hipMemcpy(ctx->d_motion + slot, &q, sizeof(Motion), hipMemcpyHostToDevice));
+  ctx->any_motion = 1;
+  // the initial acceleration: poten and forceimp (DFT4He3d.f90:437-453)
+  std::vector<int> one(ctx->B, 0);
+  one[slot] = 1;
+  HIPCHK(hipMemcpy(ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
+  RC(den_from_cur(ctx, ctx->d_mask_tmp, false));
+  RC(poten_t(ctx, ctx->s64, ctx->d_mask_tmp));
+  RC(force_t(ctx, ctx->d_mask_tmp));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_get_impurity(effpot_ctx* ctx, int slot, double* r, double* v, double* a) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  Motion q;
+  HIPCHK(hipMemcpy(&q, ctx->d_motion + slot, sizeof(Motion), hipMemcpyDeviceToHost));
+  for (int c = 0; c < 3; c++) {
+    if (r) r[c] = q.on ? q.r[c] : ctx->h_plus[slot].rimp[c];
+    if (v) v[c] = q.on ? q.v[c] : 0.0;
+    if (a) a[c] = q.on ? q.a[c] : 0.0;
+  }
   return EFFPOT_OK;
 }
 
@@ -1512,6 +1650,8 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
 int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots) {
   RC(check_ctx(ctx));
   if (!opt || !slots) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  if (ctx->rt || ctx->any_motion)
+    return ctx->fail(EFFPOT_ERR_STATE, "relaxation with real time or a moving impurity: step with effpot_step_rk/pc");
   if (!(opt->dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
   if (opt->pener < 1 || opt->max_steps < 1 || opt->n_rk < 0) return ctx->fail(EFFPOT_ERR_ARG, "bad solve options");
   hipSetDevice(ctx->dev);

@@ -196,6 +196,9 @@ template <typename T, typename V> EP_DEV V hpsi_of(V psi, const V (&d2)[3], T h2
   V a = h2o2m4 * lap - p4 * psi;
   return a - u * psi;
 }
+// Real time (timec = i, steppc.f90:53): the step integrates i * H psi.
+template <typename T> EP_DEV cx<T> times_i(cx<T> v) { return cx<T>{-v.y, v.x}; }
+template <typename T> EP_DEV T times_i(T v) { return v; }  // never used: real time needs complex psi
 template <typename T> EP_DEV T norm2(cx<T> v) { return v.x * v.x + v.y * v.y; }
 template <typename T> EP_DEV T norm2(T v) { return v * v; }
 template <typename T> EP_DEV T dotre(cx<T> a, cx<T> b) { return a.x * b.x + a.y * b.y; }
@@ -228,6 +231,7 @@ template <typename T, typename V, typename P> struct StepArgs {
   T arun, brun, crun;
   int first_stage;
   int write_den;  // corrector: also store the density (no renormalisation follows)
+  int rt;         // real time: H psi -> i H psi (complex psi only)
 };
 
 // Hamming predictor + modifier (steppc.f90:53-64).  out = o3's buffer.
@@ -248,6 +252,7 @@ EP_DEV void k_pc_predict_body(const StepArgs<T, V, P>& a) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      if (a.rt) hp = times_i(hp);
       V pred = o.o3 + c4o3dt * ((T(2) * hp - o.h1) + T(2) * o.h2);
       V pn = pred - c112 * o.pc;
       o3[i] = pn;  // modified psi goes to the retired psi_{n-3} buffer
@@ -284,6 +289,7 @@ EP_DEV void k_pc_correct_body(const StepArgs<T, V, P>& a) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      if (a.rt) hp = times_i(hp);
       V corr = T(0.125) * ((T(9) * o.o3 - o.o2) + dt3 * ((hp + T(2) * o.h1) - o.h2));
       V pcv = o.pc - corr;
       V pn = corr + c9 * pcv;
@@ -324,6 +330,7 @@ EP_DEV void k_rk_stage_body(const StepArgs<T, V, P>& a) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
+      if (a.rt) hp = times_i(hp);
       V a2 = a.arun * (hp - a.brun * o.q);
       q[i] = (o.q + T(3) * a2) - a.crun * hp;
       if (a.first_stage) h1[i] = hp;

@@ -272,6 +272,27 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp);
 int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
                     double* eso);
 
+/* ---- real-time propagation and a moving impurity ------------------------
+ * Real time (timec = i; steppc.f90:53, steprk.f90:304-305): effpot_step_rk /
+ * effpot_step_pc integrate d psi/dt = i (h2o2m4 lap - pot4 - uimp) psi with
+ * the same integrators and no renormalisation.  Needs psi_complex = 1 and
+ * EFFPOT_PREC_FP64; effpot_solve and effpot_tick refuse it (they relax). */
+int effpot_set_real_time(effpot_ctx* ctx, int on);
+
+/* The slot's isotropic impurity moves classically: r and v advance with the
+ * helium integrators (steprk.f90:117-150, steppc.f90:68-203) under forceimp,
+ * F = -sum grad(den) uimp dV (potenimp.f90:193-258), a = F / m with
+ * m = mass_u * mp_u (mp_u = 0.0207569277 K^-1 A^-2, modules.f90:333; v in
+ * A per K^-1 of time), and uimp follows r.  Sets the initial acceleration from
+ * the slot's current psi (DFT4He3d.f90:437-453).  mass_u <= 0 stops the
+ * motion; effpot_set_impurity / effpot_set_impurity_state also do.  Energies
+ * then hold ekinx = m |v|^2 / 2 (energy.f90:112) in eimpu.  FP64 contexts
+ * only; relaxation (effpot_solve / effpot_tick) refuses it. */
+int effpot_set_impurity_motion(effpot_ctx* ctx, int slot, const double* vimp, double mass_u);
+
+/* Current r (A), v and a of the slot's impurity (v = a = 0 if it does not move). */
+int effpot_get_impurity(effpot_ctx* ctx, int slot, double* r, double* v, double* a);
+
 /* X+-He potential of one slot for the uplus energy term, evaluated at the
  * slot's impurity position (energy.f90:146-163).  A context starts with no
  * X+ potential (uplus = 0); NULL removes it again. */

@@ -129,6 +129,11 @@ typedef struct oracle {
   effpot_impurity imp; int have_imp;
   effpot_plus plus; /* X+ potential of uplus (kind NULL: none) */
   double eso;       /* spin-orbit energy of a P/D-state impurity */
+  int rt;           /* real time: the step integrates i H psi */
+  /* moving impurity (classicimp): r = imp.rimp, v, a, histories, pc, q */
+  int moving; double mimp, vimp[3], aimp[3];
+  double rimpold[3][3], vimpold[3][3], aimpold[2][3], pcr[3], pcv[3], qr[3], qv[3];
+  int ioldr[3], ioldv[3], iolda[2];
   double *stab, *sf[3]; /* P/D state: potential tables [3][npot+2], updatepoten fields */
 } oracle_t;
 
@@ -516,7 +521,7 @@ double oracle_update(double r, int npot, double rmax, const double* f) {
 
 int oracle_set_impurity(oracle_t* o, const effpot_impurity* imp) {
   if (imp->kind != EFFPOT_POT_MORSE && imp->kind != EFFPOT_POT_NULL) return EFFPOT_ERR_POT;
-  o->imp = *imp; o->have_imp = 1; o->eso = 0.0;
+  o->imp = *imp; o->have_imp = 1; o->eso = 0.0; o->moving = 0;
   /* potenimpini.f90:35-58 (isotropic: V_pi = V, Delta = 0) */
   for (int i = 1; i <= o->npot; i++) o->vtab[i] = oracle_select_pot(imp, (double)(i - 1) * o->delinter);
   o->vtab[0] = o->vtab[1];
@@ -892,7 +897,9 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
     e->etot4 += e->ealphas;
   }
   e->eHeX = ehex * o->dxyz;
-  e->ekinx = 0.0; e->eso = o->eso;
+  e->ekinx = o->moving ? 0.5 * o->mimp * (o->vimp[0] * o->vimp[0] + o->vimp[1] * o->vimp[1] + o->vimp[2] * o->vimp[2])
+                       : 0.0; /* energy.f90:112 */
+  e->eso = o->eso;
   e->uplus = upl * o->dxyz;
   e->eimpu = e->ekinx + e->eHeX + e->eso;
   e->etot = e->etot4 + e->eimpu;
@@ -934,7 +941,64 @@ static void compute_hpsi(oracle_t* o, double* hp) {
       double lp = (d1[k] + d2[k]) + d3[k];
       hp[k] = (lp * h2 - o->pot4[i] * o->psi[k]) - o->uimp[i] * o->psi[k];
     }
+  if (o->rt) /* timec = i (steppc.f90:53): i*H psi */
+    for (size_t i = 0; i < N; i++) { double re = hp[2 * i]; hp[2 * i] = -hp[2 * i + 1]; hp[2 * i + 1] = re; }
   free(d3);
+}
+
+/* The impurity field at r from the table of oracle_set_impurity (updatepoten
+ * with Delta = 0, cap 1000 K or the impurity's cap). */
+static void imp_field_at(oracle_t* o, const double* rr) {
+  double rmax = (double)(o->npot - 1) * o->delinter;
+  for (int iz = 0; iz < o->nz; iz++) {
+    double zt = (o->z[iz] - rr[2]) * (o->z[iz] - rr[2]);
+    for (int iy = 0; iy < o->ny; iy++) {
+      double yt = (o->y[iy] - rr[1]) * (o->y[iy] - rr[1]) + zt;
+      for (int ix = 0; ix < o->nx; ix++) {
+        double r = sqrt((o->x[ix] - rr[0]) * (o->x[ix] - rr[0]) + yt);
+        double u = oracle_update(r, o->npot, rmax, o->vtab);
+        o->uimp[IDX(o, ix, iy, iz)] = u < o->imp.cap ? u : o->imp.cap;
+      }
+    }
+  }
+}
+
+/* forceimp (potenimp.f90:213-254 without the exciplex filter): a = F / m. */
+static void force_imp(oracle_t* o) {
+  double f[3] = {0.0, 0.0, 0.0};
+  for (size_t i = 0; i < o->N; i++) {
+    f[0] += o->dxden[i] * o->uimp[i];
+    f[1] += o->dyden[i] * o->uimp[i];
+    f[2] += o->dzden[i] * o->uimp[i];
+  }
+  for (int c = 0; c < 3; c++) o->aimp[c] = (-f[c] * o->dxyz) / o->mimp;
+}
+
+void oracle_set_real_time(oracle_t* o, int on) { o->rt = on; }
+
+/* The impurity moves with velocity v and mass mass_u (u); its acceleration
+ * from the current density (DFT4He3d.f90:437-453: poten, forceimp). */
+int oracle_set_impurity_motion(oracle_t* o, const double* v, double mass_u) {
+  if (!o->have_imp) return EFFPOT_ERR_STATE;
+  o->moving = mass_u > 0;
+  if (!o->moving) return EFFPOT_OK;
+  o->mimp = mass_u * 0.0207569277; /* mp_u, modules.f90:333 */
+  memset(o->rimpold, 0, sizeof(o->rimpold)); memset(o->vimpold, 0, sizeof(o->vimpold));
+  memset(o->aimpold, 0, sizeof(o->aimpold));
+  for (int c = 0; c < 3; c++) { o->vimp[c] = v[c]; o->pcr[c] = o->pcv[c] = o->qr[c] = o->qv[c] = 0.0; }
+  for (int k = 0; k < 3; k++) o->ioldr[k] = o->ioldv[k] = k;
+  o->iolda[0] = 0; o->iolda[1] = 1;
+  oracle_poten(o);
+  force_imp(o);
+  return EFFPOT_OK;
+}
+
+void oracle_get_impurity(const oracle_t* o, double* r, double* v, double* a) {
+  for (int c = 0; c < 3; c++) {
+    r[c] = o->imp.rimp[c];
+    v[c] = o->moving ? o->vimp[c] : 0.0;
+    a[c] = o->moving ? o->aimp[c] : 0.0;
+  }
 }
 
 static void update_den(oracle_t* o) {
@@ -973,10 +1037,32 @@ int oracle_step_rk(oracle_t* o, double dt) {
     }
     update_den(o);
     if (!den_ok(o)) return EFFPOT_ERR_NONFINITE;
+    if (o->moving) { /* steprk.f90:117-150 */
+      double* r = o->imp.rimp;
+      for (int c = 0; c < 3; c++) {
+        double a2 = arun[j] * (o->vimp[c] - brun[j] * o->qr[c]);
+        o->qr[c] = o->qr[c] + 3.0 * a2 - crun[j] * o->vimp[c];
+        if (j == 0) { o->rimpold[2][c] = o->rimpold[1][c]; o->rimpold[1][c] = o->rimpold[0][c]; o->rimpold[0][c] = r[c]; }
+        r[c] = r[c] + dt * a2;
+      }
+      for (int c = 0; c < 3; c++) {
+        double a2 = arun[j] * (o->aimp[c] - brun[j] * o->qv[c]);
+        o->qv[c] = o->qv[c] + 3.0 * a2 - crun[j] * o->aimp[c];
+        if (j == 0) {
+          o->aimpold[1][c] = o->aimpold[0][c]; o->aimpold[0][c] = o->aimp[c];
+          o->vimpold[2][c] = o->vimpold[1][c]; o->vimpold[1][c] = o->vimpold[0][c]; o->vimpold[0][c] = o->vimp[c];
+        }
+        o->vimp[c] = o->vimp[c] + dt * a2;
+      }
+      if (j <= 2) imp_field_at(o, r);
+    }
     if (j <= 2) oracle_poten(o);
+    if (o->moving && j <= 2) force_imp(o);
   }
   for (int i = 0; i < 3; i++) o->ioldp[i] = i;
   for (int i = 0; i < 2; i++) o->ioldh[i] = i;
+  for (int i = 0; i < 3; i++) o->ioldr[i] = o->ioldv[i] = i;
+  o->iolda[0] = 0; o->iolda[1] = 1;
   return EFFPOT_OK;
 }
 
@@ -998,7 +1084,27 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
   }
   update_den(o);
   { int t = o->ioldh[1]; o->ioldh[1] = o->ioldh[0]; o->ioldh[0] = t; }
+  const double c4o3m = 4.0 / 3.0;
+  if (o->moving) { /* steppc.f90:68-103 */
+    double* r = o->imp.rimp;
+    int* ir = o->ioldr; int* iv = o->ioldv; int* ia = o->iolda;
+    for (int c = 0; c < 3; c++) {
+      double aux = o->rimpold[ir[2]][c] + c4o3m * dt * (2.0 * o->vimp[c] - o->vimpold[iv[0]][c] + 2.0 * o->vimpold[iv[1]][c]);
+      o->rimpold[ir[2]][c] = r[c];
+      r[c] = aux - c112 * o->pcr[c];
+      o->pcr[c] = aux;
+    }
+    for (int c = 0; c < 3; c++) {
+      double aux = o->vimpold[iv[2]][c] + c4o3m * dt * (2.0 * o->aimp[c] - o->aimpold[ia[0]][c] + 2.0 * o->aimpold[ia[1]][c]);
+      o->vimpold[iv[2]][c] = aux - c112 * o->pcv[c];
+      o->pcv[c] = aux;
+    }
+    for (int c = 0; c < 3; c++) o->aimpold[ia[1]][c] = o->aimp[c];
+    { int t = ia[1]; ia[1] = ia[0]; ia[0] = t; }
+    imp_field_at(o, r); /* potenimp(rimp) */
+  }
   oracle_poten(o);
+  if (o->moving) force_imp(o);
   compute_hpsi(o, hp);
   p3 = o->psiold[o->ioldp[2]];
   double* p2 = o->psiold[o->ioldp[1]];
@@ -1016,6 +1122,25 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
   }
   update_den(o);
   { int t = o->ioldp[2]; o->ioldp[2] = o->ioldp[1]; o->ioldp[1] = o->ioldp[0]; o->ioldp[0] = t; }
+  if (o->moving) { /* steppc.f90:160-203 */
+    double* r = o->imp.rimp;
+    int* ir = o->ioldr; int* iv = o->ioldv; int* ia = o->iolda;
+    for (int c = 0; c < 3; c++) {
+      double aux = 0.125 * (9.0 * o->rimpold[ir[2]][c] - o->rimpold[ir[1]][c] +
+                            3.0 * dt * (o->vimpold[iv[2]][c] + 2.0 * o->vimp[c] - o->vimpold[iv[0]][c]));
+      o->pcr[c] = o->pcr[c] - aux;
+      r[c] = aux + c9 * o->pcr[c];
+    }
+    { int t = ir[2]; ir[2] = ir[1]; ir[1] = ir[0]; ir[0] = t; }
+    for (int c = 0; c < 3; c++) {
+      double aux = 0.125 * (9.0 * o->vimp[c] - o->vimpold[iv[1]][c] +
+                            3.0 * dt * (o->aimp[c] + 2.0 * o->aimpold[ia[0]][c] - o->aimpold[ia[1]][c]));
+      o->pcv[c] = o->pcv[c] - aux;
+      o->vimpold[iv[2]][c] = o->vimp[c];
+      o->vimp[c] = aux + c9 * o->pcv[c];
+    }
+    { int t = iv[2]; iv[2] = iv[1]; iv[1] = iv[0]; iv[0] = t; }
+  }
   if (err_he) *err_he = err / (double)o->N;
   if (!den_ok(o)) return EFFPOT_ERR_NONFINITE;
   return EFFPOT_OK;

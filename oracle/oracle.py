@@ -52,6 +52,9 @@ def lib():
         L.oracle_plus_pot.argtypes = [C.POINTER(Plus), C.c_double]
         L.oracle_plus_pot.restype = C.c_double
         L.oracle_set_plus.argtypes = [C.c_void_p, C.POINTER(Plus)]
+        L.oracle_set_real_time.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_set_impurity_motion.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.oracle_get_impurity.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.oracle_state_moments.argtypes = [C.c_void_p, C.POINTER(StateImp), _dp]
         L.oracle_state_matrix.argtypes = [C.c_int, _dp, C.c_double, _dp]
         L.oracle_state_eso.argtypes = [C.c_int, _dp, C.c_double]
@@ -208,6 +211,18 @@ class Oracle:
         rc = lib().oracle_set_impurity(self.h, C.byref(imp))
         if rc != OK:
             raise ValueError(f"oracle_set_impurity failed: {rc}")
+
+    def set_real_time(self, on=True):
+        lib().oracle_set_real_time(self.h, 1 if on else 0)
+
+    def set_impurity_motion(self, v, mass_u):
+        rc = lib().oracle_set_impurity_motion(self.h, np.asarray(v, dtype=np.float64), mass_u)
+        assert rc == OK
+
+    def impurity(self):
+        r, v, a = np.zeros(3), np.zeros(3), np.zeros(3)
+        lib().oracle_get_impurity(self.h, r, v, a)
+        return r, v, a
 
     def state_moments(self, imp: StateImp):
         m = np.zeros(31)
