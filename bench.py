@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--no-ref", action="store_true",
                     help="skip the plain FP64 solve of slot 0's input that gives |dE_imp| and the speed-up")
     ap.add_argument("--tile", type=int, nargs=2, default=None, help="stencil tile (TX, z chunk) instead of autotuning")
-    ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256", "corrector", "poten"],
+    ap.add_argument("--workload", choices=["solve128", "cfg0", "scan", "serve", "stress256", "corrector", "poten",
+                                           "oneshot", "realtime"],
                     default="solve128",
                     help="solve128 (default, the headline); the others are the batched scan, the serving stream, "
                          "the 256^3 stress test and the small CPU-runnable case")
@@ -295,6 +296,132 @@ def run_stress(a):
                        1.0, "f64 / fp32 fields", "He4 N=5000, 256^3 h=0.4 A", **out)
 
 
+def run_oneshot(a):
+    """Impurity energies of stored snapshots (the eff_pot one-shot job): 32
+    snapshots of the 128^3 N=1000 droplet, evaluated 8 at a time; the rate with
+    the unformatted file reads, the formatted (text) read rate, the GPU part
+    alone, and the oracle's poten + energy of one snapshot for the CPU."""
+    import tempfile
+    import eff_pot_amd as ep
+    from eff_pot_amd import oneshot
+    K, B = 32, 8
+    p, R = workload(128, "fp64", 1)
+    d = tempfile.mkdtemp(prefix="effpot_oneshot_")
+    paths, txt = [], []
+    with ep.Context(p) as c:
+        for k in range(K):
+            c.init_fermi(0.0218, R, 0.6, 1e-6, 100 + k)
+            h = ep.Snapshot()
+            h.nx = h.ny = h.nz = 128
+            h.xmax = h.ymax = h.zmax = p.xmax
+            h.hx = h.hy = h.hz = 2 * p.xmax / 128
+            h.rimp[:] = (0.0, 0.0, 0.5 * R * k / K)
+            path = os.path.join(d, f"den{k}.bin")
+            ep.write_snapshot(path, h, c.psi(), binary=True)
+            paths.append(path)
+            if k < 2:
+                t = os.path.join(d, f"den{k}.txt")
+                ep.write_snapshot(t, h, c.psi(), binary=False)
+                txt.append(t)
+    imp = ep.make_morse()
+    oneshot.evaluate(paths[:B], imp, batch=B)  # warm-up
+    t0 = time.perf_counter()
+    res = oneshot.evaluate(paths, imp, batch=B)
+    el = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for t in txt:
+        ep.read_snapshot(t)
+    el_txt = (time.perf_counter() - t1) / len(txt)
+    # the GPU part alone: B snapshots already resident
+    pb = ep.make_params(128, xmax=p.xmax, n4=1.0, core=ep.CORE_OTC, h_core=2.377, batch=B, psi_complex=1)
+    with ep.Context(pb) as c:
+        for s_ in range(B):
+            c.upload_psi(ep.read_snapshot(paths[s_])[1], slot=s_)
+        plus = ep.make_plus()
+        c.sync()
+        t2 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            for s_ in range(B):
+                c.set_impurity(ep.make_morse(rimp=(0.0, 0.0, 0.1 * s_)), slot=s_)
+                c.set_plus(plus, slot=s_)
+            c.poten()
+            c.energy()
+        c.sync()
+        el_gpu = (time.perf_counter() - t2) / (reps * B)
+    cpu = None
+    if not a.no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(p)
+        o.set_psi(ep.read_snapshot(paths[0])[1])
+        t3 = time.perf_counter()
+        o.set_impurity(imp)
+        o.set_plus(ep.make_plus())
+        o.poten()
+        o.energy()
+        el_cpu = time.perf_counter() - t3
+        o.close()
+        cpu = {"value": 1.0 / el_cpu, "unit": "snapshots/s", "cores": int(os.environ.get("OMP_NUM_THREADS", "1")),
+               "kind": "port", "sample": "potenimp + poten + energy of one 128^3 snapshot (resident)"}
+    for f_ in paths + txt:
+        os.remove(f_)
+    os.rmdir(d)
+    line = common_line(a, "snapshot impurity energies/s (128^3)", K / el, "snapshots/s", el, "f64",
+                       "He4 N=1000 snapshots, 128^3 h=0.5 A, Morse, Ba_plus_2D2 uplus; 32 unformatted files, "
+                       "batch 8", text_read_s_per_file=el_txt, gpu_only_ms_per_snapshot=1000 * el_gpu,
+                       E_imp_K_first=res[0]["eimpu"])
+    line["cpu_baseline"] = cpu
+    return line
+
+
+def run_realtime(a):
+    """Real-time PC steps of the 128^3 N=1000 droplet (complex psi, FP64) with a
+    moving 40 u Morse impurity, against imaginary-time steps of the same state;
+    plus the set-up of a D-state impurity (instates + uimp) at 128^3."""
+    import eff_pot_amd as ep
+    p, R = workload(128, "fp64", 1)
+    p.psi_complex = 1
+    nstep = max(a.steps, 50)
+    with ep.Context(p) as c:
+        c.init_fermi(0.0218, R, 0.6, 1e-6, 7)
+        c.set_impurity(ep.make_morse(rimp=(0.0, 0.0, 0.5 * R)))
+        c.poten()
+        dt = ep.default_dt(p)
+        for _ in range(3):
+            c.step_rk(dt)
+            c.renormalize()
+        for _ in range(5):
+            c.step_pc(dt)
+            c.renormalize()
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(nstep):
+            c.step_pc(dt)
+        c.sync()
+        el_it = (time.perf_counter() - t0) / nstep
+        c.set_real_time(True)
+        c.set_impurity_motion([0.0, 0.0, -2.0], 40.0)
+        for _ in range(3):
+            c.step_rk(0.5 * dt)
+        c.sync()
+        t1 = time.perf_counter()
+        for _ in range(nstep):
+            c.step_pc(0.5 * dt)
+        c.sync()
+        el_rt = (time.perf_counter() - t1) / nstep
+        r, v, acc = c.impurity()
+        c.set_real_time(False)
+        t2 = time.perf_counter()
+        st = c.set_impurity_state(ep.make_state_imp(ep.STATE_D, rimp=(0.0, 0.0, 0.5 * R), instate=0, als=2.0))
+        c.sync()
+        el_state = time.perf_counter() - t2
+    return common_line(a, "real-time PC steps/s (128^3, moving impurity)", 1.0 / el_rt, "steps/s", el_rt * nstep,
+                       "f64 complex", "He4 N=1000, 128^3 h=0.5 A, OTC, complex psi, 40 u Morse impurity moving",
+                       ms_per_rt_step=1000 * el_rt, ms_per_imag_step_complex=1000 * el_it,
+                       d_state_setup_ms=1000 * el_state, impurity_r=list(r), impurity_v=list(v),
+                       d_state_eigen=list(st.eigen))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -305,7 +432,8 @@ def main():
     import eff_pot_amd as ep
     if a.workload != "solve128":
         fn = {"cfg0": run_cfg0, "scan": run_scan, "serve": run_serve, "stress256": run_stress,
-              "corrector": run_corrector, "poten": run_poten}[a.workload]
+              "corrector": run_corrector, "poten": run_poten, "oneshot": run_oneshot,
+              "realtime": run_realtime}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 
