@@ -174,6 +174,7 @@ struct effpot_ctx {
   std::vector<Motion> h_motion;
   int any_motion = 0;             // some slot's impurity moves
   int rt = 0;                     // real-time propagation
+  int plane = 1;                  // forward x and y FFT passes fused per z plane where the plane fits in LDS
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
   double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
@@ -363,6 +364,40 @@ template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* 
   LAUNCHCHK();
   return EFFPOT_OK;
 }
+// The fused forward x-y plane transform applies to square planes that fit in LDS.
+template <typename T> bool plane_ok(effpot_ctx* ctx) {
+  if (!ctx->plane || ctx->nx != ctx->ny) return false;
+  // FP64 planes above 64^2 leave one wave per SIMD and measured slower than two passes
+  if (sizeof(T) == 8 && ctx->nx > 64) return false;
+  switch (ctx->nx) {
+    case 16: return PlaneFits<T, 16>::value; case 24: return PlaneFits<T, 24>::value;
+    case 32: return PlaneFits<T, 32>::value; case 48: return PlaneFits<T, 48>::value;
+    case 64: return PlaneFits<T, 64>::value; case 96: return PlaneFits<T, 96>::value;
+    case 128: return PlaneFits<T, 128>::value; case 192: return PlaneFits<T, 192>::value;
+    case 256: return PlaneFits<T, 256>::value;
+  }
+  return false;
+}
+template <typename T>
+PlaneArgs<T> plane_args(effpot_ctx* ctx, State<T>& s, T* const* r, cx<T>* const* c, int nf, const int* act) {
+  PlaneArgs<T> a{};
+  for (int f = 0; f < nf; f++) { a.r[f] = r[f]; a.c[f] = c[f]; }
+  a.rslot = ctx->N; a.cslot = ctx->M; a.pitch = ctx->nxp;
+  a.twx = s.tw[0]; a.twy = s.tw[1]; a.active = act;
+  return a;
+}
+// real fields -> half spectra transformed along x and y (one launch per plane)
+template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf,
+                                        const int* act) {
+  PlaneArgs<T> a = plane_args(ctx, s, in, out, nf, act);
+  dim3 g(ctx->nz, nf, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c<T, NN>), g, dim3(kThreads * PlaneFits<T, NN>::G), 0, ctx->st, a)
+  FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
 // y lines: (kx, z) runs of nxp, element stride nxp; z lines: (kx, ky) runs of
 // nxp, element stride nxp*ny.  Padding lines (kx >= nxh) are skipped.
 LineMap ymap(effpot_ctx* ctx) {
@@ -517,16 +552,18 @@ template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring
 // ---- poten (poten.f90:5-131 + term_alfa.f90:5-109) ----------------------------
 // `only` >= 0 runs just launch number `only` of the sequence (benchmark hook:
 // 0-4 stage A x r2c, y, z convolution, y, x c2r; 5 k_grad; 6-10 stage B;
-// 11 k_div; 12-16 stage C; 17 k_pot4).
+// 11 k_div; 12-16 stage C; 17 k_pot4; with the fused plane transforms 0/6/12
+// are the x+y forward transforms and 1/7/13 are empty).
 template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1) {
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
+  const bool pl = plane_ok<T>(ctx);  // forward x and y passes fused: launches 1, 7, 13 are empty
   auto sel = [only](int k) { return only < 0 || only == k; };
   // stage A: FFT(den) -> dencg, delj4 [, denalf]
   {
     T* in[1] = {s.den};
     cx<T>* sp[1] = {s.spec[0]};
-    if (sel(0)) RC(fft_x_r2c(ctx, s, in, sp, 1, act));
-    if (sel(1)) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    if (sel(0)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act) : fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(1) && !pl) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[0];
     ca.nout[0] = otc ? 3 : 2;
@@ -561,8 +598,8 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     const int nf = otc ? 4 : 1;
     T* in[4] = {s.src[0], s.src[1], s.src[2], s.src[3]};
     cx<T>* sp[4] = {s.spec[4], s.spec[5], s.spec[6], s.spec[7]};
-    if (sel(6)) RC(fft_x_r2c(ctx, s, in, sp, nf, act));
-    if (sel(7)) RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
+    if (sel(6)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, nf, act) : fft_x_r2c(ctx, s, in, sp, nf, act));
+    if (sel(7) && !pl) RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
     ConvArgs<T> ca{};
     for (int g = 0; g < nf; g++) {
       ca.in[g] = sp[g];
@@ -591,8 +628,8 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     // stage C: FFT(sum_i d_i den int_i) * kalfs
     T* in[1] = {s.src[0]};
     cx<T>* sp[1] = {s.spec[4]};
-    if (sel(12)) RC(fft_x_r2c(ctx, s, in, sp, 1, act));
-    if (sel(13)) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
+    if (sel(12)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act) : fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(13) && !pl) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[4]; ca.nout[0] = 1; ca.out[0][0] = s.spec[0]; ca.kern[0][0] = s.kern[2];
     if (sel(14)) RC(fft_zconv(ctx, s, ca, 1, act));
@@ -844,6 +881,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_plus.assign(ctx->B, PlusSlot{});
   ctx->h_eso.assign(ctx->B, 0.0);
+  if (const char* e = std::getenv("EFFPOT_PLANE")) ctx->plane = std::atoi(e) != 0;
   ctx->h_imp.assign(ctx->B, effpot_impurity{});
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -1388,8 +1426,12 @@ int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out) {
   HIPCHK(hipMemcpyAsync(s.src[0], in, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->st));
   double* r[1] = {s.src[0]};
   cx<double>* c[1] = {s.spec[0]};
-  RC(fft_x_r2c(ctx, s, r, c, 1, ctx->d_mask_tmp));
-  RC(fft_strided(ctx, s, 1, c, 1, -1, ctx->d_mask_tmp));
+  if (plane_ok<double>(ctx)) {
+    RC(fft_plane_fwd(ctx, s, r, c, 1, ctx->d_mask_tmp));
+  } else {
+    RC(fft_x_r2c(ctx, s, r, c, 1, ctx->d_mask_tmp));
+    RC(fft_strided(ctx, s, 1, c, 1, -1, ctx->d_mask_tmp));
+  }
   RC(fft_strided(ctx, s, 2, c, 1, -1, ctx->d_mask_tmp));
   HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<double>) * ctx->nxh, s.spec[0], sizeof(cx<double>) * ctx->nxp,
                           sizeof(cx<double>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyDeviceToHost, ctx->st));

@@ -478,4 +478,156 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
   }
 }
 
+// ---- forward x and y transforms of one z plane in one work-group ------------
+// The half spectrum of a whole (nx = ny = N) plane stays in LDS between the x
+// and the y transform, so the forward y pass costs no HBM round trip: real
+// rows -> x r2c (pairs of rows as in k_fft_x_r2c) -> plane -> y FFT of the NH
+// columns -> half spectrum in global memory.  (The same fusion of the inverse
+// y and x c2r passes measured slower than the two separate passes.)
+// The work-group is G sub-groups of kThreads threads, each with its own
+// scratch, working on different row-pair or column batches at the same time
+// (more waves in flight per CU than one 256-thread group with an 80 KB plane).
+// grid: (nz, nfields, nslots).  Used where the plane and the G scratches fit
+// in LDS (PlaneFits).
+template <typename T, int N> struct PlaneFits {
+  static constexpr int C = kThreads / Shape<N>::N1;
+  static constexpr int G = sizeof(T) == 4 ? 4 : 1;
+  static constexpr long bytes = ((long)N * (N / 2 + 1) + (long)G * N * (C + 1)) * (long)sizeof(cx<T>);
+  static constexpr bool value = bytes <= 160 * 1024;
+};
+template <typename T> struct PlaneArgs {
+  T* r[4];
+  cx<T>* c[4];
+  long rslot, cslot;
+  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
+  const cx<T>* twx;
+  const cx<T>* twy;
+  const int* active;
+};
+
+// fft_step1 / fft_step3 for sub-group thread t (0 <= t < kThreads).
+template <typename T, int N, class Load>
+EP_DEV void sub_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load, int t, bool on) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  if (on && t < C * N2) {
+    const int c = t % C, n2 = t / C;
+    cx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = load(c, N2 * n1 + n2);
+    RegFFT<T, N1>::run(a, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> v = a[k1];
+      if (k1 > 0 && n2 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
+      lds[(k1 * N2 + n2) * C + c] = v;
+    }
+  }
+}
+template <typename T, int N>
+EP_DEV void sub_step3(const cx<T>* lds, cx<T> (&b)[Shape<N>::N2], int sign, int t) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  const int c = t % C, k1 = t / C;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; n2++) b[n2] = lds[(k1 * N2 + n2) * C + c];
+  RegFFT<T, N2>::run(b, sign);
+}
+// x transform of C row pairs staged as lds[i*P + c]; leaves X[k] in lds[k*P + c].
+template <typename T, int N>
+EP_DEV void plane_x_fft(cx<T>* lds, const cx<T>* twN, int sign, int t, bool on) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1, P = C + 1;
+  cx<T> a[N1];
+  const bool s1 = on && t < C * N2;
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = lds[(N2 * n1 + n2) * P + c];
+  }
+  __syncthreads();
+  if (s1) {
+    const int c = t % C, n2 = t / C;
+    RegFFT<T, N1>::run(a, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> v = a[k1];
+      if (k1 > 0 && n2 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
+      lds[(k1 * N2 + n2) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  cx<T> b[N2];
+  if (on) sub_step3<T, N>(lds, b, sign, t);
+  __syncthreads();
+  if (on) {
+    const int c = t % C, k1 = t / C;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) lds[(k1 + N1 * k2) * P + c] = b[k2];
+  }
+  __syncthreads();
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((kThreads * PlaneFits<T, N>::G)) void k_fft_plane_r2c(PlaneArgs<T> a) {
+  constexpr bool fits = PlaneFits<T, N>::value;
+  constexpr int G = PlaneFits<T, N>::G;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1, NH = N / 2 + 1, P = C + 1;
+  if (!a.active[blockIdx.z]) return;
+  __shared__ cx<T> plane[fits ? N * NH : 1];
+  __shared__ cx<T> scratch[fits ? G * N * P : 1];
+  if constexpr (fits) {
+    const int f = blockIdx.y, iz = blockIdx.x, grp = threadIdx.x / kThreads, t = threadIdx.x % kThreads;
+    cx<T>* lds = scratch + grp * N * P;
+    const T* src = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
+    cx<T>* dst = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    constexpr int NB = (N / 2 + C - 1) / C;  // row-pair batches
+    for (int j = 0; j < (NB + G - 1) / G; j++) {  // x: pairs of rows
+      const int p0 = (j * G + grp) * C;
+      const bool on = p0 < N / 2;
+      if (on)
+        for (int e = t; e < C * N; e += kThreads) {
+          const int c = e / N, i = e - c * N, pr = p0 + c;
+          cx<T> v{T(0), T(0)};
+          if (pr < N / 2) {
+            const T* ra = src + (long)(2 * pr) * N;
+            v = cx<T>{ra[i], ra[N + i]};
+          }
+          lds[i * P + c] = v;
+        }
+      __syncthreads();
+      plane_x_fft<T, N>(lds, a.twx, -1, t, on);
+      if (on)
+        for (int e = t; e < C * NH; e += kThreads) {
+          const int c = e / NH, k = e - c * NH, pr = p0 + c;
+          if (pr >= N / 2) continue;
+          const cx<T> z = lds[k * P + c], zm = lds[((N - k) % N) * P + c];
+          const cx<T> sm = cx<T>{z.x + zm.x, z.y - zm.y};  // z + conj(zm)
+          const cx<T> df = cx<T>{z.x - zm.x, z.y + zm.y};  // z - conj(zm)
+          plane[(2 * pr) * NH + k] = cx<T>{T(0.5) * sm.x, T(0.5) * sm.y};
+          plane[(2 * pr + 1) * NH + k] = cx<T>{T(0.5) * df.y, T(-0.5) * df.x};
+        }
+      __syncthreads();
+    }
+    constexpr int NK = (NH + C - 1) / C;  // column batches
+    for (int j = 0; j < (NK + G - 1) / G; j++) {  // y: columns of the half spectrum
+      const int k0 = (j * G + grp) * C;
+      const bool on = k0 < NH;
+      auto load = [&](int c, int i) -> cx<T> {
+        const int k = k0 + c;
+        return k < NH ? plane[i * NH + k] : cx<T>{T(0), T(0)};
+      };
+      sub_step1<T, N>(lds, a.twy, -1, load, t, on);
+      __syncthreads();
+      if (on) {
+        cx<T> b[N2];
+        sub_step3<T, N>(lds, b, -1, t);
+        const int c = t % C, k1 = t / C, k = k0 + c;
+        if (k < NH) {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) dst[k + (long)(k1 + N1 * k2) * a.pitch] = b[k2];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace ep
