@@ -2,7 +2,7 @@
 FP64, and continuous batching (a request solved alone and in a mix).
 
 Tolerances: FP64 trajectories 1e-8 relative after 2000 steps; converged
-E_imp within 1e-3 K of the FP64 oracle for both the FP64
+E_imp within 1e-3 K of the FP64 oracle (the accuracy the mixed form must keep) for both the FP64
 and the mixed FP32/FP64 path; a slot's result is independent of the others
 (1e-12 relative psi, 1e-9 K energies)."""
 import json
