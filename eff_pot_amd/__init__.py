@@ -74,6 +74,8 @@ def lib():
         L.effpot_set_real_time.argtypes = [vp, C.c_int]
         L.effpot_set_impurity_motion.argtypes = [vp, C.c_int, _dp, C.c_double]
         L.effpot_get_impurity.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        L.effpot_set_state_dynamics.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.effpot_get_invar.argtypes = [vp, C.c_int, _dp]
         L.effpot_set_impurity_state.argtypes = [vp, C.c_int, C.POINTER(StateImp)]
         L.effpot_instates.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.effpot_plus_preset.argtypes = [C.c_char_p, C.c_double, C.c_double, C.POINTER(Plus)]
@@ -248,6 +250,15 @@ class Context:
         r, v, a = np.zeros(3), np.zeros(3), np.zeros(3)
         self._chk(lib().effpot_get_impurity(self.h, slot, r, v, a))
         return r, v, a
+
+    def set_state_dynamics(self, on=True, ev0=0.0, slot=0):
+        """The slot's P/D invar evolves in step_rk / step_pc (potinvar)."""
+        self._chk(lib().effpot_set_state_dynamics(self.h, slot, 1 if on else 0, ev0))
+
+    def invar(self, slot=0):
+        v = np.zeros(20)
+        self._chk(lib().effpot_get_invar(self.h, slot, v))
+        return v[0::2] + 1j * v[1::2]
 
     def set_impurity_state(self, imp: StateImp, slot=0) -> StateImp:
         """P/D-state impurity (instates on the slot's current psi when

@@ -174,6 +174,11 @@ struct effpot_ctx {
   std::vector<Motion> h_motion;
   int any_motion = 0;             // some slot's impurity moves
   int rt = 0;                     // real-time propagation
+  InvarDyn* d_dyn = nullptr;      // [B] P/D-state invar dynamics
+  std::vector<InvarDyn> h_dyn;
+  std::vector<StateArgs> h_sargs; // [B] the P/D-state set-up of each slot
+  int any_dyn = 0;
+  double* d_dynpart = nullptr;    // [B][kStateBlocks][kMomD] moment partials
   int plane = 1;                  // forward x and y FFT passes fused per z plane where the plane fits in LDS
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
@@ -680,12 +685,43 @@ int stop_motion(effpot_ctx* ctx, int slot) {
   return EFFPOT_OK;
 }
 
+// The slot's invar stops evolving (a new impurity was set, or switched off).
+int stop_dyn(effpot_ctx* ctx, int slot) {
+  if (!ctx->h_dyn[slot].on) return EFFPOT_OK;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const double als = ctx->h_dyn[slot].als;
+  ctx->h_dyn[slot] = InvarDyn{};
+  ctx->h_dyn[slot].als = als;
+  HIPCHK(hipMemcpy(ctx->d_dyn + slot, &ctx->h_dyn[slot], sizeof(InvarDyn), hipMemcpyHostToDevice));
+  ctx->any_dyn = 0;
+  for (const InvarDyn& q : ctx->h_dyn) ctx->any_dyn |= q.on;
+  return EFFPOT_OK;
+}
+
 // uimp of the moving slots at their current positions (FP64 fields).
 int move_field(effpot_ctx* ctx, const int* act) {
   hipLaunchKernelGGL(k_imp_field_moving, dim3(256, ctx->B), dim3(256), 0, ctx->st, (const Motion*)ctx->d_motion, act,
                      ctx->nx, ctx->ny, ctx->nz, -ctx->p.xmax, -ctx->p.ymax, -ctx->p.zmax, ctx->hx, ctx->hy, ctx->hz,
                      ctx->npot, (double)(ctx->npot - 1) * ctx->delinter, (const double*)ctx->imptab, ctx->s64.uimp,
                      ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+// potenimp of the slots whose invar evolves, with their current invar.
+int dyn_field(effpot_ctx* ctx, const int* act) {
+  hipLaunchKernelGGL(k_dyn_uimp, dim3(256, ctx->B), dim3(256), 0, ctx->st, (const InvarDyn*)ctx->d_dyn, act,
+                     ctx->s64.uimp, ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+// potinvar after poten: Hinvar from the moments of the current density.
+int dyn_hiv(effpot_ctx* ctx, const int* act) {
+  if (!ctx->d_dynpart) RC(dalloc(ctx, &ctx->d_dynpart, (size_t)ctx->B * kStateBlocks * kMomD));
+  hipLaunchKernelGGL(k_dyn_moments, dim3(kStateBlocks, ctx->B), dim3(256), 0, ctx->st, (const InvarDyn*)ctx->d_dyn,
+                     act, (const double*)ctx->s64.den, ctx->N, ctx->d_dynpart);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_dyn_hiv, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_dyn, act, ctx->B,
+                     (const double*)ctx->d_dynpart, ctx->dxyz);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -720,8 +756,15 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
       LAUNCHCHK();
       if (j <= 2) RC(move_field(ctx, act));
     }
+    if (ctx->any_dyn) {  // invar stage; potenimp with the new invar (steprk.f90:152-170)
+      hipLaunchKernelGGL(k_dyn_rk, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_dyn, act, ctx->B, j,
+                         (const double*)ctx->d_dt, c.arun, c.brun, c.crun);
+      LAUNCHCHK();
+      if (j <= 2) RC(dyn_field(ctx, act));
+    }
     if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
     if (ctx->any_motion && j <= 2) RC(force_t(ctx, act));
+    if (ctx->any_dyn && j <= 2) RC(dyn_hiv(ctx, act));
   }
   RC(reduce<2>(ctx, 1, act));
   return EFFPOT_OK;
@@ -736,13 +779,25 @@ template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act
     LAUNCHCHK();
     RC(move_field(ctx, act));
   }
+  if (ctx->any_dyn) {  // invar predictor-modifier and potenimp with it (steppc.f90:84-112)
+    hipLaunchKernelGGL(k_dyn_pred, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_dyn, act, ctx->B,
+                       (const double*)ctx->d_dt);
+    LAUNCHCHK();
+    RC(dyn_field(ctx, act));
+  }
   RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
   if (ctx->any_motion) RC(force_t(ctx, act));
+  if (ctx->any_dyn) RC(dyn_hiv(ctx, act));  // potinvar
   RC((launch_step<T, 1>(ctx, s, act, f32)));  // roles as after RING_AFTER_PRED
   // norm sums, then both role rotations of the step in the same launch
   RC(reduce<2>(ctx, 1, act, s.ring, RING_AFTER_PRED, RING_AFTER_CORR));
   if (ctx->any_motion) {
     hipLaunchKernelGGL(k_motion_corr, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B,
+                       (const double*)ctx->d_dt);
+    LAUNCHCHK();
+  }
+  if (ctx->any_dyn) {
+    hipLaunchKernelGGL(k_dyn_corr, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_dyn, act, ctx->B,
                        (const double*)ctx->d_dt);
     LAUNCHCHK();
   }
@@ -762,6 +817,10 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
   a.dx = s.dx; a.dy = s.dy; a.dz = s.dz; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz;
   a.uimp = s.uimp; a.pot4 = s.pot4; a.partial = ctx->d_partial; a.nblk = nblk(ctx);
   a.psi_all = s.psi; a.ring = s.ring; a.plus = ctx->d_plus;
+  if (ctx->any_dyn) {  // current invar for eso
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipMemcpy(ctx->h_dyn.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
+  }
   if (ctx->any_motion) {  // current positions for uplus, velocities for ekinx
     HIPCHK(hipStreamSynchronize(ctx->st));
     HIPCHK(hipMemcpy(ctx->h_motion.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
@@ -798,6 +857,7 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
     e.eHeX = S[5] * dv;
     e.uplus = S[11] * dv;
     e.eso = ctx->h_eso[b];
+    if (ctx->any_dyn && ctx->h_dyn[b].on) e.eso = state_eso(ctx->h_dyn[b].state, ctx->h_dyn[b].iv, ctx->h_dyn[b].als);
     if (ctx->any_motion && ctx->h_motion[b].on) {  // energy.f90:112
       const Motion& q = ctx->h_motion[b];
       e.ekinx = 0.5 * q.m * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]);
@@ -886,6 +946,11 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_motion.assign(ctx->B, Motion{});
+  if ((rc = dalloc(ctx, &ctx->d_dyn, ctx->B)) != EFFPOT_OK) return bail(rc);
+  ctx->h_dyn.assign(ctx->B, InvarDyn{});
+  ctx->h_sargs.assign(ctx->B, StateArgs{});
+  if (hipMemcpy(ctx->d_dyn, ctx->h_dyn.data(), sizeof(InvarDyn) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(EFFPOT_ERR_DEVICE);
   if (hipMemcpy(ctx->d_motion, ctx->h_motion.data(), sizeof(Motion) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if (hipMemcpy(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
@@ -1156,6 +1221,7 @@ int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
   ctx->h_eso[slot] = 0.0;
   ctx->h_imp[slot] = *imp;
   ctx->h_is_state[slot] = 0;
+  RC(stop_dyn(ctx, slot));
   return stop_motion(ctx, slot);
 }
 
@@ -1258,7 +1324,59 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
   ctx->h_eso[slot] = imp->eso;
   ctx->have_imp[slot] = 1;
   ctx->h_is_state[slot] = 1;
+  ctx->h_sargs[slot] = a;
+  ctx->h_dyn[slot].als = imp->als;  // kept for effpot_set_state_dynamics
+  RC(stop_dyn(ctx, slot));
   return stop_motion(ctx, slot);
+}
+
+int effpot_set_state_dynamics(effpot_ctx* ctx, int slot, int on, double ev0) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  if (!on) return stop_dyn(ctx, slot);
+  if (ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "invar dynamics need an FP64 context");
+  if (!ctx->h_is_state[slot]) return ctx->fail(EFFPOT_ERR_STATE, "set a P/D-state impurity first");
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  std::vector<int> one(ctx->B, 0);
+  one[slot] = 1;
+  HIPCHK(hipMemcpy(ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
+  RC(den_from_cur(ctx, ctx->d_mask_tmp, false));
+  RC(poten_t(ctx, ctx->s64, ctx->d_mask_tmp));
+  if (ev0 == 0.0) {  // Ev0 = eHeX + eso of the starting state (DFT4He3d.f90:444)
+    std::vector<effpot_energies> e(ctx->B);
+    RC(energy_t(ctx, ctx->s64, ctx->d_mask_tmp, e.data()));
+    ev0 = e[slot].eHeX + e[slot].eso;
+  }
+  const StateArgs& a = ctx->h_sargs[slot];
+  InvarDyn q{};
+  q.on = 1;
+  q.state = a.state;
+  q.n = a.state == EFFPOT_STATE_P ? 6 : 10;
+  q.als = ctx->h_dyn[slot].als;
+  q.ev0 = ev0;
+  for (int k = 0; k < 20; k++) q.iv[k] = a.inv[k];
+  for (int k = 0; k < 3; k++) q.iiv[k] = k;
+  q.ihiv[0] = 0; q.ihiv[1] = 1;
+  q.sa = a;
+  ctx->h_dyn[slot] = q;
+  HIPCHK(hipMemcpy(ctx->d_dyn + slot, &q, sizeof(InvarDyn), hipMemcpyHostToDevice));
+  ctx->any_dyn = 1;
+  RC(dyn_hiv(ctx, ctx->d_mask_tmp));  // the initial Hinvar (DFT4He3d.f90:454)
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_get_invar(effpot_ctx* ctx, int slot, double* invar) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B || !invar) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  InvarDyn q;
+  HIPCHK(hipMemcpy(&q, ctx->d_dyn + slot, sizeof(InvarDyn), hipMemcpyDeviceToHost));
+  const double* src = q.on ? q.iv : ctx->h_sargs[slot].inv;
+  for (int k = 0; k < 20; k++) invar[k] = src[k];
+  return EFFPOT_OK;
 }
 
 int effpot_set_real_time(effpot_ctx* ctx, int on) {
@@ -1692,8 +1810,9 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
 int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slots) {
   RC(check_ctx(ctx));
   if (!opt || !slots) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
-  if (ctx->rt || ctx->any_motion)
-    return ctx->fail(EFFPOT_ERR_STATE, "relaxation with real time or a moving impurity: step with effpot_step_rk/pc");
+  if (ctx->rt || ctx->any_motion || ctx->any_dyn)
+    return ctx->fail(EFFPOT_ERR_STATE,
+                     "relaxation with real time, a moving impurity or evolving invar: step with effpot_step_rk/pc");
   if (!(opt->dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
   if (opt->pener < 1 || opt->max_steps < 1 || opt->n_rk < 0) return ctx->fail(EFFPOT_ERR_ARG, "bad solve options");
   hipSetDevice(ctx->dev);

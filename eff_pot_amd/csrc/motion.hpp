@@ -154,4 +154,166 @@ __global__ __launch_bounds__(256) void k_force_partials(const Motion* M, const i
   }
 }
 
+// ---- internal-state dynamics of a P/D impurity (potinvar) -----------------
+// steprk.f90:152-167 and steppc.f90:84-102,205-226 advance invar with the
+// same integrators under Hinvar = -i (H(rho) invar - Ev0 invar), H(rho) the
+// density-weighted V + (Als/2) SO matrix of instates (potinvar,
+// potenimp.f90:261-492); potenimp then rebuilds uimp from the new invar.
+struct InvarDyn {
+  int32_t on, state, n, pad_;
+  double als, ev0;
+  double iv[20], hiv[20];                         // invar, Hinvar (complex, interleaved)
+  double ivold[3][20], hivold[2][20], pciv[20], qiv[20];
+  int32_t iiv[3], ihiv[2], pad2_;                 // ioldiv, ioldhiv (0-based storage slots)
+  StateArgs sa;                                   // the slot's tables and position
+};
+
+template <int K>
+EP_DEV void dyn_moments_body(const StateArgs& a, const double* den, double* partial) {
+  __shared__ double red[4 * K];
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  const long N = (long)a.nx * a.ny * a.nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % a.nx);
+    const long rr = i / a.nx;
+    const int iy = (int)(rr % a.ny), iz = (int)(rr / a.ny);
+    const double xx = (a.x0 + a.hx * ix) - a.rx, yy = (a.y0 + a.hy * iy) - a.ry, zz = (a.z0 + a.hz * iz) - a.rz;
+    const double d = den[i];
+    double t0, t1, t2;
+    state_fields(a, (xx * xx + yy * yy) + zz * zz, t0, t1, t2);
+    if constexpr (K == kMomP) {
+      v[0] += d;
+      v[1] += d * (t0 + t1 * xx * xx);
+      v[2] += d * (t0 + t1 * yy * yy);
+      v[3] += d * (t0 + t1 * zz * zz);
+      v[4] += d * t1 * xx * yy;
+      v[5] += d * t1 * xx * zz;
+      v[6] += d * t1 * yy * zz;
+    } else {
+      DForms f;
+      d_forms(xx, yy, zz, f);
+      const double dv1 = t1 * d, dv2 = t2 * d;
+#pragma unroll
+      for (int k = 0; k < 15; k++) v[k] += f.mp[k] * dv2;
+      int k = 15;
+#pragma unroll
+      for (int p = 0; p < 5; p++)
+#pragma unroll
+        for (int q = p; q < 5; q++) v[k++] += (f.d[p] * f.d[q]) * dv1;
+      v[30] += t0 * d;
+    }
+  }
+  block_sum<K>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partial[(long)blockIdx.x * kMomD + k] = v[k];
+  }
+}
+// Density moments of every slot whose invar evolves: partial[slot][block][31].
+__global__ __launch_bounds__(256) void k_dyn_moments(const InvarDyn* D, const int* active, const double* den, long N,
+                                                     double* partial) {
+  const int s = blockIdx.y;
+  if (!active[s] || !D[s].on) return;
+  double* p = partial + (long)s * kStateBlocks * kMomD;
+  if (D[s].state == EFFPOT_STATE_P) dyn_moments_body<kMomP>(D[s].sa, den + (long)s * N, p);
+  else dyn_moments_body<kMomD>(D[s].sa, den + (long)s * N, p);
+}
+// potinvar: Hinvar = -i (H invar - Ev0 invar) from the reduced moments.
+__global__ void k_dyn_hiv(InvarDyn* D, const int* active, int B, const double* partial, double dxyz) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B || !active[s] || !D[s].on) return;
+  InvarDyn& q = D[s];
+  const double* p = partial + (long)s * kStateBlocks * kMomD;
+  double m[kMomD];
+  const int K = q.state == EFFPOT_STATE_P ? kMomP : kMomD;
+  for (int k = 0; k < kMomD; k++) m[k] = 0.0;
+  for (int k = 0; k < K; k++) {
+    double acc = 0.0;
+    for (int b = 0; b < kStateBlocks; b++) acc += p[(long)b * kMomD + k];
+    m[k] = acc * dxyz;
+  }
+  double Hr[100], Hi[100];
+  const int n = state_matrix_ri(q.state, m, q.als, Hr, Hi);
+  for (int i = 0; i < n; i++) {
+    double re = -q.ev0 * q.iv[2 * i], im = -q.ev0 * q.iv[2 * i + 1];
+    for (int j = 0; j < n; j++) {
+      re += Hr[i * n + j] * q.iv[2 * j] - Hi[i * n + j] * q.iv[2 * j + 1];
+      im += Hr[i * n + j] * q.iv[2 * j + 1] + Hi[i * n + j] * q.iv[2 * j];
+    }
+    q.hiv[2 * i] = im;       // -i (re + i im) = im - i re
+    q.hiv[2 * i + 1] = -re;
+  }
+}
+// potenimp with the slot's current invar (grid: blocks x slots).
+__global__ __launch_bounds__(256) void k_dyn_uimp(const InvarDyn* D, const int* active, double* uimp, long N) {
+  const int s = blockIdx.y;
+  if (!active[s] || !D[s].on) return;
+  StateArgs a = D[s].sa;
+  for (int k = 0; k < 20; k++) a.inv[k] = D[s].iv[k];
+  double* u = uimp + (long)s * N;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    u[i] = state_uimp_at(a, i);
+}
+// One RK-Gill stage of invar (steprk.f90:152-167); the history indices are the
+// identity after the last stage.
+__global__ void k_dyn_rk(InvarDyn* D, const int* active, int B, int j, const double* dtv, double arun, double brun,
+                         double crun) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B || !active[s] || !D[s].on) return;
+  InvarDyn& q = D[s];
+  const double dt = dtv[s];
+  for (int k = 0; k < 20; k++) {
+    const double a2 = arun * (q.hiv[k] - brun * q.qiv[k]);
+    q.qiv[k] = q.qiv[k] + 3.0 * a2 - crun * q.hiv[k];
+    if (j == 0) {
+      q.hivold[1][k] = q.hivold[0][k]; q.hivold[0][k] = q.hiv[k];
+      q.ivold[2][k] = q.ivold[1][k]; q.ivold[1][k] = q.ivold[0][k]; q.ivold[0][k] = q.iv[k];
+    }
+    q.iv[k] = q.iv[k] + dt * a2;
+  }
+  if (j == 3) {
+    for (int k = 0; k < 3; k++) q.iiv[k] = k;
+    q.ihiv[0] = 0; q.ihiv[1] = 1;
+  }
+}
+// Predictor + modifier of invar, then Hinvarold(ioldhiv(2)) = Hinvar and the
+// ioldhiv swap (steppc.f90:84-102).
+__global__ void k_dyn_pred(InvarDyn* D, const int* active, int B, const double* dtv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B || !active[s] || !D[s].on) return;
+  InvarDyn& q = D[s];
+  const double dt = dtv[s], c4o3 = 4.0 / 3.0, c112 = 112.0 / 121.0;
+  for (int k = 0; k < 20; k++) {
+    const double p = q.ivold[q.iiv[2]][k] +
+                     c4o3 * dt * (2.0 * q.hiv[k] - q.hivold[q.ihiv[0]][k] + 2.0 * q.hivold[q.ihiv[1]][k]);
+    q.ivold[q.iiv[2]][k] = q.iv[k];
+    q.iv[k] = p - c112 * q.pciv[k];
+    q.pciv[k] = p;
+  }
+  for (int k = 0; k < 20; k++) q.hivold[q.ihiv[1]][k] = q.hiv[k];
+  const int t = q.ihiv[1]; q.ihiv[1] = q.ihiv[0]; q.ihiv[0] = t;
+}
+// Corrector of invar and the ioldiv rotation (steppc.f90:205-226); a P state
+// keeps components 7-10 at zero.
+__global__ void k_dyn_corr(InvarDyn* D, const int* active, int B, const double* dtv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B || !active[s] || !D[s].on) return;
+  InvarDyn& q = D[s];
+  const double dt = dtv[s], c9 = 9.0 / 121.0;
+  const int nn = 2 * q.n;
+  for (int k = 0; k < 20; k++) {
+    const double x = 0.125 * (9.0 * q.ivold[q.iiv[2]][k] - q.ivold[q.iiv[1]][k] +
+                              3.0 * dt * (q.hiv[k] + 2.0 * q.hivold[q.ihiv[0]][k] - q.hivold[q.ihiv[1]][k]));
+    if (k < nn) {
+      q.pciv[k] = q.pciv[k] - x;
+      q.iv[k] = x + c9 * q.pciv[k];
+    } else {
+      q.iv[k] = 0.0;
+    }
+  }
+  const int t = q.iiv[2]; q.iiv[2] = q.iiv[1]; q.iiv[1] = q.iiv[0]; q.iiv[0] = t;
+}
+
 }  // namespace ep

@@ -161,91 +161,99 @@ __global__ __launch_bounds__(256) void k_state_moments(const V* psi, StateArgs a
   }
 }
 
-// potenimp (:34-45 P, :54-158 D) and the cap (:184-186) for one slot.
+// potenimp (:34-45 P, :54-158 D) and the cap (:184-186) at grid point i.
+EP_DEV double state_uimp_at(const StateArgs& a, long i) {
+  const int ix = (int)(i % a.nx);
+  const long rr = i / a.nx;
+  const int iy = (int)(rr % a.ny), iz = (int)(rr / a.ny);
+  const double xx = (a.x0 + a.hx * ix) - a.rx, yy = (a.y0 + a.hy * iy) - a.ry, zz = (a.z0 + a.hz * iz) - a.rz;
+  double t0, t1, t2;
+  state_fields(a, (xx * xx + yy * yy) + zz * zz, t0, t1, t2);
+  double u;
+  if (a.state == EFFPOT_STATE_P) {
+    // odd (spin up) and even (spin down) components of iv . r
+    const double ore = a.inv[0] * xx + a.inv[4] * yy + a.inv[8] * zz;
+    const double oim = a.inv[1] * xx + a.inv[5] * yy + a.inv[9] * zz;
+    const double ere = a.inv[2] * xx + a.inv[6] * yy + a.inv[10] * zz;
+    const double eim = a.inv[3] * xx + a.inv[7] * yy + a.inv[11] * zz;
+    u = t0 + t1 * ((ore * ore + oim * oim) + (ere * ere + eim * eim));
+  } else {
+    DForms f;
+    d_forms(xx, yy, zz, f);
+    // M = Delta*I + Pi_Del*(d d^T) + Sig_Del*MP; u = sum_ij M_ij Re(conj(c_i) c_j)
+    // over both spin components, c_i = iv(2i-1) (up) and iv(2i) (down)
+    u = 0.0;
+    int k = 0;
+#pragma unroll
+    for (int p = 0; p < 5; p++)
+#pragma unroll
+      for (int q = p; q < 5; q++, k++) {
+        double m = f.mp[k] * t2 + (f.d[p] * f.d[q]) * t1;
+        if (p == q) m += t0;
+        const double* cp = a.inv + 4 * p;
+        const double* cq = a.inv + 4 * q;
+        const double g = (cp[0] * cq[0] + cp[1] * cq[1]) + (cp[2] * cq[2] + cp[3] * cq[3]);
+        u += (p == q ? 1.0 : 2.0) * m * g;
+      }
+  }
+  return u < a.cap ? u : a.cap;
+}
 template <typename T>
 __global__ __launch_bounds__(256) void k_state_uimp(StateArgs a, T* uimp) {
   const long N = (long)a.nx * a.ny * a.nz;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    const int ix = (int)(i % a.nx);
-    const long rr = i / a.nx;
-    const int iy = (int)(rr % a.ny), iz = (int)(rr / a.ny);
-    const double xx = (a.x0 + a.hx * ix) - a.rx, yy = (a.y0 + a.hy * iy) - a.ry, zz = (a.z0 + a.hz * iz) - a.rz;
-    double t0, t1, t2;
-    state_fields(a, (xx * xx + yy * yy) + zz * zz, t0, t1, t2);
-    double u;
-    if (a.state == EFFPOT_STATE_P) {
-      // odd (spin up) and even (spin down) components of iv . r
-      const double ore = a.inv[0] * xx + a.inv[4] * yy + a.inv[8] * zz;
-      const double oim = a.inv[1] * xx + a.inv[5] * yy + a.inv[9] * zz;
-      const double ere = a.inv[2] * xx + a.inv[6] * yy + a.inv[10] * zz;
-      const double eim = a.inv[3] * xx + a.inv[7] * yy + a.inv[11] * zz;
-      u = t0 + t1 * ((ore * ore + oim * oim) + (ere * ere + eim * eim));
-    } else {
-      DForms f;
-      d_forms(xx, yy, zz, f);
-      // M = Delta*I + Pi_Del*(d d^T) + Sig_Del*MP; u = sum_ij M_ij Re(conj(c_i) c_j)
-      // over both spin components, c_i = iv(2i-1) (up) and iv(2i) (down)
-      u = 0.0;
-      int k = 0;
-#pragma unroll
-      for (int p = 0; p < 5; p++)
-#pragma unroll
-        for (int q = p; q < 5; q++, k++) {
-          double m = f.mp[k] * t2 + (f.d[p] * f.d[q]) * t1;
-          if (p == q) m += t0;
-          const double* cp = a.inv + 4 * p;
-          const double* cq = a.inv + 4 * q;
-          const double g = (cp[0] * cq[0] + cp[1] * cq[1]) + (cp[2] * cq[2] + cp[3] * cq[3]);
-          u += (p == q ? 1.0 : 2.0) * m * g;
-        }
-    }
-    uimp[i] = (T)(u < a.cap ? u : a.cap);
-  }
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    uimp[i] = (T)state_uimp_at(a, i);
 }
 
 // ---- host half of instates -------------------------------------------------
 using zd = std::complex<double>;
 
-// V + (Als/2) SO of the moments (instates.f90:159-195 P, :520-565 D), n x n
-// row-major.
-inline int state_matrix(int state, const double* m, double als, zd* H) {
+// V + (Als/2) SO of the moments (instates.f90:159-195 P, :520-565 D; the
+// same matrix potinvar applies, potenimp.f90:318-488), n x n row-major, real
+// and imaginary parts apart so that the device can build it too.
+__host__ __device__ inline int state_matrix_ri(int state, const double* m, double als, double* Hr, double* Hi) {
   const int n = state == EFFPOT_STATE_P ? 6 : 10;
-  for (int i = 0; i < n * n; i++) H[i] = 0.0;
-  auto at = [&](int i, int j) -> zd& { return H[(i - 1) * n + (j - 1)]; };  // 1-based
-  const zd I(0.0, 1.0);
-  const double also2 = 0.5 * als;
-  zd so[10][10] = {};
-  auto S = [&](int i, int j) -> zd& { return so[i - 1][j - 1]; };
+  for (int i = 0; i < n * n; i++) { Hr[i] = 0.0; Hi[i] = 0.0; }
+  const double h = 0.5 * als;  // Also2
+  // upper-triangle spin-orbit entries (1-based i < j), value h * (re + i im)
+  auto so = [&](int i, int j, double re, double im) {
+    Hr[(i - 1) * n + (j - 1)] += h * re; Hi[(i - 1) * n + (j - 1)] += h * im;
+    Hr[(j - 1) * n + (i - 1)] += h * re; Hi[(j - 1) * n + (i - 1)] -= h * im;
+  };
   if (state == EFFPOT_STATE_P) {
     const double Vx = m[1], Vy = m[2], Vz = m[3], Vxy = m[4], Vxz = m[5], Vyz = m[6];
-    at(1, 1) = at(2, 2) = Vx; at(3, 3) = at(4, 4) = Vy; at(5, 5) = at(6, 6) = Vz;
-    at(1, 3) = at(2, 4) = Vxy; at(1, 5) = at(2, 6) = Vxz; at(3, 5) = at(4, 6) = Vyz;
-    S(1, 3) = -I; S(2, 4) = I; S(2, 5) = -1.0; S(1, 6) = 1.0; S(4, 5) = -I; S(3, 6) = -I;
+    const double d[3] = {Vx, Vy, Vz};
+    for (int a = 0; a < 3; a++) { Hr[(2 * a) * n + 2 * a] = d[a]; Hr[(2 * a + 1) * n + 2 * a + 1] = d[a]; }
+    auto sym = [&](int i, int j, double v) { Hr[(i - 1) * n + (j - 1)] = v; Hr[(j - 1) * n + (i - 1)] = v; };
+    sym(1, 3, Vxy); sym(2, 4, Vxy); sym(1, 5, Vxz); sym(2, 6, Vxz); sym(3, 5, Vyz); sym(4, 6, Vyz);
+    // SO(1,3)=-i SO(2,4)=i SO(2,5)=-1 SO(1,6)=1 SO(4,5)=-i SO(3,6)=-i
+    so(1, 3, 0.0, -1.0); so(2, 4, 0.0, 1.0); so(2, 5, -1.0, 0.0); so(1, 6, 1.0, 0.0);
+    so(4, 5, 0.0, -1.0); so(3, 6, 0.0, -1.0);
   } else {
     double mp[5][5], ms[5][5];
     int k = 0;
     for (int p = 0; p < 5; p++)
-      for (int q = p; q < 5; q++, k++) { mp[p][q] = mp[q][p] = m[k]; }
+      for (int q = p; q < 5; q++, k++) { mp[p][q] = m[k]; mp[q][p] = m[k]; }
     for (int p = 0; p < 5; p++)
-      for (int q = p; q < 5; q++, k++) { ms[p][q] = ms[q][p] = m[k]; }
+      for (int q = p; q < 5; q++, k++) { ms[p][q] = m[k]; ms[q][p] = m[k]; }
     for (int p = 0; p < 5; p++)
       for (int q = 0; q < 5; q++) {
         const double v = (p == q ? m[30] : 0.0) + ms[p][q] + mp[p][q];
-        H[(2 * p) * n + 2 * q] = v;          // MatD(2i-1, 2j-1)
-        H[(2 * p + 1) * n + 2 * q + 1] = v;  // MatD(2i, 2j)
+        Hr[(2 * p) * n + 2 * q] = v;          // MatD(2i-1, 2j-1)
+        Hr[(2 * p + 1) * n + 2 * q + 1] = v;  // MatD(2i, 2j)
       }
-    const double s3 = std::sqrt(3.0);
-    S(1, 4) = 1.0; S(1, 8) = -I; S(1, 9) = 2.0 * I; S(2, 3) = -1.0; S(2, 7) = -I; S(2, 10) = -2.0 * I;
-    S(3, 6) = -I * s3; S(3, 7) = I; S(3, 10) = -I; S(4, 5) = -I * s3; S(4, 8) = -I; S(4, 9) = -I;
-    S(5, 8) = -s3; S(6, 7) = s3; S(7, 10) = -1.0; S(8, 9) = 1.0;
+    const double s3 = 1.7320508075688772;
+    so(1, 4, 1.0, 0.0); so(1, 8, 0.0, -1.0); so(1, 9, 0.0, 2.0); so(2, 3, -1.0, 0.0); so(2, 7, 0.0, -1.0);
+    so(2, 10, 0.0, -2.0); so(3, 6, 0.0, -s3); so(3, 7, 0.0, 1.0); so(3, 10, 0.0, -1.0); so(4, 5, 0.0, -s3);
+    so(4, 8, 0.0, -1.0); so(4, 9, 0.0, -1.0); so(5, 8, -s3, 0.0); so(6, 7, s3, 0.0); so(7, 10, -1.0, 0.0);
+    so(8, 9, 1.0, 0.0);
   }
-  for (int i = 1; i <= n; i++)
-    for (int j = i + 1; j <= n; j++) {
-      S(j, i) = std::conj(S(i, j));
-      if (state == EFFPOT_STATE_P) at(j, i) = at(i, j);  // Mat(j,i) = Mat(i,j)
-    }
-  for (int i = 1; i <= n; i++)
-    for (int j = 1; j <= n; j++) at(i, j) += also2 * S(i, j);
+  return n;
+}
+inline int state_matrix(int state, const double* m, double als, zd* H) {
+  double Hr[100], Hi[100];
+  const int n = state_matrix_ri(state, m, als, Hr, Hi);
+  for (int i = 0; i < n * n; i++) H[i] = zd(Hr[i], Hi[i]);
   return n;
 }
 

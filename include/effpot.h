@@ -293,6 +293,20 @@ int effpot_set_impurity_motion(effpot_ctx* ctx, int slot, const double* vimp, do
 /* Current r (A), v and a of the slot's impurity (v = a = 0 if it does not move). */
 int effpot_get_impurity(effpot_ctx* ctx, int slot, double* r, double* v, double* a);
 
+/* The invar of the slot's P/D-state impurity evolves in effpot_step_rk /
+ * effpot_step_pc (the reference's default Lfix_lambda = .false.):
+ * d invar/dt = -i (H(rho) invar - Ev0 invar), H(rho) the density-weighted
+ * V + (Als/2) SO of instates recomputed after every poten (potinvar,
+ * potenimp.f90:261-492), with the same RK-Gill / Hamming integrators
+ * (steprk.f90:152-167, steppc.f90:84-102,205-226), and uimp and eso follow
+ * invar.  ev0 = 0 takes Ev0 = eHeX + eso of the current state
+ * (DFT4He3d.f90:444).  on = 0 freezes invar again.  FP64 contexts only;
+ * relaxation (effpot_solve / effpot_tick) refuses it. */
+int effpot_set_state_dynamics(effpot_ctx* ctx, int slot, int on, double ev0);
+
+/* Current invar (complex(10), interleaved) of the slot's P/D-state impurity. */
+int effpot_get_invar(effpot_ctx* ctx, int slot, double* invar);
+
 /* X+-He potential of one slot for the uplus energy term, evaluated at the
  * slot's impurity position (energy.f90:146-163).  A context starts with no
  * X+ potential (uplus = 0); NULL removes it again. */

@@ -134,6 +134,10 @@ typedef struct oracle {
   int moving; double mimp, vimp[3], aimp[3];
   double rimpold[3][3], vimpold[3][3], aimpold[2][3], pcr[3], pcv[3], qr[3], qv[3];
   int ioldr[3], ioldv[3], iolda[2];
+  /* evolving invar of a P/D state (potinvar + the invar integrators) */
+  int dyn; double ev0, hiv[20], ivold[3][20], hivold[2][20], pciv[20], qiv[20];
+  int iiv[3], ihiv[2];
+  effpot_state_imp simp; /* the state's set-up; simp.invar is the current invar */
   double *stab, *sf[3]; /* P/D state: potential tables [3][npot+2], updatepoten fields */
 } oracle_t;
 
@@ -521,7 +525,7 @@ double oracle_update(double r, int npot, double rmax, const double* f) {
 
 int oracle_set_impurity(oracle_t* o, const effpot_impurity* imp) {
   if (imp->kind != EFFPOT_POT_MORSE && imp->kind != EFFPOT_POT_NULL) return EFFPOT_ERR_POT;
-  o->imp = *imp; o->have_imp = 1; o->eso = 0.0; o->moving = 0;
+  o->imp = *imp; o->have_imp = 1; o->eso = 0.0; o->moving = 0; o->dyn = 0;
   /* potenimpini.f90:35-58 (isotropic: V_pi = V, Delta = 0) */
   for (int i = 1; i <= o->npot; i++) o->vtab[i] = oracle_select_pot(imp, (double)(i - 1) * o->delinter);
   o->vtab[0] = o->vtab[1];
@@ -783,6 +787,53 @@ int oracle_set_impurity_state(oracle_t* o, const effpot_state_imp* s) {
   o->have_imp = 1;
   return EFFPOT_OK;
 }
+
+void oracle_poten(oracle_t* o);
+void oracle_energy(oracle_t* o, effpot_energies* e);
+
+/* potinvar (potenimp.f90:261-492): Hinvar = -i (H(rho) invar - Ev0 invar). */
+static void potinvar(oracle_t* o) {
+  double mom[31], H[200];
+  oracle_state_moments(o, &o->simp, mom);
+  const int n = oracle_state_matrix(o->simp.state, mom, o->simp.als, H);
+  const double* iv = o->simp.invar;
+  for (int i = 0; i < n; i++) {
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < n; j++) {
+      double hr = H[2 * (i * n + j)], hi = H[2 * (i * n + j) + 1];
+      re += hr * iv[2 * j] - hi * iv[2 * j + 1];
+      im += hr * iv[2 * j + 1] + hi * iv[2 * j];
+    }
+    re -= o->ev0 * iv[2 * i];
+    im -= o->ev0 * iv[2 * i + 1];
+    o->hiv[2 * i] = im; /* -i (re + i im) */
+    o->hiv[2 * i + 1] = -re;
+  }
+}
+
+/* invar evolves from now on (Lfix_lambda = .false.); ev0 = 0: Ev0 = eHeX + eso
+ * of the current state (DFT4He3d.f90:444).  s is the P/D set-up of
+ * oracle_set_impurity_state with the starting invar. */
+int oracle_set_state_dynamics(oracle_t* o, const effpot_state_imp* s, double ev0) {
+  o->simp = *s;
+  oracle_set_impurity_state(o, &o->simp);
+  oracle_poten(o);
+  if (ev0 == 0.0) {
+    effpot_energies e;
+    oracle_energy(o, &e);
+    ev0 = e.eHeX + e.eso;
+  }
+  o->ev0 = ev0;
+  memset(o->hiv, 0, sizeof(o->hiv)); memset(o->ivold, 0, sizeof(o->ivold));
+  memset(o->hivold, 0, sizeof(o->hivold)); memset(o->pciv, 0, sizeof(o->pciv)); memset(o->qiv, 0, sizeof(o->qiv));
+  for (int k = 0; k < 3; k++) o->iiv[k] = k;
+  o->ihiv[0] = 0; o->ihiv[1] = 1;
+  o->dyn = 1;
+  potinvar(o);
+  return EFFPOT_OK;
+}
+
+void oracle_get_invar(const oracle_t* o, double* iv) { memcpy(iv, o->simp.invar, sizeof(o->simp.invar)); }
 
 /* ------------------------------------------------------------------------ */
 /* poten (poten.f90:5-131) and term_alfa (term_alfa.f90:5-109).              */
@@ -1056,9 +1107,25 @@ int oracle_step_rk(oracle_t* o, double dt) {
       }
       if (j <= 2) imp_field_at(o, r);
     }
+    if (o->dyn) { /* steprk.f90:152-167 */
+      double* iv = o->simp.invar;
+      for (int k = 0; k < 20; k++) {
+        double a2 = arun[j] * (o->hiv[k] - brun[j] * o->qiv[k]);
+        o->qiv[k] = o->qiv[k] + 3.0 * a2 - crun[j] * o->hiv[k];
+        if (j == 0) {
+          o->hivold[1][k] = o->hivold[0][k]; o->hivold[0][k] = o->hiv[k];
+          o->ivold[2][k] = o->ivold[1][k]; o->ivold[1][k] = o->ivold[0][k]; o->ivold[0][k] = iv[k];
+        }
+        iv[k] = iv[k] + dt * a2;
+      }
+      if (j <= 2) oracle_set_impurity_state(o, &o->simp); /* potenimp(rimp, invar) */
+    }
     if (j <= 2) oracle_poten(o);
     if (o->moving && j <= 2) force_imp(o);
+    if (o->dyn && j <= 2) potinvar(o);
   }
+  for (int i = 0; i < 3; i++) o->iiv[i] = i;
+  o->ihiv[0] = 0; o->ihiv[1] = 1;
   for (int i = 0; i < 3; i++) o->ioldp[i] = i;
   for (int i = 0; i < 2; i++) o->ioldh[i] = i;
   for (int i = 0; i < 3; i++) o->ioldr[i] = o->ioldv[i] = i;
@@ -1103,8 +1170,22 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
     { int t = ia[1]; ia[1] = ia[0]; ia[0] = t; }
     imp_field_at(o, r); /* potenimp(rimp) */
   }
+  if (o->dyn) { /* steppc.f90:84-102 */
+    double* iv = o->simp.invar;
+    int* ii = o->iiv; int* ih = o->ihiv;
+    for (int k = 0; k < 20; k++) {
+      double aux = o->ivold[ii[2]][k] + c4o3m * dt * (2.0 * o->hiv[k] - o->hivold[ih[0]][k] + 2.0 * o->hivold[ih[1]][k]);
+      o->ivold[ii[2]][k] = iv[k];
+      iv[k] = aux - c112 * o->pciv[k];
+      o->pciv[k] = aux;
+    }
+    for (int k = 0; k < 20; k++) o->hivold[ih[1]][k] = o->hiv[k];
+    { int t = ih[1]; ih[1] = ih[0]; ih[0] = t; }
+    oracle_set_impurity_state(o, &o->simp); /* potenimp(rimp, invar) */
+  }
   oracle_poten(o);
   if (o->moving) force_imp(o);
+  if (o->dyn) potinvar(o);
   compute_hpsi(o, hp);
   p3 = o->psiold[o->ioldp[2]];
   double* p2 = o->psiold[o->ioldp[1]];
@@ -1140,6 +1221,19 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
       o->vimp[c] = aux + c9 * o->pcv[c];
     }
     { int t = iv[2]; iv[2] = iv[1]; iv[1] = iv[0]; iv[0] = t; }
+  }
+  if (o->dyn) { /* steppc.f90:205-226; a P state keeps components 7-10 at zero */
+    double* iv = o->simp.invar;
+    int* ii = o->iiv; int* ih = o->ihiv;
+    int nn = o->simp.state == EFFPOT_STATE_P ? 12 : 20;
+    for (int k = 0; k < 20; k++) {
+      double aux = 0.125 * (9.0 * o->ivold[ii[2]][k] - o->ivold[ii[1]][k] +
+                            3.0 * dt * (o->hiv[k] + 2.0 * o->hivold[ih[0]][k] - o->hivold[ih[1]][k]));
+      if (k < nn) { o->pciv[k] = o->pciv[k] - aux; iv[k] = aux + c9 * o->pciv[k]; }
+      else iv[k] = 0.0;
+    }
+    { int t = ii[2]; ii[2] = ii[1]; ii[1] = ii[0]; ii[0] = t; }
+    o->eso = oracle_state_eso(o->simp.state, iv, o->simp.als);
   }
   if (err_he) *err_he = err / (double)o->N;
   if (!den_ok(o)) return EFFPOT_ERR_NONFINITE;

@@ -60,6 +60,8 @@ def lib():
         L.oracle_state_eso.argtypes = [C.c_int, _dp, C.c_double]
         L.oracle_state_eso.restype = C.c_double
         L.oracle_set_impurity_state.argtypes = [C.c_void_p, C.POINTER(StateImp)]
+        L.oracle_set_state_dynamics.argtypes = [C.c_void_p, C.POINTER(StateImp), C.c_double]
+        L.oracle_get_invar.argtypes = [C.c_void_p, _dp]
         L.oracle_update.argtypes = [C.c_double, C.c_int, C.c_double, _dp]
         L.oracle_update.restype = C.c_double
         L.oracle_poten.argtypes = [C.c_void_p]
@@ -234,6 +236,16 @@ class Oracle:
         """potenimp for imp.invar (the caller chooses it)."""
         rc = lib().oracle_set_impurity_state(self.h, C.byref(imp))
         assert rc == OK
+
+    def set_state_dynamics(self, imp: StateImp, ev0=0.0):
+        """invar evolves from imp.invar on (potinvar + the invar integrators)."""
+        rc = lib().oracle_set_state_dynamics(self.h, C.byref(imp), ev0)
+        assert rc == OK
+
+    def invar(self):
+        v = np.zeros(20)
+        lib().oracle_get_invar(self.h, v)
+        return v[0::2] + 1j * v[1::2]
 
     def set_plus(self, plus):
         lib().oracle_set_plus(self.h, C.byref(plus) if plus is not None else None)

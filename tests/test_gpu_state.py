@@ -124,3 +124,52 @@ def test_fortran_driver_p_state(gpu_available, tmp_path):
     assert abs(val(r"Impurity energy \(X\)") - e["eimpu"]) < 1e-5
     assert abs(val(r"Spin-Orbit energy \(X\)") - e["eso"]) < 1e-5 and abs(e["eso"]) > 0.1
     assert abs(val(r"TOTAL ENERGY \(He\+X\)") - e["etot"]) < 1e-5
+
+
+def test_state_dynamics_matches_oracle(gpu_available):
+    """invar evolving under potinvar (real time, 3 RK + 50 PC steps) for a P
+    and a D state in one context, against the oracle from the same start."""
+    print("[state] invar dynamics 32^3 x 2 slots", flush=True)
+    cases = [(ep.STATE_P, (0.0, 0.5, -1.0)), (ep.STATE_D, (1.0, 0.0, 0.75))]
+    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=2, psi_complex=1)
+    dt = 5e-5
+    with ep.Context(p) as ctx:
+        start = []
+        for s, (state, rimp) in enumerate(cases):
+            ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 60 + s, slot=s)
+            st = ctx.set_impurity_state(_imp(state, 0, rimp, False, als=6.0), slot=s)
+            start.append((ctx.psi(s), st.invar_c()))
+            ctx.set_state_dynamics(True, slot=s)
+        ctx.set_real_time(True)
+        ctx.poten()
+        for _ in range(3):
+            ctx.step_rk(dt)
+        for _ in range(50):
+            ctx.step_pc(dt)
+        got = [(ctx.invar(s), ctx.psi(s)) for s in range(2)]
+        ctx.poten()
+        es = ctx.energy()
+    for s, (state, rimp) in enumerate(cases):
+        n = 6 if state == ep.STATE_P else 10
+        o = O.Oracle(make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377))
+        o.set_psi(start[s][0])
+        imp = ep.make_state_imp(state, rimp=rimp, instate=-1, als=6.0, invar=start[s][1])
+        o.set_impurity_state(imp)
+        o.poten()
+        o.set_state_dynamics(imp, ev0=0.0)
+        o.set_real_time(True)
+        for _ in range(3):
+            o.step_rk(dt)
+        for _ in range(50):
+            o.step_pc(dt)
+        iv, psi = o.invar(), o.psi()
+        o.poten()
+        e = o.energy()
+        o.close()
+        moved = np.linalg.norm(got[s][0][:n] - np.asarray(start[s][1]))
+        print(s, "invar moved by", moved, "diff", np.max(np.abs(got[s][0] - iv)))
+        assert moved > 1e-3
+        assert np.max(np.abs(got[s][0] - iv)) <= 1e-9
+        assert np.linalg.norm(got[s][1] - psi) <= 1e-9 * np.linalg.norm(psi)
+        for k in ("eso", "eHeX", "eimpu", "etot"):
+            assert abs(es[s][k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (s, k, es[s][k], e[k])

@@ -1,0 +1,49 @@
+"""Internal-state dynamics (potinvar + the invar integrators) in the oracle.
+
+Pin: with a negligible helium density (psi = 1e-6 everywhere, real time) H(rho) is the
+spin-orbit matrix alone, so invar must follow exp(-i (H - Ev0) t) invar0
+through the RK-Gill starter and the Hamming PMC; the closed form comes from
+LAPACK (numpy.linalg.eigh).  A wrong sign of i, a missing Ev0, a wrong
+spin-orbit entry or a history index out of place fails it.  The norm of invar
+stays 1 and eso follows invar."""
+import numpy as np
+import pytest
+
+import eff_pot_amd as ep
+from eff_pot_amd.abi import make_params, CORE_OT
+from oracle import oracle as O
+from test_state_impurity import POTS
+
+
+@pytest.mark.parametrize("state", [ep.STATE_P, ep.STATE_D])
+def test_spin_orbit_precession(state):
+    n = 6 if state == ep.STATE_P else 10
+    o = O.Oracle(make_params(16, xmax=6.0, n4=1e-9, core=CORE_OT, h_core=2.377))
+    o.set_psi(np.full((16, 16, 16), 1e-6 + 0j))
+    rng = np.random.default_rng(7 + state)
+    iv0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    iv0 /= np.linalg.norm(iv0)
+    als, ev0 = 30.0, 4.0
+    imp = ep.make_state_imp(state, rimp=(0.0, 0.0, 0.0), instate=-1, als=als, invar=iv0, **POTS)
+    o.set_impurity_state(imp)
+    o.poten()
+    o.set_real_time(True)  # the helium part stays put (unitary); invar's equation is the same either way
+    o.set_state_dynamics(imp, ev0=ev0)
+    dt, nrk, npc = 5e-5, 3, 200
+    for _ in range(nrk):
+        o.step_rk(dt)
+    for _ in range(npc):
+        o.step_pc(dt)
+    t = (nrk + npc) * dt
+    H = O.state_matrix(state, np.zeros(31), als) - ev0 * np.eye(n)
+    w, V = np.linalg.eigh(H)
+    want = V @ (np.exp(-1j * w * t) * (V.conj().T @ iv0))
+    got = o.invar()[:n]
+    assert np.max(np.abs(got - want)) < 1e-9, np.max(np.abs(got - want))
+    assert abs(np.linalg.norm(got) - 1.0) < 1e-10
+    if n < 10:
+        assert np.max(np.abs(o.invar()[n:])) == 0.0
+    o.poten()
+    e = o.energy()
+    assert abs(e["eso"] - O.state_eso(state, got, als)) < 1e-13
+    o.close()
