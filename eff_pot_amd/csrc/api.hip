@@ -177,6 +177,7 @@ struct effpot_ctx {
   InvarDyn* d_dyn = nullptr;      // [B] P/D-state invar dynamics
   std::vector<InvarDyn> h_dyn;
   std::vector<StateArgs> h_sargs; // [B] the P/D-state set-up of each slot
+  std::vector<double> h_sals;     // [B] its spin-orbit splitting
   int any_dyn = 0;
   double* d_dynpart = nullptr;    // [B][kStateBlocks][kMomD] moment partials
   int plane = 1;                  // forward x and y FFT passes fused per z plane where the plane fits in LDS
@@ -689,9 +690,7 @@ int stop_motion(effpot_ctx* ctx, int slot) {
 int stop_dyn(effpot_ctx* ctx, int slot) {
   if (!ctx->h_dyn[slot].on) return EFFPOT_OK;
   HIPCHK(hipStreamSynchronize(ctx->st));
-  const double als = ctx->h_dyn[slot].als;
   ctx->h_dyn[slot] = InvarDyn{};
-  ctx->h_dyn[slot].als = als;
   HIPCHK(hipMemcpy(ctx->d_dyn + slot, &ctx->h_dyn[slot], sizeof(InvarDyn), hipMemcpyHostToDevice));
   ctx->any_dyn = 0;
   for (const InvarDyn& q : ctx->h_dyn) ctx->any_dyn |= q.on;
@@ -949,6 +948,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_dyn, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_dyn.assign(ctx->B, InvarDyn{});
   ctx->h_sargs.assign(ctx->B, StateArgs{});
+  ctx->h_sals.assign(ctx->B, 0.0);
   if (hipMemcpy(ctx->d_dyn, ctx->h_dyn.data(), sizeof(InvarDyn) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if (hipMemcpy(ctx->d_motion, ctx->h_motion.data(), sizeof(Motion) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
@@ -1325,7 +1325,7 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
   ctx->have_imp[slot] = 1;
   ctx->h_is_state[slot] = 1;
   ctx->h_sargs[slot] = a;
-  ctx->h_dyn[slot].als = imp->als;  // kept for effpot_set_state_dynamics
+  ctx->h_sals[slot] = imp->als;
   RC(stop_dyn(ctx, slot));
   return stop_motion(ctx, slot);
 }
@@ -1353,7 +1353,7 @@ int effpot_set_state_dynamics(effpot_ctx* ctx, int slot, int on, double ev0) {
   q.on = 1;
   q.state = a.state;
   q.n = a.state == EFFPOT_STATE_P ? 6 : 10;
-  q.als = ctx->h_dyn[slot].als;
+  q.als = ctx->h_sals[slot];
   q.ev0 = ev0;
   for (int k = 0; k < 20; k++) q.iv[k] = a.inv[k];
   for (int k = 0; k < 3; k++) q.iiv[k] = k;
