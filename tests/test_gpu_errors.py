@@ -37,8 +37,15 @@ def test_refusals(gpu_available):
         c.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 1)
         c.set_impurity_state(ep.make_state_imp(ep.STATE_P, instate=0))
         assert _code(lambda: c.set_impurity_motion([0, 0, 1], 40.0)) == ERR_STATE
+        # evolving invar needs a P/D state; relaxation refuses it
+        c.set_state_dynamics(True)
+        opts = SolveOpts(dt=1e-4, tol=1e-8, n_rk=3, pener=50, max_steps=10, polish_steps=0, tol_fp32=0.0,
+                         check_finite=1)
+        assert _code(lambda: c.solve(opts)) == ERR_STATE
+        c.set_state_dynamics(False)
         # still usable
         c.set_impurity(make_morse())
+        assert _code(lambda: c.set_state_dynamics(True)) == ERR_STATE  # isotropic impurity
         c.poten()
         assert np.isfinite(c.energy()[0]["etot"])
     cplx = make_params(32, xmax=12.0, n4=100.0, psi_complex=1)
@@ -61,3 +68,5 @@ def test_refusals(gpu_available):
         c.set_impurity(make_morse())
         assert _code(lambda: c.set_real_time(True)) == ERR_ARG
         assert _code(lambda: c.set_impurity_motion([0, 0, 1], 40.0)) == ERR_ARG
+        c.set_impurity_state(ep.make_state_imp(ep.STATE_D, instate=0))
+        assert _code(lambda: c.set_state_dynamics(True)) == ERR_ARG
