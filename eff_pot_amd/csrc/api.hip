@@ -689,6 +689,7 @@ int stop_motion(effpot_ctx* ctx, int slot) {
 // The slot's invar stops evolving (a new impurity was set, or switched off).
 int stop_dyn(effpot_ctx* ctx, int slot) {
   if (!ctx->h_dyn[slot].on) return EFFPOT_OK;
+  if (ctx->h_is_state[slot]) RC(stop_motion(ctx, slot));  // a P/D impurity moves only with its invar
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->h_dyn[slot] = InvarDyn{};
   HIPCHK(hipMemcpy(ctx->d_dyn + slot, &ctx->h_dyn[slot], sizeof(InvarDyn), hipMemcpyHostToDevice));
@@ -708,8 +709,8 @@ int move_field(effpot_ctx* ctx, const int* act) {
 }
 // potenimp of the slots whose invar evolves, with their current invar.
 int dyn_field(effpot_ctx* ctx, const int* act) {
-  hipLaunchKernelGGL(k_dyn_uimp, dim3(256, ctx->B), dim3(256), 0, ctx->st, (const InvarDyn*)ctx->d_dyn, act,
-                     ctx->s64.uimp, ctx->N);
+  hipLaunchKernelGGL(k_dyn_uimp, dim3(256, ctx->B), dim3(256), 0, ctx->st, (const InvarDyn*)ctx->d_dyn,
+                     (const Motion*)ctx->d_motion, act, ctx->s64.uimp, ctx->N);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -717,7 +718,7 @@ int dyn_field(effpot_ctx* ctx, const int* act) {
 int dyn_hiv(effpot_ctx* ctx, const int* act) {
   if (!ctx->d_dynpart) RC(dalloc(ctx, &ctx->d_dynpart, (size_t)ctx->B * kStateBlocks * kMomD));
   hipLaunchKernelGGL(k_dyn_moments, dim3(kStateBlocks, ctx->B), dim3(256), 0, ctx->st, (const InvarDyn*)ctx->d_dyn,
-                     act, (const double*)ctx->s64.den, ctx->N, ctx->d_dynpart);
+                     (const Motion*)ctx->d_motion, act, (const double*)ctx->s64.den, ctx->N, ctx->d_dynpart);
   LAUNCHCHK();
   hipLaunchKernelGGL(k_dyn_hiv, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_dyn, act, ctx->B,
                      (const double*)ctx->d_dynpart, ctx->dxyz);
@@ -1395,15 +1396,19 @@ int effpot_set_impurity_motion(effpot_ctx* ctx, int slot, const double* vimp, do
   if (!(mass_u > 0)) return stop_motion(ctx, slot);
   if (!vimp) return ctx->fail(EFFPOT_ERR_ARG, "null velocity");
   if (ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "a moving impurity needs an FP64 context");
-  if (!ctx->have_imp[slot] || ctx->h_is_state[slot])
-    return ctx->fail(EFFPOT_ERR_STATE, "set an isotropic impurity (effpot_set_impurity) first");
+  if (!ctx->have_imp[slot]) return ctx->fail(EFFPOT_ERR_STATE, "set an impurity first");
+  if (ctx->h_is_state[slot] && !ctx->h_dyn[slot].on)
+    return ctx->fail(EFFPOT_ERR_STATE, "a P/D-state impurity moves with its invar: effpot_set_state_dynamics first");
   hipSetDevice(ctx->dev);
   HIPCHK(hipStreamSynchronize(ctx->st));
   Motion q{};
   q.on = 1;
+  q.iso = ctx->h_is_state[slot] ? 0 : 1;
   q.m = mass_u * 0.0207569277;  // mp_u, modules.f90:333
   const effpot_impurity& im = ctx->h_imp[slot];
-  for (int c = 0; c < 3; c++) { q.r[c] = im.rimp[c]; q.v[c] = vimp[c]; }
+  const StateArgs& sa = ctx->h_sargs[slot];
+  const double r0[3] = {q.iso ? im.rimp[0] : sa.rx, q.iso ? im.rimp[1] : sa.ry, q.iso ? im.rimp[2] : sa.rz};
+  for (int c = 0; c < 3; c++) { q.r[c] = r0[c]; q.v[c] = vimp[c]; }
   for (int k = 0; k < 3; k++) { q.ir[k] = k; q.iv[k] = k; }
   q.ia[0] = 0; q.ia[1] = 1;
   q.ip = ImpParams{im.kind, im.D, im.re, im.a, im.r_cutoff, im.umax, im.cap, im.rimp[0], im.rimp[1], im.rimp[2]};

@@ -17,7 +17,7 @@
 namespace ep {
 
 struct Motion {
-  int32_t on, pad_;
+  int32_t on, iso;  // iso: isotropic field from the slot's table (else a P/D state's)
   double m;  // impurity mass in K^-1 A^-2 (mimpur * mp_u, DFT4He3d.f90:203)
   double r[3], v[3], a[3];
   double rold[3][3], vold[3][3], aold[2][3];  // rimpold, vimpold, aimpold: [storage][xyz]
@@ -114,7 +114,7 @@ __global__ void k_imp_field_moving(const Motion* M, const int* active, int nx, i
                                    double z0, double hx, double hy, double hz, int npot, double rmax, const double* f,
                                    double* uimp, long N) {
   const int s = blockIdx.y;
-  if (!active[s] || !M[s].on) return;
+  if (!active[s] || !M[s].on || !M[s].iso) return;
   const Motion& q = M[s];
   const double* tab = f + (long)s * (npot + 2);
   const double h = rmax / (double)(npot - 1), c1oh = 1.0 / h, c1o2h2 = c1oh * c1oh * 0.5;
@@ -212,13 +212,20 @@ EP_DEV void dyn_moments_body(const StateArgs& a, const double* den, double* part
   }
 }
 // Density moments of every slot whose invar evolves: partial[slot][block][31].
-__global__ __launch_bounds__(256) void k_dyn_moments(const InvarDyn* D, const int* active, const double* den, long N,
-                                                     double* partial) {
+// The slot's state set-up at the impurity's current position (it may move).
+EP_DEV StateArgs dyn_args(const InvarDyn& d, const Motion& m) {
+  StateArgs a = d.sa;
+  if (m.on) { a.rx = m.r[0]; a.ry = m.r[1]; a.rz = m.r[2]; }
+  return a;
+}
+__global__ __launch_bounds__(256) void k_dyn_moments(const InvarDyn* D, const Motion* M, const int* active,
+                                                     const double* den, long N, double* partial) {
   const int s = blockIdx.y;
   if (!active[s] || !D[s].on) return;
   double* p = partial + (long)s * kStateBlocks * kMomD;
-  if (D[s].state == EFFPOT_STATE_P) dyn_moments_body<kMomP>(D[s].sa, den + (long)s * N, p);
-  else dyn_moments_body<kMomD>(D[s].sa, den + (long)s * N, p);
+  const StateArgs a = dyn_args(D[s], M[s]);
+  if (D[s].state == EFFPOT_STATE_P) dyn_moments_body<kMomP>(a, den + (long)s * N, p);
+  else dyn_moments_body<kMomD>(a, den + (long)s * N, p);
 }
 // potinvar: Hinvar = -i (H invar - Ev0 invar) from the reduced moments.
 __global__ void k_dyn_hiv(InvarDyn* D, const int* active, int B, const double* partial, double dxyz) {
@@ -247,10 +254,11 @@ __global__ void k_dyn_hiv(InvarDyn* D, const int* active, int B, const double* p
   }
 }
 // potenimp with the slot's current invar (grid: blocks x slots).
-__global__ __launch_bounds__(256) void k_dyn_uimp(const InvarDyn* D, const int* active, double* uimp, long N) {
+__global__ __launch_bounds__(256) void k_dyn_uimp(const InvarDyn* D, const Motion* M, const int* active,
+                                                  double* uimp, long N) {
   const int s = blockIdx.y;
   if (!active[s] || !D[s].on) return;
-  StateArgs a = D[s].sa;
+  StateArgs a = dyn_args(D[s], M[s]);
   for (int k = 0; k < 20; k++) a.inv[k] = D[s].iv[k];
   double* u = uimp + (long)s * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)

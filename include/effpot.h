@@ -279,7 +279,8 @@ int effpot_instates(int state, const double* moments, double als, int instate, d
  * EFFPOT_PREC_FP64; effpot_solve and effpot_tick refuse it (they relax). */
 int effpot_set_real_time(effpot_ctx* ctx, int on);
 
-/* The slot's isotropic impurity moves classically: r and v advance with the
+/* The slot's impurity moves classically (an isotropic one, or a P/D state
+ * whose invar evolves, effpot_set_state_dynamics): r and v advance with the
  * helium integrators (steprk.f90:117-150, steppc.f90:68-203) under forceimp,
  * F = -sum grad(den) uimp dV (potenimp.f90:193-258), a = F / m with
  * m = mass_u * mp_u (mp_u = 0.0207569277 K^-1 A^-2, modules.f90:333; v in

@@ -1105,9 +1105,10 @@ int oracle_step_rk(oracle_t* o, double dt) {
         }
         o->vimp[c] = o->vimp[c] + dt * a2;
       }
-      if (j <= 2) imp_field_at(o, r);
+      if (j <= 2 && !o->dyn) imp_field_at(o, r);
     }
     if (o->dyn) { /* steprk.f90:152-167 */
+      if (o->moving) memcpy(o->simp.rimp, o->imp.rimp, sizeof(o->simp.rimp));
       double* iv = o->simp.invar;
       for (int k = 0; k < 20; k++) {
         double a2 = arun[j] * (o->hiv[k] - brun[j] * o->qiv[k]);
@@ -1168,9 +1169,10 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
     }
     for (int c = 0; c < 3; c++) o->aimpold[ia[1]][c] = o->aimp[c];
     { int t = ia[1]; ia[1] = ia[0]; ia[0] = t; }
-    imp_field_at(o, r); /* potenimp(rimp) */
+    if (!o->dyn) imp_field_at(o, r); /* potenimp(rimp) */
   }
   if (o->dyn) { /* steppc.f90:84-102 */
+    if (o->moving) memcpy(o->simp.rimp, o->imp.rimp, sizeof(o->simp.rimp));
     double* iv = o->simp.invar;
     int* ii = o->iiv; int* ih = o->ihiv;
     for (int k = 0; k < 20; k++) {

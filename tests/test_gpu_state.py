@@ -177,3 +177,49 @@ def test_state_dynamics_matches_oracle(gpu_available):
         assert np.linalg.norm(got[s][1] - psi) <= 1e-9 * np.linalg.norm(psi)
         for k in ("eso", "eHeX", "eimpu", "etot"):
             assert abs(es[s][k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (s, k, es[s][k], e[k])
+
+
+def test_moving_state_matches_oracle(gpu_available):
+    """A D-state impurity that moves while its invar evolves (real time),
+    against the oracle: r, v, a, invar and the energies after 3 RK + 40 PC."""
+    from test_state_impurity import _series
+    print("[state] moving D state 32^3", flush=True)
+    soft = dict(v_pi=_series(-300.0, 3, 2.0), v_sigma=_series(-600.0, 3, 2.0), v_delta=_series(-150.0, 3, 2.0))
+    p = make_params(32, xmax=12.0, n4=20.0, core=CORE_OTC, h_core=2.377, psi_complex=1)
+    rimp, dt = (0.0, 0.5, 7.0), 5e-5
+    with ep.Context(p) as ctx:
+        ctx.init_fermi(0.0218, 2.22 * 20 ** (1 / 3), 0.6, 1e-6, 9)
+        st = ctx.set_impurity_state(ep.make_state_imp(ep.STATE_D, rimp=rimp, instate=1, als=3.0, **soft))
+        psi0, iv0 = ctx.psi(), st.invar_c()
+        ctx.set_state_dynamics(True)
+        ctx.set_real_time(True)
+        ctx.set_impurity_motion([0.0, 0.0, -2.0], 40.0)
+        for _ in range(3):
+            ctx.step_rk(dt)
+        for _ in range(40):
+            ctx.step_pc(dt)
+        got = ctx.impurity() + (ctx.invar(),)
+        ctx.poten()
+        e_gpu = ctx.energy()[0]
+    o = O.Oracle(make_params(32, xmax=12.0, n4=20.0, core=CORE_OTC, h_core=2.377))
+    o.set_psi(psi0)
+    imp = ep.make_state_imp(ep.STATE_D, rimp=rimp, instate=-1, als=3.0, invar=iv0, **soft)
+    o.set_impurity_state(imp)
+    o.poten()
+    o.set_state_dynamics(imp, ev0=0.0)
+    o.set_real_time(True)
+    o.set_impurity_motion([0.0, 0.0, -2.0], 40.0)
+    for _ in range(3):
+        o.step_rk(dt)
+    for _ in range(40):
+        o.step_pc(dt)
+    want = o.impurity() + (o.invar(),)
+    o.poten()
+    e = o.energy()
+    o.close()
+    print("r", got[0], want[0], "dinvar", np.max(np.abs(got[3] - want[3])))
+    for x, y in zip(got, want):
+        assert np.allclose(x, y, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(y).max()))
+    assert got[0][2] < 7.0
+    for k in ("eso", "eHeX", "ekinx", "eimpu", "etot"):
+        assert abs(e_gpu[k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (k, e_gpu[k], e[k])

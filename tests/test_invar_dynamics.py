@@ -47,3 +47,39 @@ def test_spin_orbit_precession(state):
     e = o.energy()
     assert abs(e["eso"] - O.state_eso(state, got, als)) < 1e-13
     o.close()
+
+
+@pytest.mark.parametrize("state", [ep.STATE_P, ep.STATE_D])
+def test_moving_state_conserves_momentum(state):
+    """A P/D impurity moving in real time while its invar evolves: with soft
+    potentials (-c/r^4 beyond 2 A) the total momentum, helium plus m v, is
+    conserved to 1e-4 of the impulse exchanged, and |invar| stays 1.  (Steep
+    fields near the impurity break it on a 0.75 A grid, as for a steep
+    isotropic potential.)"""
+    from test_real_time import _droplet, _p_fd
+    from test_state_impurity import _series
+    soft = dict(v_pi=_series(-300.0, 3, 2.0), v_sigma=_series(-600.0, 3, 2.0), v_delta=_series(-150.0, 3, 2.0))
+    o, p = _droplet()
+    imp = ep.make_state_imp(state, rimp=(0.0, 0.0, 7.5), instate=0, als=3.0, **soft)
+    w, V = np.linalg.eigh(O.state_matrix(state, o.state_moments(imp), 3.0))
+    inv = (V[:, 0] + V[:, 1]) / np.sqrt(2.0)
+    imp = ep.make_state_imp(state, rimp=(0.0, 0.0, 7.5), instate=-1, als=3.0, invar=inv, **soft)
+    o.set_impurity_state(imp)
+    o.poten()
+    o.set_state_dynamics(imp, ev0=0.0)
+    o.set_real_time(True)
+    o.set_impurity_motion([0.0, 0.0, -2.0], 40.0)
+    m = 40.0 * 0.0207569277
+    _, v0, _ = o.impurity()
+    P0 = _p_fd(o, o.psi(), 0.75) + m * v0
+    for _ in range(3):
+        o.step_rk(5e-5)
+    for _ in range(300):
+        o.step_pc(5e-5)
+    _, v, _ = o.impurity()
+    impulse = m * (v - v0)
+    dP = _p_fd(o, o.psi(), 0.75) + m * v - P0
+    assert abs(impulse[2]) > 0.05
+    assert np.max(np.abs(dP)) <= 1e-4 * abs(impulse[2]), (dP, impulse)
+    assert abs(np.linalg.norm(o.invar()) - 1.0) < 1e-8
+    o.close()
