@@ -49,11 +49,18 @@ module effpot_c
   end type
 
   integer(c_int), parameter :: EFFPOT_STATE_P = 1, EFFPOT_STATE_D = 2
+  integer(c_int), parameter :: EFFPOT_SELECT_PAIR = 0, EFFPOT_SELECT_JZ = 1, EFFPOT_SELECT_FIX = 2, &
+                               EFFPOT_SELECT_AVERAGE = 3, EFFPOT_SELECT_MIXTURE = 4
+  type, bind(c) :: effpot_select
+    integer(c_int32_t) :: mode, jz2, fixed_moments, pad_
+    real(c_double) :: a_pi_3o2, a_sig_1o2
+  end type
   type, bind(c) :: effpot_state_imp
     integer(c_int32_t) :: state, instate
     real(c_double) :: als, rimp(3)
     type(effpot_plus) :: v_pi, v_sigma, v_delta
     real(c_double) :: invar(20), eigen(10), moments(31), eso
+    type(effpot_select) :: select
   end type
 
   type, bind(c) :: effpot_result
@@ -172,6 +179,15 @@ module effpot_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: slot
       type(effpot_state_imp), intent(inout) :: imp
+    end function
+    ! Lexciplex_state_fix: P moments of a Ring (6 He) or Linear (2 He) exciplex
+    integer(c_int) function effpot_exciplex_moments(v_pi, v_sigma, n_he, axis, r_exc, moments) &
+        bind(c, name="effpot_exciplex_moments")
+      import :: c_int, c_double, effpot_plus
+      type(effpot_plus), intent(in) :: v_pi, v_sigma
+      integer(c_int), value :: n_he, axis
+      real(c_double), value :: r_exc
+      real(c_double), intent(out) :: moments(7)
     end function
     ! readenc mode 3 (formatted or unformatted file); psi = c_null_ptr reads the header only
     integer(c_int) function effpot_snapshot_read(path, hdr, psi) bind(c, name="effpot_snapshot_read")

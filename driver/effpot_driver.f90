@@ -31,6 +31,13 @@ program effpot_driver
   real(8) :: r_cutoff_plus = 1.05d0, umax_plus = 6.260015d4
   real(8) :: r_cutoff_sigma = 1.05d0, umax_sigma = 9.674836d4, r_cutoff_delta = 1.05d0, umax_delta = 6.260015d4
   real(8) :: als_p = 0.d0, als_d = 0.d0
+  ! instates' choice of invar (modules.f90:284-318)
+  logical :: ldiag_jz = .false., lexcite_state_fix = .false., laverage_p_value = .false., lmixture = .false.
+  logical :: lexciplex_state_fix = .false.
+  character(len=4) :: ljz = ''
+  character(len=6) :: exciplex = 'Ring'
+  character(len=1) :: lexciplex_axis = 'Z'
+  real(8) :: a_pi_3o2 = 0.94d0, a_sig_1o2 = 0.34117444d0, r_exc = 3.9d0
   real(8) :: tol = 1.d-9
   logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
   real(8), parameter :: ktops = 7.63822291d0
@@ -39,7 +46,9 @@ program effpot_driver
                    cp4, cpp4, den4c, alphas, den0s, h2o2m4, deltat, deltatps, ximp, yimp, zimp, &
                    morse_d, morse_re, morse_a, r_cutoff_pi, umax_pi, afermi, rho0, noise, tol, ninvar, &
                    fast_io, selec_plus, r_cutoff_plus, umax_plus, selec_sigma, selec_delta, lstate, &
-                   instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d
+                   instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d, &
+                   ldiag_jz, ljz, lexcite_state_fix, laverage_p_value, lmixture, a_pi_3o2, a_sig_1o2, &
+                   lexciplex_state_fix, exciplex, lexciplex_axis, r_exc
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
@@ -52,7 +61,7 @@ program effpot_driver
   complex(8), allocatable, target :: psi(:,:,:)
   type(effpot_snapshot) :: snap
   real(8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, rimp(3), vimp(3), hx, hy, hz
-  integer :: nxp, nyp, nzp, nninvar0, isalto
+  integer :: nxp, nyp, nzp, nninvar0, isalto, islash, iaxis
   complex(8), allocatable :: invar0(:)
   real(8) :: dist
 
@@ -142,6 +151,38 @@ program effpot_driver
     call preset(selec_pi, r_cutoff_pi, umax_pi, simp%v_pi)
     call preset(selec_sigma, r_cutoff_sigma, umax_sigma, simp%v_sigma)
     call preset(selec_delta, r_cutoff_delta, umax_delta, simp%v_delta)
+    ! the branches of instates in their order of precedence (instates.f90:228-364, :613-726)
+    simp%select%mode = EFFPOT_SELECT_PAIR; simp%select%jz2 = 0; simp%select%fixed_moments = 0
+    simp%select%pad_ = 0; simp%select%a_pi_3o2 = a_pi_3o2; simp%select%a_sig_1o2 = a_sig_1o2
+    if (ldiag_jz) then
+      simp%select%mode = EFFPOT_SELECT_JZ
+      if (len_trim(ljz) > 0) then
+        islash = index(ljz, '/2')
+        if (islash < 2) then
+          write(6, '("Wrong Ljz value...:",A)') ljz
+          stop 'Wrong Ljz value..'
+        end if
+        read(ljz(1:islash-1), *) simp%select%jz2
+      end if
+    else if (lstate == 'P' .and. lexcite_state_fix) then
+      simp%select%mode = EFFPOT_SELECT_FIX
+    else if (lstate == 'P' .and. laverage_p_value) then
+      simp%select%mode = EFFPOT_SELECT_AVERAGE
+    else if (lstate == 'P' .and. lmixture) then
+      simp%select%mode = EFFPOT_SELECT_MIXTURE
+    end if
+    simp%moments = 0.d0
+    if (lstate == 'P' .and. lexciplex_state_fix) then   ! instates.f90:54-113
+      iaxis = index('XYZ', lexciplex_axis) - 1
+      if (iaxis < 0) stop 'Instates ---> Iaxis = 0'
+      if (trim(exciplex) /= 'Ring' .and. trim(exciplex) /= 'Linear') then
+        write(6, '("From Instates: Exciplex variable is not correct...",A)') exciplex
+        stop 'From instates  001'
+      end if
+      call check(ctx, effpot_exciplex_moments(simp%v_pi, simp%v_sigma, merge(6_c_int, 2_c_int, &
+                 trim(exciplex) == 'Ring'), int(iaxis, c_int), r_exc, simp%moments(1:7)), 'effpot_exciplex_moments')
+      simp%select%fixed_moments = 1
+    end if
     call check(ctx, effpot_set_impurity_state(ctx, 0_c_int, simp), 'effpot_set_impurity_state')
   end if
   call preset(selec_plus, r_cutoff_plus, umax_plus, plus)

@@ -15,7 +15,8 @@ import numpy as np
 from .abi import (Params, Impurity, Plus, StateImp, STATE_P, STATE_D, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
                   SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
-                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, H4OT)
+                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, H4OT, ERR_ARG,
+                  Select, SELECT_PAIR, SELECT_JZ, SELECT_FIX, SELECT_AVERAGE, SELECT_MIXTURE)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeffpot.so")
@@ -78,6 +79,9 @@ def lib():
         L.effpot_get_invar.argtypes = [vp, C.c_int, _dp]
         L.effpot_set_impurity_state.argtypes = [vp, C.c_int, C.POINTER(StateImp)]
         L.effpot_instates.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
+        L.effpot_instates_select.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p, _dp, _dp,
+                                             C.POINTER(C.c_double)]
+        L.effpot_exciplex_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]
         L.effpot_plus_preset.argtypes = [C.c_char_p, C.c_double, C.c_double, C.POINTER(Plus)]
         L.effpot_plus_eval.argtypes = [C.POINTER(Plus), C.c_double]
         L.effpot_plus_eval.restype = C.c_double
@@ -130,11 +134,32 @@ def plus_eval(plus: Plus, r) -> float:
     return lib().effpot_plus_eval(C.byref(plus), float(r))
 
 
+def make_select(mode=SELECT_PAIR, ljz="", a_pi_3o2=0.94, a_sig_1o2=0.34117444, fixed_moments=False) -> Select:
+    """Selection of invar: `ljz` is the reference's Ljz string ('-3/2', '+1/2',
+    ... or '' for its default); the amplitudes default to modules.f90:305."""
+    if ljz.strip():
+        num, _, den = ljz.strip().partition("/")
+        if den != "2" or not num.lstrip("+-").isdigit():
+            raise EffpotError(ERR_ARG, f"wrong Ljz value: {ljz!r}")
+        jz2 = int(num)
+    else:
+        jz2 = 0
+    return Select(mode=mode, jz2=jz2, fixed_moments=int(bool(fixed_moments)), a_pi_3o2=a_pi_3o2,
+                  a_sig_1o2=a_sig_1o2)
+
+
 def make_state_imp(state, rimp=(0.0, 0.0, 0.0), instate=0, als=0.0, v_pi=None, v_sigma=None, v_delta=None,
-                   invar=None) -> StateImp:
+                   invar=None, select=None, moments=None) -> StateImp:
     """A P or D internal-state impurity; the potentials default to the
-    reference's selec_pi / selec_sigma / selec_delta (V_impur.f90:5-13)."""
+    reference's selec_pi / selec_sigma / selec_delta (V_impur.f90:5-13).
+    `select` (make_select) chooses invar; `moments` are kept as given when
+    select.fixed_moments (an exciplex, exciplex_moments)."""
     imp = StateImp(state=state, instate=instate, als=als)
+    if select is not None:
+        imp.select = select
+    if moments is not None:
+        for i, m in enumerate(moments):
+            imp.moments[i] = m
     for i in range(3):
         imp.rimp[i] = rimp[i]
     imp.v_pi = v_pi if v_pi is not None else make_plus("Ba_plus_2D1", 1.05, 5.216077e4)
@@ -146,17 +171,34 @@ def make_state_imp(state, rimp=(0.0, 0.0, 0.0), instate=0, als=0.0, v_pi=None, v
     return imp
 
 
-def instates(state, moments, als, instate=0):
+def instates(state, moments, als, instate=0, select=None):
     """Host half of instates: (eigenvalues, invar, eso) of V + (Als/2) SO
-    built from the density moments (effpot_instates)."""
+    built from the density moments (effpot_instates_select; `select` from
+    make_select, None for the Kramers-pair default)."""
     n = 6 if state == STATE_P else 10
     mom = np.zeros(31)
     mom[:len(moments)] = moments
     inv, eig, eso = np.zeros(20), np.zeros(10), C.c_double()
-    rc = lib().effpot_instates(state, mom, als, instate, inv, eig, C.byref(eso))
+    rc = lib().effpot_instates_select(state, mom, als, instate, C.byref(select) if select is not None else None,
+                                      inv, eig, C.byref(eso))
     if rc != OK:
-        raise EffpotError(rc, "effpot_instates: bad arguments")
+        raise EffpotError(rc, "effpot_instates_select: bad arguments (instate, mode, Ljz or amplitudes)")
     return eig[:n], inv[:2 * n:2] + 1j * inv[1:2 * n:2], eso.value
+
+
+def exciplex_moments(v_pi, v_sigma, exciplex="Ring", axis="Z", r_exc=3.9):
+    """Lexciplex_state_fix: the P moments of a 'Ring' (6 He) or 'Linear' (2 He)
+    exciplex at r_exc about `axis` (effpot_exciplex_moments; defaults
+    modules.f90:288-289, 304)."""
+    n_he = {"Ring": 6, "Linear": 2}.get(exciplex.strip())
+    ax = {"X": 0, "Y": 1, "Z": 2}.get(axis.strip().upper())
+    if n_he is None or ax is None:
+        raise EffpotError(ERR_ARG, f"exciplex must be Ring or Linear and axis X, Y or Z: {exciplex!r} {axis!r}")
+    m = np.zeros(7)
+    rc = lib().effpot_exciplex_moments(C.byref(v_pi), C.byref(v_sigma), n_he, ax, r_exc, m)
+    if rc != OK:
+        raise EffpotError(rc, "effpot_exciplex_moments: bad arguments")
+    return m
 
 
 def footprint_bytes(params: Params) -> int:

@@ -48,13 +48,21 @@ class Plus(C.Structure):
 
 
 STATE_P, STATE_D = 1, 2
+SELECT_PAIR, SELECT_JZ, SELECT_FIX, SELECT_AVERAGE, SELECT_MIXTURE = 0, 1, 2, 3, 4
+
+
+class Select(C.Structure):
+    """How instates picks invar (effpot_select)."""
+    _fields_ = [("mode", C.c_int32), ("jz2", C.c_int32), ("fixed_moments", C.c_int32), ("pad_", C.c_int32),
+                ("a_pi_3o2", C.c_double), ("a_sig_1o2", C.c_double)]
 
 
 class StateImp(C.Structure):
     """Impurity in a P or D internal state (effpot_state_imp)."""
     _fields_ = [("state", C.c_int32), ("instate", C.c_int32), ("als", C.c_double), ("rimp", C.c_double * 3),
                 ("v_pi", Plus), ("v_sigma", Plus), ("v_delta", Plus), ("invar", C.c_double * 20),
-                ("eigen", C.c_double * 10), ("moments", C.c_double * 31), ("eso", C.c_double)]
+                ("eigen", C.c_double * 10), ("moments", C.c_double * 31), ("eso", C.c_double),
+                ("select", Select)]
 
     def invar_c(self):
         n = 6 if self.state == STATE_P else 10

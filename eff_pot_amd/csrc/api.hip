@@ -1226,18 +1226,66 @@ int effpot_set_impurity(effpot_ctx* ctx, int slot, const effpot_impurity* imp) {
   return stop_motion(ctx, slot);
 }
 
-int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
-                    double* eso) {
+int effpot_instates_select(int state, const double* moments, double als, int instate, const effpot_select* sel,
+                           double* invar, double* eigen, double* eso) {
   if (state != EFFPOT_STATE_P && state != EFFPOT_STATE_D) return EFFPOT_ERR_ARG;
   const int n = state == EFFPOT_STATE_P ? 6 : 10;
   if (!moments || instate >= n / 2 || instate < -1) return EFFPOT_ERR_ARG;
+  const int mode = sel ? sel->mode : EFFPOT_SELECT_PAIR;
+  if (mode < EFFPOT_SELECT_PAIR || mode > EFFPOT_SELECT_MIXTURE) return EFFPOT_ERR_ARG;
+  if (mode >= EFFPOT_SELECT_FIX && state != EFFPOT_STATE_P) return EFFPOT_ERR_ARG;
+  if (mode == EFFPOT_SELECT_MIXTURE && !(sel->a_pi_3o2 >= 0.0 && sel->a_sig_1o2 >= 0.0)) return EFFPOT_ERR_ARG;
   zd H[100], V[100];
   double w[10];
   state_matrix(state, moments, als, H);
   herm_eigen(n, H, w, V);
   if (eigen) std::memcpy(eigen, w, sizeof(double) * n);
-  if (instate >= 0 && invar) state_select(n, V, instate, invar);
+  if (instate >= 0 && invar) {
+    switch (mode) {
+      case EFFPOT_SELECT_JZ: RC(state_select_jz(state, n, V, instate, sel->jz2, invar)); break;
+      case EFFPOT_SELECT_FIX: state_select_fix(instate, invar); break;
+      case EFFPOT_SELECT_AVERAGE: state_select_average(n, V, invar); break;
+      case EFFPOT_SELECT_MIXTURE: state_select_mixture(n, V, sel->a_pi_3o2, sel->a_sig_1o2, invar); break;
+      default: state_select(n, V, instate, invar);
+    }
+  }
   if (eso && invar) *eso = state_eso(state, invar, als);
+  return EFFPOT_OK;
+}
+
+int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
+                    double* eso) {
+  return effpot_instates_select(state, moments, als, instate, nullptr, invar, eigen, eso);
+}
+
+int effpot_exciplex_moments(const effpot_plus* v_pi, const effpot_plus* v_sigma, int n_he, int axis, double r_exc,
+                            double* moments) {
+  if (!v_pi || !v_sigma || !moments || (n_he != 2 && n_he != 6) || axis < 0 || axis > 2 || !(r_exc > 0.0))
+    return EFFPOT_ERR_ARG;
+  for (const effpot_plus* v : {v_pi, v_sigma})
+    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES) ||
+        (v->kind == EFFPOT_POT_SERIES && (v->nterms < 1 || v->nterms > EFFPOT_SERIES_MAX || v->k0 < 0)))
+      return EFFPOT_ERR_POT;
+  // the two axes of the ring plane, lower first (instates.f90:72-81)
+  const int ax = axis == 0 ? 1 : 0, ay = axis == 2 ? 1 : 2;
+  double m[7] = {};
+  for (int i = 0; i < n_he; i++) {
+    double r3[3] = {0.0, 0.0, 0.0};
+    if (n_he == 2) {
+      r3[axis] = i == 0 ? r_exc : -r_exc;
+    } else {
+      const double t = (double)i * (2.0 * M_PI / (double)n_he);
+      r3[ax] = r_exc * std::cos(t);
+      r3[ay] = r_exc * std::sin(t);
+    }
+    const double r2 = r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2], r = std::sqrt(r2);
+    const double vpi = plus_pot(*v_pi, r), dv = plus_pot(*v_sigma, r) - vpi;
+    for (int c = 0; c < 3; c++) m[1 + c] += vpi + dv * (r3[c] / r) * (r3[c] / r);
+    m[4] += dv * r3[0] * r3[1] / r2;
+    m[5] += dv * r3[0] * r3[2] / r2;
+    m[6] += dv * r3[1] * r3[2] / r2;
+  }
+  std::memcpy(moments, m, sizeof(m));
   return EFFPOT_OK;
 }
 
@@ -1281,8 +1329,10 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
   a.tab = tab; a.umax_pi = imp->v_pi.umax; a.umax_sigma = imp->v_sigma.umax;
   a.rx = imp->rimp[0]; a.ry = imp->rimp[1]; a.rz = imp->rimp[2];
   a.cap = 1000.0;  // potenimp.f90:184-186
-  std::memset(imp->moments, 0, sizeof(imp->moments));
-  if (imp->instate >= 0) {  // instates on the slot's current density
+  const bool fixed = imp->select.fixed_moments != 0;
+  if (fixed && st != EFFPOT_STATE_P) return ctx->fail(EFFPOT_ERR_ARG, "fixed moments (exciplex) are for P states");
+  if (!fixed) std::memset(imp->moments, 0, sizeof(imp->moments));
+  if (imp->instate >= 0 && !fixed) {  // instates on the slot's current density
     const int K = st == EFFPOT_STATE_P ? kMomP : kMomD;
     // the slot's current psi buffer (ring roles live on the device)
     std::vector<int> ring(8);
@@ -1308,7 +1358,11 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
       imp->moments[k] = acc * ctx->dxyz;
     }
   }
-  RC(effpot_instates(st, imp->moments, imp->als, imp->instate, imp->invar, imp->eigen, &imp->eso));
+  {
+    const int rc = effpot_instates_select(st, imp->moments, imp->als, imp->instate, &imp->select, imp->invar,
+                                          imp->eigen, &imp->eso);
+    if (rc != EFFPOT_OK) return ctx->fail(rc, "bad invar selection (mode, Ljz or mixture amplitudes)");
+  }
   if (imp->instate < 0) imp->eso = state_eso(st, imp->invar, imp->als);
   std::memcpy(a.inv, imp->invar, sizeof(a.inv));
   hipLaunchKernelGGL((k_state_uimp<double>), dim3(1024), dim3(256), 0, ctx->st, a, ctx->s64.uimp + (long)slot * ctx->N);

@@ -341,4 +341,103 @@ inline void state_select(int n, const zd* V, int instate, double* inv) {
   }
 }
 
+// Jz = Lz + Sz in the instates basis (even 0-based components spin up).  P:
+// orbitals (x, y, z) with <x|Lz|y> = -i (instates.f90:239-248); D: orbitals
+// 1..5 with Lz coupling 1 and 5 (m = +-2, weight 2) and 2 and 4 (m = +-1),
+// orbital 3 m = 0 (instates.f90:663-680).  Row-major n x n.
+inline void state_jz(int state, zd* J) {
+  const int n = state == EFFPOT_STATE_P ? 6 : 10;
+  for (int i = 0; i < n * n; i++) J[i] = 0.0;
+  for (int i = 0; i < n; i++) J[i * n + i] = (i % 2 == 0) ? 0.5 : -0.5;
+  auto lz = [&](int a, int b, double w) {  // orbitals a, b (0-based): <a|Lz|b> = -i w
+    for (int s = 0; s < 2; s++) {
+      J[(2 * a + s) * n + 2 * b + s] = zd(0.0, -w);
+      J[(2 * b + s) * n + 2 * a + s] = zd(0.0, w);
+    }
+  };
+  if (state == EFFPOT_STATE_P) {
+    lz(0, 1, 1.0);
+  } else {
+    lz(4, 0, 2.0);
+    lz(1, 3, -1.0);
+  }
+}
+
+// The Jz manifold of Ldiag_jz: first eigenvector (0-based) and count, and
+// the reference's default Ljz (as 2 jz) for `instate` (instates.f90:229-234,
+// 259-274 for P; 614-661 for D).
+inline void jz_manifold(int state, int instate, int* k0, int* nj, int* jz2_default) {
+  if (state == EFFPOT_STATE_P) {
+    if (instate == 0) { *k0 = 0; *nj = 2; *jz2_default = -1; }
+    else { *k0 = 2; *nj = 4; *jz2_default = instate == 1 ? -3 : -1; }
+  } else {
+    if (instate <= 1) { *k0 = 0; *nj = 4; *jz2_default = instate == 0 ? -3 : -1; }
+    else { *k0 = 4; *nj = 6; *jz2_default = instate == 2 ? -5 : (instate == 3 ? -3 : -1); }
+  }
+}
+
+// Ldiag_jz: diagonalise Jz within the eigenvectors k0..k0+nj-1 of V + SO
+// (columns of V) and return the eigenvector of the (jz2 + nj - 1)/2-th Jz
+// eigenvalue (ascending), expressed in the instates basis.  EFFPOT_ERR_ARG
+// for an Ljz outside the manifold (the reference stops there).
+inline int state_select_jz(int state, int n, const zd* V, int instate, int jz2, double* inv) {
+  int k0, nj, jdef;
+  jz_manifold(state, instate, &k0, &nj, &jdef);
+  if (jz2 == 0) jz2 = jdef;
+  if ((jz2 & 1) == 0 || jz2 < -(nj - 1) || jz2 > nj - 1) return EFFPOT_ERR_ARG;
+  const int il = (jz2 + nj - 1) / 2;
+  zd J[100], M[36], W[36];
+  double w[10];
+  state_jz(state, J);
+  for (int k = 0; k < nj; k++)
+    for (int l = 0; l < nj; l++) {
+      zd acc = 0.0;
+      for (int a = 0; a < n; a++) {
+        zd jv = 0.0;
+        for (int b = 0; b < n; b++) jv += J[a * n + b] * V[b * n + k0 + l];
+        acc += std::conj(V[a * n + k0 + k]) * jv;
+      }
+      M[k * nj + l] = acc;
+    }
+  herm_eigen(nj, M, w, W);
+  std::memset(inv, 0, sizeof(double) * 20);
+  for (int i = 0; i < n; i++) {
+    zd v = 0.0;
+    for (int k = 0; k < nj; k++) v += W[k * nj + il] * V[i * n + k0 + k];
+    inv[2 * i] = v.real();
+    inv[2 * i + 1] = v.imag();
+  }
+  return EFFPOT_OK;
+}
+
+// Lexcite_state_fix (instates.f90:301-327): a fixed spin-up P orbital.
+inline void state_select_fix(int instate, double* inv) {
+  std::memset(inv, 0, sizeof(double) * 20);
+  const int m = instate >= 2 ? 3 : instate + 1;
+  for (int o = 0; o < m; o++) inv[4 * o] = 1.0 / std::sqrt((double)m);
+}
+
+// Laverage_P_value (:328-335) and Lmixture (:336-358) for P.
+inline void state_select_average(int n, const zd* V, double* inv) {
+  std::memset(inv, 0, sizeof(double) * 20);
+  const double c = 1.0 / std::sqrt((double)n);
+  for (int i = 0; i < n; i++) {
+    zd v = 0.0;
+    for (int k = 0; k < n; k++) v += c * V[i * n + k];
+    inv[2 * i] = v.real();
+    inv[2 * i + 1] = v.imag();
+  }
+}
+inline void state_select_mixture(int n, const zd* V, double a_pi, double a_sig, double* inv) {
+  const double c = 1.0 / std::sqrt(2.0);
+  const double amp[3] = {std::sqrt(std::max(1.0 - a_pi - a_sig, 0.0)), std::sqrt(a_pi), std::sqrt(a_sig)};
+  std::memset(inv, 0, sizeof(double) * 20);
+  for (int i = 0; i < n; i++) {
+    zd v = 0.0;
+    for (int p = 0; p < 3; p++) v += amp[p] * (c * V[i * n + 2 * p] + c * V[i * n + 2 * p + 1]);
+    inv[2 * i] = v.real();
+    inv[2 * i + 1] = v.imag();
+  }
+}
+
 }  // namespace ep

@@ -25,7 +25,11 @@ from collections import OrderedDict
 import math
 
 from . import (Context, make_params, make_morse, make_plus, make_state_imp, read_snapshot, StateImp, STATE_P,
-               STATE_D, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO, H4OT, PLUS_DEFAULT)
+               STATE_D, CORE_OTC, CORE_OT, PREC_FP64, ICON_ZERO, H4OT, PLUS_DEFAULT, make_select, exciplex_moments,
+               SELECT_PAIR, SELECT_JZ, SELECT_FIX, SELECT_AVERAGE, SELECT_MIXTURE)
+
+SELECT_MODES = {"pair": SELECT_PAIR, "jz": SELECT_JZ, "fix": SELECT_FIX, "average": SELECT_AVERAGE,
+                "mixture": SELECT_MIXTURE}
 
 
 def _grid_key(h):
@@ -129,6 +133,16 @@ def main(argv=None):
                     help="P/D internal-state impurity (Ba+ fits of selec_pi/sigma/delta) instead of --morse")
     ap.add_argument("--instate", type=int, default=0)
     ap.add_argument("--als", type=float, default=0.0, help="spin-orbit splitting Als_P / Als_D (K)")
+    ap.add_argument("--select", choices=list(SELECT_MODES), default="pair",
+                    help="instates' choice of invar: pair (default), jz (Ldiag_jz), fix (Lexcite_state_fix), "
+                         "average (Laverage_P_value), mixture (Lmixture)")
+    ap.add_argument("--ljz", default="", help="Ljz of --select jz ('-3/2', '+1/2', ...; '' = default)")
+    ap.add_argument("--a-pi-3o2", type=float, default=0.94)
+    ap.add_argument("--a-sig-1o2", type=float, default=0.34117444)
+    ap.add_argument("--exciplex", choices=["Ring", "Linear"], default=None,
+                    help="Lexciplex_state_fix: P moments of a fixed exciplex instead of the density")
+    ap.add_argument("--exciplex-axis", choices=["X", "Y", "Z"], default="Z")
+    ap.add_argument("--r-exc", type=float, default=3.9)
     ap.add_argument("--core", choices=["OT", "OTC"], default="OTC")
     ap.add_argument("--h-core", type=float, default=2.377)
     ap.add_argument("--batch", type=int, default=8)
@@ -138,7 +152,14 @@ def main(argv=None):
     plus = make_plus(a.selec_plus, a.r_cutoff_plus, a.umax_plus)
     state = None
     if a.lstate:
-        state = make_state_imp(STATE_P if a.lstate == "P" else STATE_D, instate=a.instate, als=a.als)
+        sel = make_select(SELECT_MODES[a.select], a.ljz, a.a_pi_3o2, a.a_sig_1o2, fixed_moments=bool(a.exciplex))
+        state = make_state_imp(STATE_P if a.lstate == "P" else STATE_D, instate=a.instate, als=a.als, select=sel)
+        if a.exciplex:
+            if a.lstate != "P":
+                ap.error("--exciplex needs --lstate P")
+            m = exciplex_moments(state.v_pi, state.v_sigma, a.exciplex, a.exciplex_axis, a.r_exc)
+            for i in range(7):
+                state.moments[i] = m[i]
     res = evaluate(a.files, imp, core=CORE_OTC if a.core == "OTC" else CORE_OT,
                    h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus, state=state)
     for d in res:

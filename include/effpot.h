@@ -125,11 +125,41 @@ typedef struct effpot_plus {
  * r <= 1e-3 A (potenimp.f90:536-563).  Fields marked "out" are written. */
 #define EFFPOT_STATE_P 1   /* 6 components: (x,y,z) orbital x (up,down) spin  */
 #define EFFPOT_STATE_D 2   /* 10 components: d1..d5 x (up,down)               */
+
+/* How instates picks invar among the eigenvectors of V + SO (the branches
+ * of instates.f90:228-364 for P and :613-726 for D, in the reference's order
+ * of precedence Ldiag_jz > Lexcite_state_fix > Laverage_P_value > Lmixture).
+ * FIX, AVERAGE and MIXTURE exist for P only. */
+#define EFFPOT_SELECT_PAIR 0    /* (v_2k+1 + v_2k+2)/sqrt2, k = instate (default) */
+#define EFFPOT_SELECT_JZ 1      /* Ldiag_jz: the eigenvector of Jz = Lz + Sz within
+                                   the j manifold of `instate` (P: 0 -> j=1/2,
+                                   1,2 -> j=3/2; D: 0,1 -> j=3/2, 2..4 -> j=5/2)
+                                   whose eigenvalue is the jz2/2-th (Ljz)       */
+#define EFFPOT_SELECT_FIX 2     /* Lexcite_state_fix: |x up>, (|x up>+|y up>)/sqrt2
+                                   or (|x up>+|y up>+|z up>)/sqrt3 for instate
+                                   0, 1, >= 2; no diagonalisation              */
+#define EFFPOT_SELECT_AVERAGE 3 /* Laverage_P_value: sum of the six eigenvectors
+                                   over sqrt6                                  */
+#define EFFPOT_SELECT_MIXTURE 4 /* Lmixture: sqrt(a_pi_3o2) pair 1 + sqrt(a_sig_1o2)
+                                   pair 2 + sqrt(max(1-a_pi_3o2-a_sig_1o2,0))
+                                   pair 0, each pair as in PAIR                */
+typedef struct effpot_select {
+  int32_t mode;        /* EFFPOT_SELECT_*; 0 (a zeroed struct) is the default */
+  int32_t jz2;         /* JZ: twice Ljz (-5..5, odd); 0 = the reference's
+                          default for `instate` (modules.f90:310-318)      */
+  int32_t fixed_moments; /* 1: `moments` is input, not summed over the
+                          density (Lexciplex_state_fix, instates.f90:54-113,
+                          with effpot_exciplex_moments)                     */
+  int32_t pad_;
+  double a_pi_3o2, a_sig_1o2; /* MIXTURE amplitudes (>= 0)                 */
+} effpot_select;
+
 typedef struct effpot_state_imp {
   int32_t state;       /* EFFPOT_STATE_P | EFFPOT_STATE_D                   */
-  int32_t instate;     /* >= 0: the equal-weight sum of Kramers pair `instate`
-                          (0 = lowest) of V + SO (instates.f90:360-364, :722-726);
-                          -1: invar as given (readenc instate 5)          */
+  int32_t instate;     /* >= 0: the level (Kramers pair, 0 = lowest) or j
+                          manifold that `select` picks from (instates.f90:
+                          360-364, :722-726); -1: invar as given (readenc
+                          instate 5)                                       */
   double als;          /* spin-orbit splitting Als_P / Als_D (K)            */
   double rimp[3];      /* impurity position (A)                            */
   effpot_plus v_pi, v_sigma, v_delta; /* v_delta is unused for P           */
@@ -137,8 +167,10 @@ typedef struct effpot_state_imp {
   double eigen[10];    /* out: eigenvalues of V + SO, ascending (K)         */
   double moments[31];  /* out: density moments of instates (P: A, Vx, Vy, Vz,
                           Vxy, Vxz, Vyz; D: MP(i<=j), MS(i<=j) row-major,
-                          then the Delta sum), times dxyz                  */
+                          then the Delta sum), times dxyz; input when
+                          select.fixed_moments                              */
   double eso;          /* out: spin-orbit energy (energy.f90:170-192)       */
+  effpot_select select; /* how invar is chosen (zeroed: PAIR)               */
 } effpot_state_imp;
 
 /* Energies as printed by DFT4He3d.f90:460-463 (formats 6050/6060) and
@@ -271,6 +303,25 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp);
  * instate >= 0, the selected invar and its eso.  Host only. */
 int effpot_instates(int state, const double* moments, double als, int instate, double* invar, double* eigen,
                     double* eso);
+
+/* effpot_instates with the selection `sel` (NULL: PAIR).  JZ returns an
+ * invar fixed up to a global phase (the reference's eigensolver fixes that
+ * phase its own way; no observable depends on it).  AVERAGE and MIXTURE add
+ * eigenvectors of different levels, so their invar and the field built from
+ * it depend on the eigensolver's phase and, within a Kramers pair, basis
+ * conventions: this library's are those of its Jacobi solver.  Host only. */
+int effpot_instates_select(int state, const double* moments, double als, int instate, const effpot_select* sel,
+                           double* invar, double* eigen, double* eso);
+
+/* Lexciplex_state_fix (instates.f90:54-113): the P-state moments of n_he He
+ * atoms at fixed positions about the impurity instead of the density:
+ * n_he = 2 ('Linear'): at +-r_exc on axis `axis` (0 x, 1 y, 2 z);
+ * n_he = 6 ('Ring'): on the circle of radius r_exc normal to `axis`, the
+ * first atom on the lower remaining axis.  Writes moments[0..6] (moment 0,
+ * the atom count, is 0 as in the reference) for effpot_state_imp with
+ * select.fixed_moments = 1.  Host only. */
+int effpot_exciplex_moments(const effpot_plus* v_pi, const effpot_plus* v_sigma, int n_he, int axis, double r_exc,
+                            double* moments);
 
 /* ---- real-time propagation and a moving impurity ------------------------
  * Real time (timec = i; steppc.f90:53, steprk.f90:304-305): effpot_step_rk /

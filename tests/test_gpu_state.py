@@ -94,9 +94,22 @@ def test_state_relaxation_matches_oracle(gpu_available):
         assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), (k, a, b)
 
 
-def test_fortran_driver_p_state(gpu_available, tmp_path):
-    """The driver's Lstate/instate/selec_* keys reach the same P-state
-    impurity as the Python binding (Fermi start, one energy evaluation)."""
+DRIVER_SELECT = [  # (namelist keys, make_select arguments, exciplex)
+    ("", {}, None),
+    (" ldiag_jz=.true., ljz='+1/2',", dict(mode=ep.SELECT_JZ, ljz="+1/2"), None),
+    (" lmixture=.true., a_pi_3o2=0.5d0, a_sig_1o2=0.25d0,",
+     dict(mode=ep.SELECT_MIXTURE, a_pi_3o2=0.5, a_sig_1o2=0.25), None),
+    (" lexciplex_state_fix=.true., exciplex='Linear', lexciplex_axis='X', r_exc=3.5d0,",
+     dict(fixed_moments=True), ("Linear", "X", 3.5)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(DRIVER_SELECT)))
+def test_fortran_driver_p_state(gpu_available, tmp_path, case):
+    """The driver's Lstate/instate/selec_* keys, and the keys of instates'
+    other choices of invar, reach the same P-state impurity as the Python
+    binding (Fermi start, one energy evaluation)."""
+    keys, sel, exc = DRIVER_SELECT[case]
     import os
     import re
     import subprocess
@@ -105,7 +118,11 @@ def test_fortran_driver_p_state(gpu_available, tmp_path):
         pytest.skip("Fortran driver not built")
     print("[state] driver P state", flush=True)
     p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
-    imp = ep.make_state_imp(ep.STATE_P, rimp=(0.0, 0.75, 1.5), instate=1, als=3.0)
+    imp = ep.make_state_imp(ep.STATE_P, rimp=(0.0, 0.75, 1.5), instate=1, als=3.0, select=ep.make_select(**sel))
+    if exc:
+        m = ep.exciplex_moments(imp.v_pi, imp.v_sigma, *exc)
+        for i in range(7):
+            imp.moments[i] = m[i]
     with ep.Context(p) as ctx:
         ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 20261203)
         ctx.set_impurity_state(imp)
@@ -115,7 +132,7 @@ def test_fortran_driver_p_state(gpu_available, tmp_path):
     nml = ("&input\n nx=32, ny=32, nz=32, xmax=12.d0, ymax=12.d0, zmax=12.d0, n4=100, mode=0,\n"
            " core4='OTC', h_core=2.377d0, niter=0, ximp=0.d0, yimp=0.75d0, zimp=1.5d0,\n"
            " lstate='P', instate=1, als_p=3.d0, selec_pi='Ba_plus_2D1', r_cutoff_pi=1.05d0,\n"
-           " umax_pi=5.216077d4\n/\n")
+           " umax_pi=5.216077d4," + keys + "\n/\n")
     out = subprocess.run([exe], input=nml, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert out.returncode == 0, out.stderr
 
@@ -223,3 +240,70 @@ def test_moving_state_matches_oracle(gpu_available):
     assert got[0][2] < 7.0
     for k in ("eso", "eHeX", "ekinx", "eimpu", "etot"):
         assert abs(e_gpu[k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (k, e_gpu[k], e[k])
+
+
+SELECT_CASES = [  # (state, instate, select, rimp)
+    (ep.STATE_P, 1, dict(mode=ep.SELECT_JZ, ljz="+1/2"), (0.0, 0.5, -1.0)),
+    (ep.STATE_D, 3, dict(mode=ep.SELECT_JZ), (1.0, 0.0, 0.75)),
+    (ep.STATE_P, 2, dict(mode=ep.SELECT_MIXTURE), (-0.75, 0.25, 0.5)),
+    (ep.STATE_P, 0, dict(mode=ep.SELECT_PAIR, fixed_moments=True), (0.5, -0.5, 0.0)),
+]
+
+
+def test_selection_modes_match_oracle(gpu_available):
+    """Ldiag_jz, Lmixture and an exciplex (fixed moments) through
+    effpot_set_impurity_state: the moments, the chosen invar (the host
+    instates on the oracle's moments) and the field of the oracle's potenimp
+    for that invar."""
+    print("[state] selection modes 32^3 x 4 slots", flush=True)
+    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=4)
+    imps = []
+    for state, instate, sel, rimp in SELECT_CASES:
+        imp = ep.make_state_imp(state, rimp=rimp, instate=instate, als=4.0, select=ep.make_select(**sel))
+        if sel.get("fixed_moments"):
+            m = ep.exciplex_moments(imp.v_pi, imp.v_sigma, "Ring", "Z", 3.9)
+            for i in range(7):
+                imp.moments[i] = m[i]
+        imps.append(imp)
+    with ep.Context(p) as ctx:
+        psis, fields = [], []
+        for s, imp in enumerate(imps):
+            ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 60 + s, slot=s)
+            ctx.set_impurity_state(imp, slot=s)
+            psis.append(ctx.psi(s))
+            fields.append(ctx.field("uimp", slot=s))
+        ctx.poten()
+        es = ctx.energy()
+    for s, ((state, instate, sel, rimp), g) in enumerate(zip(SELECT_CASES, imps)):
+        n = 6 if state == ep.STATE_P else 10
+        o = O.Oracle(make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377))
+        o.set_psi(psis[s])
+        if sel.get("fixed_moments"):
+            m = ep.exciplex_moments(g.v_pi, g.v_sigma, "Ring", "Z", 3.9)
+            assert np.array_equal(np.array(g.moments)[:7], m)
+        else:
+            m = o.state_moments(g)
+            assert np.max(np.abs(np.array(g.moments) - m)) <= 1e-12 * np.abs(m).max(), s
+        _, inv, eso = ep.instates(state, m, g.als, instate, ep.make_select(**sel))
+        gi = np.array(g.invar_c())
+        # Kramers pairs are degenerate, so the eigenvectors inside a pair (and
+        # any global phase) follow rounding: compare what does not depend on them
+        if sel.get("mode") == ep.SELECT_MIXTURE:
+            H = O.state_matrix(state, m, g.als)
+            w = np.linalg.eigvalsh(H)
+            a, b = g.select.a_pi_3o2, g.select.a_sig_1o2
+            amp2 = np.array([max(1.0 - a - b, 0.0), a, b])
+            assert abs(np.vdot(gi, gi).real - amp2.sum()) < 1e-12
+            assert abs(np.vdot(gi, H @ gi).real - amp2 @ w[0::2]) <= 1e-10 * np.abs(w).max()
+        else:
+            assert abs(abs(np.vdot(inv, gi)) - 1.0) < 1e-10, s
+            assert abs(g.eso - eso) <= 1e-9 * max(1.0, abs(eso))
+        o.set_impurity_state(g)
+        u = o.field("uimp")
+        assert np.max(np.abs(fields[s] - u)) <= 1e-12 * np.abs(u).max(), s
+        o.poten()
+        e = o.energy()
+        o.close()
+        for k in ("eHeX", "eso", "eimpu", "etot"):
+            assert abs(es[s][k] - e[k]) <= 1e-10 * max(1.0, abs(e[k])), (s, k, es[s][k], e[k])
+        print(s, n, g.eso, es[s]["eimpu"], flush=True)
