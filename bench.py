@@ -236,6 +236,29 @@ POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s)
     ("C z conv (1 in, 1 out)", 0, 5), ("C y inv (1)", 0, 4), ("C x c2r (1)", 1, 2), ("k_pot4", 8, 0)]
 
 
+def pc_step_bytes(n, mode, B, plane):
+    """Algorithmic bytes of one predictor-corrector step of B slots (real psi,
+    OTC): every array a launch reads or writes counted once.  Per point and
+    slot, with s the field size (4 B mixed, 8 B FP64) and psi and the
+    integrator history FP64: H psi + predictor/modifier reads psi, o3, h1, h2,
+    pc and pot4, uimp and writes o3, pc, h2 and the density (5*8 + 2s + 3*8 +
+    s); H psi + corrector reads psi, o3, o2, h1, h2, pc, pot4, uimp and writes
+    pc and psi (6*8 + 2s + 2*8); the renormalisation reads and writes psi and
+    writes the density (2*8 + s); poten is the sum of its launches
+    (POTEN_LAUNCHES, the forward y passes absent when fused into the plane
+    launch)."""
+    s = 4 if mode == "mixed" else 8
+    N = n ** 3
+    M = (n // 2 + 1) * n * n
+    psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + s)
+    poten = 0
+    for k, (name, cn, cm) in enumerate(POTEN_LAUNCHES):
+        if plane and k in (1, 7, 13):
+            continue
+        poten += s * (cn * N + cm * M) + (4 * N if (k == 17 and mode == "mixed") else 0)
+    return B * (psi_side * N + poten)
+
+
 def run_poten(a):
     """Each launch of the mean-field sequence of the headline context timed
     alone (20 back-to-back launches, HIP events) with its algorithmic bytes."""
@@ -512,6 +535,13 @@ def main():
                             "ms_per_launch_back_to_back": ms_b2b,
                             "bytes_per_launch": bytes_alg}
         line["poten_ms"] = ctx.kernel_time(2, reps=5)
+        # the whole PC step (predictor, poten, corrector, norm, renormalisation)
+        # against its algorithmic bytes: the fraction of HBM peak a step reaches
+        ms_step = ctx.kernel_time(4, reps=10)
+        step_b = pc_step_bytes(a.n, a.mode, B, ctx.plane_fused(a.mode == "mixed"))
+        line["pc_step"] = {"ms": ms_step, "bytes": step_b, "achieved": step_b / (ms_step * 1e-3) / 1e9,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": step_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "slots": B}
     ctx.close()
     # ---- the plain FP64 form on slot 0's input (untimed reference) -----------
     if a.mode == "mixed" and not a.no_ref:

@@ -66,6 +66,7 @@ def lib():
         L.effpot_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Result)]
         L.effpot_sync.argtypes = [vp]
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.effpot_plane_fused.argtypes = [vp, C.c_int]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
         L.effpot_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
@@ -378,6 +379,10 @@ class Context:
         ms = C.c_double()
         self._chk(lib().effpot_kernel_time(self.h, which, reps, C.byref(ms)))
         return ms.value
+
+    def plane_fused(self, fp32_fields):
+        """True when poten() runs the forward x and y transforms as one launch."""
+        return bool(lib().effpot_plane_fused(self.h, 1 if fp32_fields else 0))
 
     def sync(self):
         self._chk(lib().effpot_sync(self.h))

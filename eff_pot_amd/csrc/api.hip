@@ -1894,23 +1894,27 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
 // The corrector timed inside whole predictor-corrector steps: the caches hold
 // what a relaxation leaves in them (poten runs in between), not the
 // corrector's own previous launch.
+// whole = false: the corrector inside `reps` whole PC steps; true: the whole
+// steps (predictor, poten, corrector, norm reduction, renormalisation).
 template <typename T> static int time_corrector_in_steps(effpot_ctx* ctx, State<T>& s, int reps, double* ms,
-                                                         bool f32) {
+                                                         bool f32, bool whole = false) {
   const int* act = ctx->d_active;
   std::vector<hipEvent_t> ev(2 * reps);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   int rc = EFFPOT_OK;
   for (int r = 0; r < reps && rc == EFFPOT_OK; r++) {
+    if (whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
     rc = launch_step<T, 0>(ctx, s, act, f32);
     if (rc == EFFPOT_OK) rc = f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act);
     if (rc != EFFPOT_OK) break;
-    HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
+    if (!whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
     ctx->corr_den = false;
     rc = launch_step<T, 1>(ctx, s, act, f32);
     ctx->corr_den = true;
-    HIPCHK(hipEventRecord(ev[2 * r + 1], ctx->st));
+    if (!whole) HIPCHK(hipEventRecord(ev[2 * r + 1], ctx->st));
     if (rc == EFFPOT_OK) rc = reduce<2>(ctx, 1, act, s.ring, RING_AFTER_PRED, RING_AFTER_CORR);
     if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, act, f32);
+    if (whole) HIPCHK(hipEventRecord(ev[2 * r + 1], ctx->st));
   }
   HIPCHK(hipStreamSynchronize(ctx->st));
   double sum = 0.0;
@@ -1926,7 +1930,7 @@ template <typename T> static int time_corrector_in_steps(effpot_ctx* ctx, State<
 
 template <typename T> static int time_kernel_t(effpot_ctx* ctx, State<T>& s, int which, int reps, double* ms,
                                                bool f32 = false) {
-  if (which == 3) return time_corrector_in_steps(ctx, s, reps, ms, f32);
+  if (which == 3 || which == 4) return time_corrector_in_steps(ctx, s, reps, ms, f32, which == 4);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -2003,6 +2007,11 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
   hipSetDevice(ctx->dev);
   RC(push_dt(ctx, 1e-4));
   return time_kernel_t(ctx, ctx->s64, which, reps, ms_per_launch, ctx->mixed);
+}
+
+int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields) {
+  if (check_ctx(ctx) != EFFPOT_OK) return 0;
+  return fp32_fields ? (plane_ok<float>(ctx) ? 1 : 0) : (plane_ok<double>(ctx) ? 1 : 0);
 }
 
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {

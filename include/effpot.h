@@ -403,12 +403,18 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
  * events over `reps` back-to-back launches on the current (live-precision)
  * state: 0 = H psi + predictor/modifier, 1 = H psi + corrector, 2 = the whole
  * poten() kernel sequence, 3 = the corrector timed inside `reps` whole
- * predictor-corrector steps (caches as in a relaxation), 100 + k = launch k
+ * predictor-corrector steps (caches as in a relaxation), 4 = those whole
+ * steps (predictor, poten, corrector, norm, renormalisation), 100 + k = launch k
  * of the poten() sequence alone (0-4 stage A: x r2c, y, z convolution, y,
  * x c2r; 5 derden; 6-10 stage B; 11 alpha_s divergence; 12-16 stage C;
  * 17 field assembly).  The launches
  * overwrite the integrator state: call it after the solves that matter. */
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
+
+/* Benchmark hook (no reference counterpart): 1 when poten() of FP32
+ * (fp32_fields = 1) or FP64 fields runs the forward x and y transforms as
+ * one fused plane launch (launches 1, 7 and 13 are then empty), else 0. */
+int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields);
 
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
  * one step of its own phase (a START slot gets poten + its first RK step).
