@@ -33,6 +33,16 @@ def test_refusals(gpu_available):
         st = ep.make_state_imp(ep.STATE_D, instate=0)
         st.v_pi.kind = 9
         assert _code(lambda: c.set_impurity_state(st)) == ERR_POT
+        # instates' choices of invar: an Ljz outside the j manifold, P-only
+        # modes and fixed (exciplex) moments on a D state, a negative amplitude
+        for state, instate, sel in ((ep.STATE_P, 0, dict(mode=ep.SELECT_JZ, ljz="+3/2")),
+                                    (ep.STATE_D, 2, dict(mode=ep.SELECT_JZ, ljz="+7/2")),
+                                    (ep.STATE_D, 0, dict(mode=ep.SELECT_FIX)),
+                                    (ep.STATE_D, 0, dict(fixed_moments=True)),
+                                    (ep.STATE_P, 0, dict(mode=ep.SELECT_MIXTURE, a_sig_1o2=-1.0)),
+                                    (ep.STATE_P, 0, dict(mode=7))):
+            st = ep.make_state_imp(state, instate=instate, select=ep.make_select(**sel))
+            assert _code(lambda: c.set_impurity_state(st)) == ERR_ARG, sel
         # a P-state impurity cannot move (its invar equation of motion is not built)
         c.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 1)
         c.set_impurity_state(ep.make_state_imp(ep.STATE_P, instate=0))
