@@ -228,12 +228,28 @@ def run_corrector(a):
                        mode=a.mode, batch=B, stencil_tile=list(tile), ms_per_launch=ms, bytes_per_launch=byt)
 
 
-POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s)
+POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s), OTC
     ("A x r2c (1 field)", 1, 2), ("A y fwd (1)", 0, 4), ("A z conv (1 in, 3 out)", 0, 2 + 6 + 3),
     ("A y inv (3)", 0, 12), ("A x c2r (3)", 3, 6), ("k_grad", 10, 0),
     ("B x r2c (4)", 4, 8), ("B y fwd (4)", 0, 16), ("B z conv (4 in, 4 out)", 0, 20), ("B y inv (4)", 0, 16),
     ("B x c2r (4)", 4, 8), ("k_div", 9, 0), ("C x r2c (1)", 1, 2), ("C y fwd (1)", 0, 4),
     ("C z conv (1 in, 1 out)", 0, 5), ("C y inv (1)", 0, 4), ("C x c2r (1)", 1, 2), ("k_pot4", 8, 0)]
+
+
+def poten_launch_bytes(k, n, mode, B, plane):
+    """Algorithmic bytes of launch k of poten (OTC) for B slots.  k_pot4 reads
+    delj4, dencg, sto1, ua, div, den and writes pot4 and ualphas; with FP32
+    fields it also reads the defect correction (4 B).  The forward y passes
+    are absent when fused into the plane launch."""
+    s = 4 if mode == "mixed" else 8
+    N, M = n ** 3, (n // 2 + 1) * n * n
+    if plane and k in (1, 7, 13):
+        return 0
+    _, cn, cm = POTEN_LAUNCHES[k]
+    b = s * (cn * N + cm * M)
+    if k == 17 and mode == "mixed":
+        b += 4 * N
+    return B * b
 
 
 def pc_step_bytes(n, mode, B, plane):
@@ -243,19 +259,14 @@ def pc_step_bytes(n, mode, B, plane):
     integrator history FP64: H psi + predictor/modifier reads psi, o3, h1, h2,
     pc and pot4, uimp and writes o3, pc, h2 and the density (5*8 + 2s + 3*8 +
     s); H psi + corrector reads psi, o3, o2, h1, h2, pc, pot4, uimp and writes
-    pc and psi (6*8 + 2s + 2*8); the renormalisation reads and writes psi and
-    writes the density (2*8 + s); poten is the sum of its launches
-    (POTEN_LAUNCHES, the forward y passes absent when fused into the plane
-    launch)."""
+    pc and psi (6*8 + 2s + 2*8); the renormalisation reads and writes psi and,
+    for FP64 fields, writes the density (2*8 [+ s]); poten is the sum of its
+    launches (poten_launch_bytes)."""
     s = 4 if mode == "mixed" else 8
     N = n ** 3
-    M = (n // 2 + 1) * n * n
-    psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + s)
-    poten = 0
-    for k, (name, cn, cm) in enumerate(POTEN_LAUNCHES):
-        if plane and k in (1, 7, 13):
-            continue
-        poten += s * (cn * N + cm * M) + (4 * N if (k == 17 and mode == "mixed") else 0)
+    # the renormalisation stores the density only for FP64 fields
+    psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + (0 if mode == "mixed" else s))
+    poten = sum(poten_launch_bytes(k, n, mode, 1, plane) for k in range(len(POTEN_LAUNCHES)))
     return B * (psi_side * N + poten)
 
 
@@ -265,9 +276,6 @@ def run_poten(a):
     import eff_pot_amd as ep
     B = a.batch if a.mode == "mixed" else 1
     p, R = workload(a.n, a.mode, B)
-    s = 4 if a.mode == "mixed" else 8
-    N = a.n ** 3
-    M = (a.n // 2 + 1) * a.n * a.n
     out = []
     with ep.Context(p) as c:
         tile = tuple(a.tile) if a.tile else c.autotune(reps=5)
@@ -277,10 +285,11 @@ def run_poten(a):
             c.set_impurity(ep.make_morse(rimp=(0.0, 0.0, z0)), slot=b)
         c.kernel_time(2, reps=2)
         tot = 0.0
-        for k, (name, cn, cm) in enumerate(POTEN_LAUNCHES):
+        plane = c.plane_fused(a.mode == "mixed")
+        for k, (name, _, _) in enumerate(POTEN_LAUNCHES):
             ms = c.kernel_time(100 + k, reps=20)
-            byt = B * s * (cn * N + cm * M) + (B * 4 * N if (k == 17 and a.mode == "mixed") else 0)
-            out.append({"launch": k, "name": name, "ms": ms, "GB_s": byt / (ms * 1e-3) / 1e9})
+            byt = poten_launch_bytes(k, a.n, a.mode, B, plane)
+            out.append({"launch": k, "name": name, "ms": ms, "GB_s": byt / (ms * 1e-3) / 1e9 if byt else 0.0})
             tot += ms
         whole = c.kernel_time(2, reps=10)
     return common_line(a, "mean-field (poten) launches: ms and algorithmic GB/s", whole, "ms", whole * 1e-3,

@@ -80,11 +80,11 @@ __global__ void k_renorm_ring(Ring<T> rg, P* den, const double* scale, const int
   if (!active[s]) return;
   const T f = (T)scale[s];
   V* psi = reinterpret_cast<V*>(rg.p(s, R_CUR));
-  P* d = den + (long)s * N;
+  P* d = den ? den + (long)s * N : nullptr;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
     V p = f * psi[i];
     psi[i] = p;
-    d[i] = (P)norm2(p);
+    if (d) d[i] = (P)norm2(p);
   }
 }
 
@@ -659,9 +659,11 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
 }
 
 // ---- integrators ----------------------------------------------------------------
+// With FP32 fields the density is not stored: every path that follows (the
+// next predictor or RK stage, an energy check) recomputes it before reading.
 template <typename T> int renorm_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   if (f32)
-    PSI_LAUNCH_P(k_renorm_ring, T, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), ctx->s32.den,
+    PSI_LAUNCH_P(k_renorm_ring, T, float, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), (float*)nullptr,
                  (const double*)ctx->d_scale, act, ctx->N);
   else
     PSI_LAUNCH_P(k_renorm_ring, T, T, pw_grid(ctx->N, ctx->B), ring_of(ctx, s), s.den, (const double*)ctx->d_scale,
