@@ -1767,12 +1767,21 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
   // START: fields of the new states; they take their first RK step below.
   if (any_s) {
     RC(push_mask(ctx, 0, ms));
-    if (f32) {  // a new solve starts without a defect correction
+    if (f32) {
+      // a new solve starts from the defect correction of its starting state,
+      // so the RK starter already runs on FP32 fields that match FP64
       hipLaunchKernelGGL((k_zero_slots<float>), pw_grid(ctx->N, B), dim3(256), 0, ctx->st, ctx->s32.corr, dS, ctx->N);
       LAUNCHCHK();
+      RC(den_from_cur(ctx, dS, false));
+      RC(poten_t(ctx, s, dS));
     }
     RC(den_from_cur(ctx, dS, f32));
     RC(f32 ? poten_t(ctx, ctx->s32, dS) : poten_t(ctx, s, dS));
+    if (f32) {
+      hipLaunchKernelGGL(k_defect, pw_grid(ctx->N, B), dim3(256), 0, ctx->st, (const double*)s.pot4,
+                         (const double*)s.uimp, ctx->s32.pot4, (const float*)ctx->s32.uimp, ctx->s32.corr, dS, ctx->N);
+      LAUNCHCHK();
+    }
     for (int b = 0; b < B; b++)
       if (ms[b]) {  // a new solve: counters start from zero
         sl[b].nrk = 0; sl[b].since = 0; sl[b].steps = 0; sl[b].converged = 0;

@@ -147,11 +147,13 @@ def test_headline_shape_fp64_matches_oracle(gpu_available):
 def test_headline_shape_mixed_batch_tracks_oracle(gpu_available):
     """The benchmark's context (4 slots, FP32 fields) over 60 steps: slot 0
     against the FP64 oracle from the same start, and every slot keeps its
-    norm.  The FP32 fields run without a defect correction through the RK
-    starter (the first correction comes with the energy check that follows
-    it), which moves the early trajectory: measured 7e-6 relative in etot and
-    9e-5 in eimpu at step 61.  The difference decays as the droplet relaxes
-    (the converged E_imp of this workload agrees with FP64 to ~4e-7 K).  The
+    norm.  The FP32 fields carry the defect correction of the last energy
+    check (here every 20 steps); while the droplet still relaxes fast, the
+    part of the FP32 error that this snapshot does not follow moves the early
+    trajectory: measured 7e-6 relative in etot and 9e-5 in eimpu at step 61,
+    the same with and without a correction at the start of the solve.  The
+    difference decays as the droplet relaxes (the converged E_imp of this
+    workload agrees with FP64 to ~4e-7 K).  The
     bit-level pin at this size is the FP64 test above; this one guards the
     4-slot batch at full size against gross errors (slot offsets, fields of
     the wrong slot), which move energies by kelvins."""
