@@ -173,6 +173,13 @@ module effpot_c
       integer(c_int), value :: slot
       type(effpot_plus), intent(in) :: plus
     end function
+    ! z_exciplex_exclusion of energy(): points nearer than r leave eHeX and uplus
+    integer(c_int) function effpot_set_exclusion(ctx, slot, r) bind(c, name="effpot_set_exclusion")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), value :: r
+    end function
     ! P/D internal-state impurity: potenimpini + instates + potenimp
     integer(c_int) function effpot_set_impurity_state(ctx, slot, imp) bind(c, name="effpot_set_impurity_state")
       import :: c_int, c_ptr, effpot_state_imp

@@ -38,6 +38,7 @@ program effpot_driver
   character(len=6) :: exciplex = 'Ring'
   character(len=1) :: lexciplex_axis = 'Z'
   real(8) :: a_pi_3o2 = 0.94d0, a_sig_1o2 = 0.34117444d0, r_exc = 3.9d0
+  real(8) :: z_exciplex_exclusion = 0.d0   ! energy.f90:155 (modules.f90 default 0)
   real(8) :: tol = 1.d-9
   logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
   real(8), parameter :: ktops = 7.63822291d0
@@ -48,7 +49,7 @@ program effpot_driver
                    fast_io, selec_plus, r_cutoff_plus, umax_plus, selec_sigma, selec_delta, lstate, &
                    instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d, &
                    ldiag_jz, ljz, lexcite_state_fix, laverage_p_value, lmixture, a_pi_3o2, a_sig_1o2, &
-                   lexciplex_state_fix, exciplex, lexciplex_axis, r_exc
+                   lexciplex_state_fix, exciplex, lexciplex_axis, r_exc, z_exciplex_exclusion
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
@@ -187,6 +188,7 @@ program effpot_driver
   end if
   call preset(selec_plus, r_cutoff_plus, umax_plus, plus)
   call check(ctx, effpot_set_plus(ctx, 0_c_int, plus), 'effpot_set_plus')
+  call check(ctx, effpot_set_exclusion(ctx, 0_c_int, z_exciplex_exclusion), 'effpot_set_exclusion')
 
   if (niter > 0) then
     opt%dt = deltat; opt%tol = tol; opt%n_rk = 3; opt%pener = pener; opt%max_steps = niter

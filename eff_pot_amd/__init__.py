@@ -67,6 +67,7 @@ def lib():
         L.effpot_sync.argtypes = [vp]
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.effpot_plane_fused.argtypes = [vp, C.c_int]
+        L.effpot_set_exclusion.argtypes = [vp, C.c_int, C.c_double]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
         L.effpot_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
@@ -308,6 +309,11 @@ class Context:
         imp.instate >= 0); fills imp.invar, eigen, moments and eso."""
         self._chk(lib().effpot_set_impurity_state(self.h, slot, C.byref(imp)))
         return imp
+
+    def set_exclusion(self, r, slot=0):
+        """z_exciplex_exclusion (energy.f90:155): grid points nearer than r to
+        the impurity are left out of eHeX and uplus (0: none)."""
+        self._chk(lib().effpot_set_exclusion(self.h, slot, r))
 
     def set_plus(self, plus, slot=0):
         """X+ potential of the uplus energy term for one slot (None: none)."""

@@ -1520,6 +1520,19 @@ int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus) {
   return EFFPOT_OK;
 }
 
+int effpot_set_exclusion(effpot_ctx* ctx, int slot, double r) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  if (!(r >= 0.0) || !std::isfinite(r)) return ctx->fail(EFFPOT_ERR_ARG, "exclusion radius must be finite and >= 0");
+  PlusSlot& ps = ctx->h_plus[slot];
+  ps.excl = r;
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));  // an energy launch may still read the slot's entry
+  HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
 int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_plus* out) {
   if (!selec || !out) return EFFPOT_ERR_ARG;
   return plus_preset(selec, r_cutoff, umax, out) ? EFFPOT_OK : EFFPOT_ERR_POT;

@@ -66,6 +66,7 @@ struct PlusSlot {
   effpot_plus p;
   double rimp[3];
   int32_t on, pad_;
+  double excl;  // z_exciplex_exclusion: points nearer than this to rimp stay out of eHeX and uplus
 };
 
 namespace plus_fits {

@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
   const long so = (long)s * a.slot;
   const V* psi = reinterpret_cast<const V*>(a.psi_all + ((long)s * 5 + a.ring[8 * s]) * a.slot);
   const PlusSlot& pl = a.plus[s];
-  const bool plus_on = pl.on != 0;
+  const bool plus_on = pl.on != 0, excl_on = pl.excl > 0.0;
   double v[kEnergySums];
 #pragma unroll
   for (int k = 0; k < kEnergySums; k++) v[k] = 0.0;
@@ -546,18 +546,22 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
         v[3] += (double)(fa * ((a.dx[i] * a.ix[i] + a.dy[i] * a.iy[i]) + a.dz[i] * a.iz[i]));
       }
       v[4] += (double)(d * (T(1) + tanh(a.beta * (d - a.den_m))));
-      v[5] += (double)(d * a.uimp[i]);
+      // |rimp - r| for uplus and the exclusion sphere (energy.f90:149-157)
+      double r = 0.0;
+      if (plus_on || excl_on) {
+        const double rx = pl.rimp[0] - (a.x0 + a.hx * x), ry = pl.rimp[1] - (a.y0 + a.hy * y),
+                     rz = pl.rimp[2] - (a.z0 + a.hz * z);
+        r = sqrt((rx * rx + ry * ry) + rz * rz);
+      }
+      const bool keep = !excl_on || r >= pl.excl;
+      if (keep) v[5] += (double)(d * a.uimp[i]);
       v[6] += (double)d;
       v[7] += (double)d * (a.x0 + a.hx * x);
       v[8] += (double)d * (a.y0 + a.hy * y);
       v[9] += (double)d * (a.z0 + a.hz * z);
       V hp = hpsi_of(p, d2, a.h2o2m4, a.pot4[i], a.uimp[i]);
       v[10] -= (double)dotre(p, hp);
-      if (plus_on) {  // uplus: Select_Pot(selec_plus, |rimp - r|) (energy.f90:149-157)
-        const double rx = pl.rimp[0] - (a.x0 + a.hx * x), ry = pl.rimp[1] - (a.y0 + a.hy * y),
-                     rz = pl.rimp[2] - (a.z0 + a.hz * z);
-        v[11] += (double)d * plus_pot(pl.p, sqrt((rx * rx + ry * ry) + rz * rz));
-      }
+      if (plus_on && keep) v[11] += (double)d * plus_pot(pl.p, r);  // uplus: Select_Pot(selec_plus, r)
     });
   block_sum<kEnergySums>(v, red);
   if (threadIdx.x == 0) {

@@ -37,7 +37,7 @@ def _grid_key(h):
 
 
 def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch=8, rimp=None, plus="default",
-             state=None):
+             state=None, exclusion=0.0):
     """E_imp and the energy balance of each snapshot in `paths`.
 
     impurity: an effpot Impurity (its rimp is replaced by the snapshot's own
@@ -45,7 +45,9 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
     Plus, None for none, "default" for the reference's selec_plus).  state: a
     StateImp template for a P/D internal-state impurity instead of `impurity`
     (instates runs on each snapshot's density, as readenc does for mode 3
-    with instate >= 0; its invar, eigenvalues and eso are returned).  Returns
+    with instate >= 0; its invar, eigenvalues and eso are returned).
+    exclusion: z_exciplex_exclusion, the radius about the impurity whose
+    points are left out of eHeX and uplus (energy.f90:155).  Returns
     one dict per path, in order, with the energies (effpot_energies fields),
     the impurity position used and the X-He centre-of-mass distance.
     """
@@ -81,6 +83,7 @@ def evaluate(paths, impurity, core=CORE_OTC, h_core=2.377, icon=ICON_ZERO, batch
                         imp.kind = impurity.kind
                         ctx.set_impurity(imp, slot=s)
                     ctx.set_plus(plus, slot=s)
+                    ctx.set_exclusion(exclusion, slot=s)
                 ctx.poten()
                 es = ctx.energy()
                 for s, i in enumerate(chunk):
@@ -143,6 +146,8 @@ def main(argv=None):
                     help="Lexciplex_state_fix: P moments of a fixed exciplex instead of the density")
     ap.add_argument("--exciplex-axis", choices=["X", "Y", "Z"], default="Z")
     ap.add_argument("--r-exc", type=float, default=3.9)
+    ap.add_argument("--z-exciplex-exclusion", type=float, default=0.0,
+                    help="radius about the impurity left out of eHeX and uplus (energy.f90:155)")
     ap.add_argument("--core", choices=["OT", "OTC"], default="OTC")
     ap.add_argument("--h-core", type=float, default=2.377)
     ap.add_argument("--batch", type=int, default=8)
@@ -161,7 +166,8 @@ def main(argv=None):
             for i in range(7):
                 state.moments[i] = m[i]
     res = evaluate(a.files, imp, core=CORE_OTC if a.core == "OTC" else CORE_OT,
-                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus, state=state)
+                   h_core=a.h_core if a.h_core else H4OT, batch=a.batch, rimp=a.rimp, plus=plus, state=state,
+                   exclusion=a.z_exciplex_exclusion)
     for d in res:
         if not a.quiet:
             print(f"# {d['path']}")

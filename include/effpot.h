@@ -372,6 +372,12 @@ int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus);
  * (V_impur.f90:927-931).  Host only; needs no context. */
 int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_plus* out);
 
+/* z_exciplex_exclusion (energy.f90:149-157): grid points nearer than r to the
+ * slot's impurity are left out of eHeX and uplus (so of E_imp); r = 0, the
+ * reference's default and every slot's initial value, keeps every point.
+ * The sphere follows a moving impurity. */
+int effpot_set_exclusion(effpot_ctx* ctx, int slot, double r);
+
 /* The potential itself at distance r (A), in K, as the device evaluates it
  * (Select_Pot's role when Lcontrol prints it).  Host only. */
 double effpot_plus_eval(const effpot_plus* plus, double r);

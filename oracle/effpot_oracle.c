@@ -128,6 +128,7 @@ typedef struct oracle {
   int npot; double delinter; double* vtab; /* vtab[0..npot+1] */
   effpot_impurity imp; int have_imp;
   effpot_plus plus; /* X+ potential of uplus (kind NULL: none) */
+  double excl;      /* z_exciplex_exclusion (energy.f90:155) */
   double eso;       /* spin-orbit energy of a P/D-state impurity */
   int rt;           /* real time: the step integrates i H psi */
   /* moving impurity (classicimp): r = imp.rimp, v, a, histories, pc, q */
@@ -572,6 +573,9 @@ void oracle_set_plus(oracle_t* o, const effpot_plus* p) {
   else memset(&o->plus, 0, sizeof(o->plus));
 }
 
+/* z_exciplex_exclusion of energy.f90:155 (0: every point) */
+void oracle_set_exclusion(oracle_t* o, double r) { o->excl = r; }
+
 /* ------------------------------------------------------------------------ */
 /* P/D internal state: potenimpini (potenimpini.f90:14-66), updatepoten     */
 /* (potenimp.f90:531-565), the grid sums and matrices of instates           */
@@ -927,10 +931,13 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
         if (p->core == EFFPOT_CORE_OTC)
           eal += o->falfs[i] * (o->dxden[i] * o->intx[i] + o->dyden[i] * o->inty[i] + o->dzden[i] * o->intz[i]);
         esol += a3 * (1.0 + tanh(p->beta * (a3 - p->den_m)));
-        ehex += a3 * o->uimp[i];
-        if (o->plus.kind == EFFPOT_POT_SERIES) { /* energy.f90:149-157, rXHe = rimp - r */
+        { /* energy.f90:149-157: rXHe = rimp - r; points with r < z_exciplex_exclusion are left out */
           double rx = ri[0] - o->x[ix], ry = ri[1] - o->y[iy], rz = ri[2] - o->z[iz];
-          upl += a3 * oracle_plus_pot(&o->plus, sqrt(rx * rx + ry * ry + rz * rz));
+          double r = sqrt(rx * rx + ry * ry + rz * rz);
+          if (r >= o->excl) {
+            ehex += a3 * o->uimp[i];
+            if (o->plus.kind == EFFPOT_POT_SERIES) upl += a3 * oracle_plus_pot(&o->plus, r);
+          }
         }
         nat += a3;
         xc += a3 * o->x[ix]; yc += a3 * o->y[iy]; zc += a3 * o->z[iz];

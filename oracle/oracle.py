@@ -52,6 +52,7 @@ def lib():
         L.oracle_plus_pot.argtypes = [C.POINTER(Plus), C.c_double]
         L.oracle_plus_pot.restype = C.c_double
         L.oracle_set_plus.argtypes = [C.c_void_p, C.POINTER(Plus)]
+        L.oracle_set_exclusion.argtypes = [C.c_void_p, C.c_double]
         L.oracle_set_real_time.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_impurity_motion.argtypes = [C.c_void_p, _dp, C.c_double]
         L.oracle_get_impurity.argtypes = [C.c_void_p, _dp, _dp, _dp]
@@ -249,6 +250,10 @@ class Oracle:
 
     def set_plus(self, plus):
         lib().oracle_set_plus(self.h, C.byref(plus) if plus is not None else None)
+
+    def set_exclusion(self, r):
+        """z_exciplex_exclusion: points nearer than r to the impurity leave eHeX and uplus."""
+        lib().oracle_set_exclusion(self.h, r)
 
     # --- operators ---
     def poten(self):

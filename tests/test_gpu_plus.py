@@ -52,3 +52,41 @@ def test_uplus_matches_oracle(gpu_available, precision):
             assert e["uplus"] != 0.0
             assert abs(got[s]["uplus"] - e["uplus"]) <= 1e-12 * abs(e["uplus"])
         assert abs(got[s]["etot"] - e["etot"]) <= 1e-10 * abs(e["etot"])
+
+
+EXCL = [2.0, 0.0, 3.5]  # radii hitting grid distances exactly (h = 0.5 A, impurities on nodes)
+
+
+def test_exclusion_matches_oracle(gpu_available):
+    """z_exciplex_exclusion per slot: eHeX, uplus and E_imp against the
+    oracle with the same sphere (points at r = radius stay in), and a
+    negative radius refused."""
+    print("[plus] exclusion sphere 32^3 x 3 slots", flush=True)
+    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=3)
+    with ep.Context(p) as ctx:
+        psis = []
+        for s in range(3):
+            ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 80 + s, slot=s)
+            ctx.set_impurity(make_morse(rimp=RIMP[s]), slot=s)
+            ctx.set_plus(ep.make_plus(*PLUS[0]), slot=s)
+            ctx.set_exclusion(EXCL[s], slot=s)
+            psis.append(ctx.psi(s))
+        with pytest.raises(ep.EffpotError) as err:
+            ctx.set_exclusion(-1.0, slot=1)
+        assert err.value.code == 1
+        ctx.poten()
+        got = ctx.energy()
+    for s in range(3):
+        o = Oracle(make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377))
+        o.set_psi(psis[s])
+        o.set_impurity(make_morse(rimp=RIMP[s]))
+        o.set_plus(ep.make_plus(*PLUS[0]))
+        o.poten()
+        full = o.energy()
+        o.set_exclusion(EXCL[s])
+        e = o.energy()
+        o.close()
+        if EXCL[s] > 0:
+            assert e["eHeX"] != full["eHeX"] and e["uplus"] != full["uplus"]
+        for k in ("eHeX", "uplus", "eimpu", "etot"):
+            assert abs(got[s][k] - e[k]) <= 1e-11 * max(1.0, abs(e[k])), (s, k, got[s][k], e[k])

@@ -92,3 +92,46 @@ def test_oracle_uplus_brute_force():
     e0 = o.energy()
     assert e0["uplus"] == 0.0 and e0["etot"] == e["etot"] and e0["eHeX"] == e["eHeX"]
     o.close()
+
+
+def test_oracle_exclusion_sphere():
+    """z_exciplex_exclusion (energy.f90:155): with the impurity on a node of
+    the h = 0.5 grid and r = 1 = 2h, exactly the 27 points of the 3x3x3 cube
+    around it (r < 1) leave eHeX and uplus, while the 6 points at r = 1 stay
+    (the test is r >= z_exciplex_exclusion).  A radius beyond the box empties
+    both sums; E_imp follows eHeX; nothing else moves."""
+    n = 16
+    prm = make_params(n, xmax=4.0, n4=20.0, core=CORE_OT, h_core=2.377)
+    o = O.Oracle(prm)
+    rng = np.random.default_rng(9)
+    psi = rng.uniform(0.05, 0.2, (n, n, n)) + 0j
+    o.set_psi(psi)
+    rimp = (0.5, -1.0, 1.5)  # grid nodes 9, 6, 11
+    o.set_impurity(make_morse(rimp=rimp))
+    plus = ep.make_plus("Ba_plus_2D2", 1.05, 6.260015e4)
+    o.set_plus(plus)
+    o.poten()
+    e0 = o.energy()
+    den = np.abs(psi) ** 2
+    uimp = o.field("uimp")
+    x = -4.0 + 0.5 * np.arange(n)
+    cube = [(9 + i, 6 + j, 11 + k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
+    dh = sum(den[iz, iy, ix] * uimp[iz, iy, ix] for ix, iy, iz in cube) * 0.125
+    du = sum(den[iz, iy, ix] * ep.plus_eval(plus, float(np.sqrt((rimp[0] - x[ix]) ** 2 + (rimp[1] - x[iy]) ** 2 +
+                                                                  (rimp[2] - x[iz]) ** 2))) for ix, iy, iz in cube) * 0.125
+    o.set_exclusion(1.0)
+    e1 = o.energy()
+    assert abs((e0["eHeX"] - e1["eHeX"]) - dh) <= 1e-12 * abs(e0["eHeX"])
+    assert abs((e0["uplus"] - e1["uplus"]) - du) <= 1e-12 * abs(e0["uplus"])
+    assert e1["eimpu"] == e1["ekinx"] + e1["eHeX"] + e1["eso"]
+    for k in ("etot4", "ekin4", "elj4", "ecor4", "natoms", "mu"):
+        assert e1[k] == e0[k], k
+    o.set_exclusion(1.0 + 1e-12)  # the 6 points at r = 1 leave too
+    e2 = o.energy()
+    assert e2["eHeX"] != e1["eHeX"]
+    o.set_exclusion(100.0)
+    e3 = o.energy()
+    assert e3["eHeX"] == 0.0 and e3["uplus"] == 0.0
+    o.set_exclusion(0.0)
+    assert o.energy() == e0
+    o.close()
