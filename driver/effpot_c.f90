@@ -180,6 +180,12 @@ module effpot_c
       integer(c_int), value :: slot
       real(c_double), value :: r
     end function
+    ! Lfilter_exciplex_force: the z force of a moving impurity beyond the exclusion only
+    integer(c_int) function effpot_set_force_filter(ctx, slot, on) bind(c, name="effpot_set_force_filter")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot, on
+    end function
     ! P/D internal-state impurity: potenimpini + instates + potenimp
     integer(c_int) function effpot_set_impurity_state(ctx, slot, imp) bind(c, name="effpot_set_impurity_state")
       import :: c_int, c_ptr, effpot_state_imp

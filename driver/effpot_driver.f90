@@ -33,7 +33,7 @@ program effpot_driver
   real(8) :: als_p = 0.d0, als_d = 0.d0
   ! instates' choice of invar (modules.f90:284-318)
   logical :: ldiag_jz = .false., lexcite_state_fix = .false., laverage_p_value = .false., lmixture = .false.
-  logical :: lexciplex_state_fix = .false.
+  logical :: lexciplex_state_fix = .false., lfilter_exciplex_force = .false.
   character(len=4) :: ljz = ''
   character(len=6) :: exciplex = 'Ring'
   character(len=1) :: lexciplex_axis = 'Z'
@@ -49,7 +49,8 @@ program effpot_driver
                    fast_io, selec_plus, r_cutoff_plus, umax_plus, selec_sigma, selec_delta, lstate, &
                    instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d, &
                    ldiag_jz, ljz, lexcite_state_fix, laverage_p_value, lmixture, a_pi_3o2, a_sig_1o2, &
-                   lexciplex_state_fix, exciplex, lexciplex_axis, r_exc, z_exciplex_exclusion
+                   lexciplex_state_fix, exciplex, lexciplex_axis, r_exc, z_exciplex_exclusion, &
+                   lfilter_exciplex_force
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp
@@ -189,6 +190,8 @@ program effpot_driver
   call preset(selec_plus, r_cutoff_plus, umax_plus, plus)
   call check(ctx, effpot_set_plus(ctx, 0_c_int, plus), 'effpot_set_plus')
   call check(ctx, effpot_set_exclusion(ctx, 0_c_int, z_exciplex_exclusion), 'effpot_set_exclusion')
+  call check(ctx, effpot_set_force_filter(ctx, 0_c_int, merge(1_c_int, 0_c_int, lfilter_exciplex_force)), &
+             'effpot_set_force_filter')
 
   if (niter > 0) then
     opt%dt = deltat; opt%tol = tol; opt%n_rk = 3; opt%pener = pener; opt%max_steps = niter

@@ -68,6 +68,7 @@ def lib():
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.effpot_plane_fused.argtypes = [vp, C.c_int]
         L.effpot_set_exclusion.argtypes = [vp, C.c_int, C.c_double]
+        L.effpot_set_force_filter.argtypes = [vp, C.c_int, C.c_int]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
         L.effpot_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.effpot_set_tiles.argtypes = [vp, C.c_int, C.c_int]
@@ -314,6 +315,11 @@ class Context:
         """z_exciplex_exclusion (energy.f90:155): grid points nearer than r to
         the impurity are left out of eHeX and uplus (0: none)."""
         self._chk(lib().effpot_set_exclusion(self.h, slot, r))
+
+    def set_force_filter(self, on, slot=0):
+        """Lfilter_exciplex_force (potenimp.f90:216-245): the z force of a
+        moving impurity sums only the planes beyond the exclusion radius."""
+        self._chk(lib().effpot_set_force_filter(self.h, slot, 1 if on else 0))
 
     def set_plus(self, plus, slot=0):
         """X+ potential of the uplus energy term for one slot (None: none)."""

@@ -730,8 +730,9 @@ int dyn_hiv(effpot_ctx* ctx, const int* act) {
 // forceimp (potenimp.f90:193-258) after poten: a = -sum grad(den) uimp dV / m.
 int force_t(effpot_ctx* ctx, const int* act) {
   hipLaunchKernelGGL(k_force_partials, dim3(nblk(ctx), ctx->B), dim3(256), 0, ctx->st, (const Motion*)ctx->d_motion,
-                     act, (const double*)ctx->s64.dx, (const double*)ctx->s64.dy, (const double*)ctx->s64.dz,
-                     (const double*)ctx->s64.uimp, ctx->N, ctx->d_partial);
+                     (const PlusSlot*)ctx->d_plus, act, (const double*)ctx->s64.dx, (const double*)ctx->s64.dy,
+                     (const double*)ctx->s64.dz, (const double*)ctx->s64.uimp, ctx->N, (long)ctx->nx * ctx->ny,
+                     ctx->nz, -ctx->p.zmax, ctx->hz, ctx->d_partial);
   LAUNCHCHK();
   RC(reduce<3>(ctx, 0, act));
   hipLaunchKernelGGL(k_motion_force, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->st, ctx->d_motion, act, ctx->B,
@@ -1528,6 +1529,18 @@ int effpot_set_exclusion(effpot_ctx* ctx, int slot, double r) {
   ps.excl = r;
   hipSetDevice(ctx->dev);
   HIPCHK(hipStreamSynchronize(ctx->st));  // an energy launch may still read the slot's entry
+  HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_set_force_filter(effpot_ctx* ctx, int slot, int on) {
+  RC(check_ctx(ctx));
+  if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
+  PlusSlot& ps = ctx->h_plus[slot];
+  ps.force_filter = on ? 1 : 0;
+  hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipMemcpyAsync(ctx->d_plus + slot, &ps, sizeof(PlusSlot), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;

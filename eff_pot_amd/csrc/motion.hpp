@@ -133,19 +133,34 @@ __global__ void k_imp_field_moving(const Motion* M, const int* active, int nx, i
 
 // forceimp partial sums [slot][block][3]: sum d_c den * uimp over the block's
 // points (grid: nblk x slots, grid-stride).
-__global__ __launch_bounds__(256) void k_force_partials(const Motion* M, const int* active, const double* dx,
-                                                        const double* dy, const double* dz, const double* uimp, long N,
-                                                        double* partial) {
+// With Lfilter_exciplex_force (potenimp.f90:216-245) F_z sums only the planes
+// on the far side of the impurity beyond z_exciplex_exclusion: 1-based iz >=
+// iz0 = int((rz - z(1) + excl)/hz + 1.5) when rz < 0, else iz <= nz0 =
+// int((rz - z(1) - excl)/hz + 1.5); z1 = z(1), plane = nx ny.
+__global__ __launch_bounds__(256) void k_force_partials(const Motion* M, const PlusSlot* P, const int* active,
+                                                        const double* dx, const double* dy, const double* dz,
+                                                        const double* uimp, long N, long plane, int nz, double z1,
+                                                        double hz, double* partial) {
   const int s = blockIdx.y;
   if (!active[s] || !M[s].on) return;
   __shared__ double red[4 * 3];
   double v[3] = {0.0, 0.0, 0.0};
   const long so = (long)s * N;
+  int zlo = 0, zhi = nz - 1;  // 0-based planes of F_z
+  if (P[s].force_filter) {
+    const double rz = M[s].r[2], gap = P[s].excl;
+    if (rz < 0.0) {
+      zlo = max(1, (int)((rz - z1 + gap) / hz + 1.5)) - 1;
+    } else {
+      zhi = min(nz, (int)((rz - z1 - gap) / hz + 1.5)) - 1;
+    }
+  }
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
     const double u = uimp[so + i];
+    const int iz = (int)(i / plane);
     v[0] += dx[so + i] * u;
     v[1] += dy[so + i] * u;
-    v[2] += dz[so + i] * u;
+    if (iz >= zlo && iz <= zhi) v[2] += dz[so + i] * u;
   }
   block_sum<3>(v, red);
   if (threadIdx.x == 0) {

@@ -65,7 +65,8 @@ __host__ __device__ inline double plus_pot(const effpot_plus& p, double r) {
 struct PlusSlot {
   effpot_plus p;
   double rimp[3];
-  int32_t on, pad_;
+  int32_t on;
+  int32_t force_filter;  // Lfilter_exciplex_force: F_z only beyond excl in z
   double excl;  // z_exciplex_exclusion: points nearer than this to rimp stay out of eHeX and uplus
 };
 

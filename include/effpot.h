@@ -378,6 +378,15 @@ int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_p
  * The sphere follows a moving impurity. */
 int effpot_set_exclusion(effpot_ctx* ctx, int slot, double r);
 
+/* Lfilter_exciplex_force (potenimp.f90:216-245): the z component of a moving
+ * impurity's force sums only the planes beyond the slot's exclusion radius on
+ * the far side of the impurity (iz >= int((rz - z(1) + r)/hz + 1.5) for rz <
+ * 0, else iz <= int((rz - z(1) - r)/hz + 1.5), 1-based), so an exciplex bound
+ * next to the impurity does not push it.  The x and y components are
+ * unchanged.  The reference's He-He correction term is commented out there
+ * and is not built. */
+int effpot_set_force_filter(effpot_ctx* ctx, int slot, int on);
+
 /* The potential itself at distance r (A), in K, as the device evaluates it
  * (Select_Pot's role when Lcontrol prints it).  Host only. */
 double effpot_plus_eval(const effpot_plus* plus, double r);

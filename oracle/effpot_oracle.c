@@ -129,6 +129,7 @@ typedef struct oracle {
   effpot_impurity imp; int have_imp;
   effpot_plus plus; /* X+ potential of uplus (kind NULL: none) */
   double excl;      /* z_exciplex_exclusion (energy.f90:155) */
+  int ffilter;      /* Lfilter_exciplex_force (potenimp.f90:216-245) */
   double eso;       /* spin-orbit energy of a P/D-state impurity */
   int rt;           /* real time: the step integrates i H psi */
   /* moving impurity (classicimp): r = imp.rimp, v, a, histories, pc, q */
@@ -575,6 +576,7 @@ void oracle_set_plus(oracle_t* o, const effpot_plus* p) {
 
 /* z_exciplex_exclusion of energy.f90:155 (0: every point) */
 void oracle_set_exclusion(oracle_t* o, double r) { o->excl = r; }
+void oracle_set_force_filter(oracle_t* o, int on) { o->ffilter = on; }
 
 /* ------------------------------------------------------------------------ */
 /* P/D internal state: potenimpini (potenimpini.f90:14-66), updatepoten     */
@@ -1024,10 +1026,25 @@ static void imp_field_at(oracle_t* o, const double* rr) {
 /* forceimp (potenimp.f90:213-254 without the exciplex filter): a = F / m. */
 static void force_imp(oracle_t* o) {
   double f[3] = {0.0, 0.0, 0.0};
+  /* Lfilter_exciplex_force: F(3) over the 1-based planes iz >= iz0 (rimp(3)
+   * < 0) or iz <= nz0, both from z_exciplex_exclusion (potenimp.f90:216-245) */
+  int lo = 1, hi = o->nz;
+  if (o->ffilter) {
+    const double rz = o->imp.rimp[2];
+    if (rz < 0.0) {
+      int iz0 = (int)((rz - o->z[0] + o->excl) / o->hz + 1.5);
+      lo = iz0 > 1 ? iz0 : 1;
+    } else {
+      int nz0 = (int)((rz - o->z[0] - o->excl) / o->hz + 1.5);
+      hi = nz0 < o->nz ? nz0 : o->nz;
+    }
+  }
+  const size_t plane = (size_t)o->nx * o->ny;
   for (size_t i = 0; i < o->N; i++) {
+    const int iz = (int)(i / plane) + 1;
     f[0] += o->dxden[i] * o->uimp[i];
     f[1] += o->dyden[i] * o->uimp[i];
-    f[2] += o->dzden[i] * o->uimp[i];
+    if (iz >= lo && iz <= hi) f[2] += o->dzden[i] * o->uimp[i];
   }
   for (int c = 0; c < 3; c++) o->aimp[c] = (-f[c] * o->dxyz) / o->mimp;
 }

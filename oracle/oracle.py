@@ -53,6 +53,7 @@ def lib():
         L.oracle_plus_pot.restype = C.c_double
         L.oracle_set_plus.argtypes = [C.c_void_p, C.POINTER(Plus)]
         L.oracle_set_exclusion.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_set_force_filter.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_real_time.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_impurity_motion.argtypes = [C.c_void_p, _dp, C.c_double]
         L.oracle_get_impurity.argtypes = [C.c_void_p, _dp, _dp, _dp]
@@ -254,6 +255,10 @@ class Oracle:
     def set_exclusion(self, r):
         """z_exciplex_exclusion: points nearer than r to the impurity leave eHeX and uplus."""
         lib().oracle_set_exclusion(self.h, r)
+
+    def set_force_filter(self, on):
+        """Lfilter_exciplex_force: F_z only beyond the exclusion in z."""
+        lib().oracle_set_force_filter(self.h, 1 if on else 0)
 
     # --- operators ---
     def poten(self):

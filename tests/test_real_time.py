@@ -9,6 +9,7 @@ helium sum Im(psi* D psi) dV (D the 13-point first derivative) plus m v of the
 impurity, is conserved to 2e-3 of the impulse exchanged, which fixes the sign
 of the force, the mass units (mp_u) and the direction of time (i H)."""
 import numpy as np
+import pytest
 
 from eff_pot_amd.abi import make_params, make_morse, CORE_OT, POT_NULL
 from oracle import oracle as O
@@ -84,4 +85,32 @@ def test_real_time_conserves_norm_and_momentum():
     dP = _p_fd(o, o.psi(), h) + m * v - P0
     assert np.max(np.abs(dP)) <= 2e-3 * abs(impulse[2]), (dP, impulse)
     assert abs(o.energy()["natoms"] - n0) <= 1e-7 * n0
+    o.close()
+
+
+@pytest.mark.parametrize("rz", [-1.5, 1.5])
+def test_exciplex_force_filter(rz):
+    """Lfilter_exciplex_force (potenimp.f90:216-245): with the impurity on a
+    grid plane and the exclusion 2h, F_z sums exactly the planes z >= rz +
+    2h (rz < 0) or z <= rz - 2h (rz > 0); F_x and F_y keep every plane."""
+    o, p = _droplet(relax=10)
+    h = 0.75
+    o.set_impurity(make_morse(rimp=(0.0, 0.0, rz), D=5.0, a=0.5))
+    o.poten()
+    o.set_impurity_motion([0.0, 0.0, 0.0], 40.0)
+    _, _, a_all = o.impurity()
+    o.set_exclusion(2 * h)
+    o.set_force_filter(True)
+    o.set_impurity_motion([0.0, 0.0, 0.0], 40.0)
+    _, _, a_f = o.impurity()
+    z = -12.0 + h * np.arange(32)
+    keep = z >= rz + 2 * h if rz < 0 else z <= rz - 2 * h
+    assert 0 < keep.sum() < 32
+    f = o.field("dzden") * o.field("uimp")
+    want = -np.sum(f[keep]) * h ** 3 / (40.0 * MP_U)
+    assert abs(a_f[2] - want) <= 1e-12 * max(abs(want), abs(a_all[2]))
+    assert a_f[0] == a_all[0] and a_f[1] == a_all[1]
+    o.set_force_filter(False)
+    o.set_impurity_motion([0.0, 0.0, 0.0], 40.0)
+    assert o.impurity()[2][2] == a_all[2]
     o.close()
