@@ -76,7 +76,9 @@ def test_converged_mixed_vs_oracle_and_fp64(gpu_available):
     assert abs(r32.e.etot - g["energies"]["etot"]) <= 1e-3
     l2 = np.linalg.norm(d32 - d64) / np.linalg.norm(d64)
     print(f"mixed: steps={r32.steps} (fp64 {r64.steps}) dE_imp={r32.e.eimpu - r64.e.eimpu:.2e} K L2(rho)={l2:.2e}")
-    assert l2 < 1e-4
+    # measured 8.6e-10 and 2.2e-9 K since a solve starts from the defect correction of its starting state
+    assert l2 < 1e-7
+    assert abs(r32.e.eimpu - r64.e.eimpu) < 1e-6
 
 
 def test_mixed_stops_where_fp64_stops(gpu_available):
@@ -91,7 +93,7 @@ def test_mixed_stops_where_fp64_stops(gpu_available):
     print(f"steps mixed {r32.steps} fp64 {r64.steps}, dE_imp {r32.e.eimpu - r64.e.eimpu:.2e} K")
     assert r32.converged and r64.converged
     assert r32.steps == r64.steps
-    assert abs(r32.e.eimpu - r64.e.eimpu) < 1e-5
+    assert abs(r32.e.eimpu - r64.e.eimpu) < 1e-6  # measured 1.9e-7 K
 
 
 # requests (z0, seed, Morse D) and the slot each takes in the mix: request 3
