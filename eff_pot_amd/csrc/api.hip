@@ -306,9 +306,9 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
   double hs[3] = {ctx->hx, ctx->hy, ctx->hz};
   for (int a = 0; a < 3; a++) {
     double s1 = 1.0 / hs[a], s2 = 1.0 / (hs[a] * hs[a]);
-    for (int m = 0; m < 13; m++) {
-      s.sc.d1[a][m] = (T)(c1[m] * s1);
-      s.sc.d2[a][m] = (T)(c2[m] * s2);
+    for (int k = 0; k <= 6; k++) {  // c1 is odd and c2 even in the offset, exactly (fd_coeffs)
+      s.sc.h1[a][k] = (T)(c1[6 + k] * s1);
+      s.sc.h2[a][k] = (T)(c2[6 + k] * s2);
     }
   }
   std::vector<int> ring(B * 8);

@@ -30,8 +30,16 @@ struct Grid {
   EP_DEV long idx(int x, int y, int z) const { return (long)x + (long)nx * ((long)y + (long)ny * z); }
 };
 
-// Scaled 13-point coefficients for each axis (c_m / h^id), first and second order.
-template <typename T> struct StenCoef { T d1[3][13]; T d2[3][13]; };
+// Scaled 13-point coefficients for each axis (c_m / h^id), first and second
+// order, indexed by m = 0..12 (offset m - 6).  The first-derivative stencil is
+// odd and the second even in the offset, exactly, so only offsets 0..6 are
+// kept (h1, h2): half the kernel-argument words, which the step kernels hold
+// in scalar registers, and the same values as the full tables.
+template <typename T> struct StenCoef {
+  T h1[3][7], h2[3][7];  // [axis][offset 0..6]
+  EP_DEV T d1(int a, int m) const { return m >= 6 ? h1[a][m - 6] : -h1[a][6 - m]; }
+  EP_DEV T d2(int a, int m) const { return h2[a][m >= 6 ? m - 6 : 6 - m]; }
+};
 
 // Offset of column (x, y) within a plane, wrapped for icon 8; -1 outside the
 // grid for icon 13.
@@ -149,9 +157,9 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
       V ax = zero, ay = zero, az = zero;
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax = vfma(sc.d1[0][m], pl[(ty + 6) * LX + tx + m], ax);
-        ay = vfma(sc.d1[1][m], pl[(ty + m) * LX + tx + 6], ay);
-        az = vfma(sc.d1[2][m], q[m], az);
+        ax = vfma(sc.d1(0, m), pl[(ty + 6) * LX + tx + m], ax);
+        ay = vfma(sc.d1(1, m), pl[(ty + m) * LX + tx + 6], ay);
+        az = vfma(sc.d1(2, m), q[m], az);
       }
       d1[0] = ax; d1[1] = ay; d1[2] = az;
     }
@@ -159,9 +167,9 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
       V ax = zero, ay = zero, az = zero;
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax = vfma(sc.d2[0][m], pl[(ty + 6) * LX + tx + m], ax);
-        ay = vfma(sc.d2[1][m], pl[(ty + m) * LX + tx + 6], ay);
-        az = vfma(sc.d2[2][m], q[m], az);
+        ax = vfma(sc.d2(0, m), pl[(ty + 6) * LX + tx + m], ax);
+        ay = vfma(sc.d2(1, m), pl[(ty + m) * LX + tx + 6], ay);
+        az = vfma(sc.d2(2, m), q[m], az);
       }
       d2[0] = ax; d2[1] = ay; d2[2] = az;
     }
@@ -471,9 +479,9 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
       T ax = T(0), ay = T(0), az = T(0);
 #pragma unroll
       for (int m = 0; m < 13; m++) {
-        ax += a.sc.d1[0][m] * px[ty * LX + tx + m];
-        ay += a.sc.d1[1][m] * py[(ty + m) * TX + tx];
-        az += a.sc.d1[2][m] * q[m];
+        ax += a.sc.d1(0, m) * px[ty * LX + tx + m];
+        ay += a.sc.d1(1, m) * py[(ty + m) * TX + tx];
+        az += a.sc.d1(2, m) * q[m];
       }
       const long i = g.idx(tp.x, tp.y, z);
       div[i] = (ax + ay) + az;
