@@ -457,7 +457,6 @@ int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc);
 /* Synchronise the context's stream. */
 int effpot_sync(effpot_ctx* ctx);
 
-#ifdef __cplusplus
 /* ---- wavefunction snapshots (host I/O, no context) ------------------------
  * effpot_snapshot_read replaces readenc mode 3 (readenc.f90:151-168 with
  * titols.f90:12-37): it reads the formatted file (leading `#` lines skipped,
@@ -470,6 +469,7 @@ int effpot_sync(effpot_ctx* ctx);
 int effpot_snapshot_read(const char* path, effpot_snapshot* hdr, double* psi);
 int effpot_snapshot_write(const char* path, const effpot_snapshot* hdr, const double* psi, int binary);
 
+#ifdef __cplusplus
 }
 #endif
 #endif /* EFFPOT_H */

@@ -37,3 +37,34 @@ def test_footprint_is_host_only():
     assert b > 4 * 128 ** 3 * 8 * 30
     p.precision = ep.PREC_MIXED
     assert ep.footprint_bytes(p) > b
+
+
+def prototypes():
+    """(name, number of parameters) of every function declared in effpot.h."""
+    txt = open(os.path.join(ROOT, "include", "effpot.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = []
+    for m in re.finditer(r"\b(effpot_[a-z0-9_]+)\s*\(([^;{}]*?)\)\s*;", txt, flags=re.S):
+        args = m.group(2).strip()
+        n = 0 if args in ("", "void") else args.count(",") + 1
+        out.append((m.group(1), n))
+    return out
+
+
+def test_header_compiles_as_c11_and_links(tmp_path):
+    """A C11 translation unit that calls every entry point compiles with
+    implicit declarations as errors (so none is hidden from C) and links
+    against libeffpot.so (so every one is exported).  Nothing is executed."""
+    import subprocess
+    protos = prototypes()
+    assert len(protos) == len(declared())
+    calls = "\n".join(f"  (void){name}({', '.join(['0'] * n)});" for name, n in protos)
+    src = tmp_path / "abi_c11.c"
+    src.write_text('#include "effpot.h"\n\nint effpot_abi_touch(void);\nint effpot_abi_touch(void) {\n'
+                   f"{calls}\n  return 0;\n}}\n\nint main(void) {{ return 0; }}\n")
+    exe = tmp_path / "abi_c11"
+    cmd = ["gcc", "-std=c11", "-Wall", "-Werror=implicit-function-declaration", "-Werror=int-conversion",
+           "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           "-L", os.path.dirname(ep.LIB_PATH), "-leffpot", "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
