@@ -25,7 +25,11 @@
  * Pins (tests/test_oracle_*.py): numpy.fft, Fornberg closed forms and the
  * reference's own 13-point tables (mates.f:786-879), scipy.special.sici and
  * scipy quadrature of the LJ transform, the uniform-density known answer,
- * analytic Gaussian kinetic energy, RK-Gill / Hamming amplification factors.
+ * analytic Gaussian kinetic energy, RK-Gill / Hamming amplification factors,
+ * and the variational identity sum(field * drho) dV = d/de E(rho + e drho)
+ * term by term (elj4/delj4, ecor4/correlation + sto1, ealphas/ualphas,
+ * esolid/penalty) on non-uniform droplets, which pins term_alfa and the
+ * correlation convolution (tests/test_oracle_variational.py).
  */
 #include <math.h>
 #include <stdint.h>
