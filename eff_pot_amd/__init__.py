@@ -59,6 +59,7 @@ def lib():
         L.effpot_step_rk.argtypes = [vp, C.c_double]
         L.effpot_step_pc.argtypes = [vp, C.c_double, C.POINTER(C.c_double)]
         L.effpot_renormalize.argtypes = [vp]
+        L.effpot_renormalize_noalloc.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_int, C.c_double, _dp, vp, vp]
         L.effpot_derivnd.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         L.effpot_fft_r2c.argtypes = [vp, _dp, _dp]
         L.effpot_fft_c2r.argtypes = [vp, _dp, _dp]
@@ -202,6 +203,18 @@ def exciplex_moments(v_pi, v_sigma, exciplex="Ring", axis="Z", r_exc=3.9):
     if rc != OK:
         raise EffpotError(rc, "effpot_exciplex_moments: bad arguments")
     return m
+
+
+def renormalize_noalloc(psi_ptr, npoints, nslots, psi_complex, dxyz, n4, den_ptr=None, natoms_ptr=None,
+                        stream=None):
+    """effpot_renormalize_noalloc on device pointers (ints) owned by the
+    caller; n4 is one value per slot.  Enqueues on `stream` (a hipStream_t
+    handle, None: the null stream) and returns without waiting."""
+    n4 = np.ascontiguousarray(np.broadcast_to(np.asarray(n4, dtype=np.float64), (nslots,)))
+    rc = lib().effpot_renormalize_noalloc(psi_ptr, den_ptr, int(npoints), int(nslots), 1 if psi_complex else 0,
+                                          float(dxyz), n4, natoms_ptr, stream)
+    if rc != OK:
+        raise EffpotError(rc, lib().effpot_last_error(None).decode())
 
 
 def footprint_bytes(params: Params) -> int:

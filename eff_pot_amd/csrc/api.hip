@@ -25,6 +25,7 @@
 #include "stencil.hpp"
 #include "state.hpp"
 #include "motion.hpp"
+#include "noalloc.hpp"
 
 using namespace ep;
 
@@ -1596,6 +1597,11 @@ int effpot_renormalize(effpot_ctx* ctx) {
   RC(renorm_t(ctx, ctx->s64, ctx->d_active));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
+}
+
+int effpot_renormalize_noalloc(double* psi, double* den, int64_t npoints, int nslots, int psi_complex, double dxyz,
+                               const double* n4, double* natoms_out, void* stream) {
+  return renormalize_noalloc(psi, den, npoints, nslots, psi_complex, dxyz, n4, natoms_out, stream, snap::g_err);
 }
 
 int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df) {

@@ -258,7 +258,9 @@ typedef struct effpot_ctx effpot_ctx;
 int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out);
 int effpot_ctx_destroy(effpot_ctx* ctx);
 const char* effpot_last_error(const effpot_ctx* ctx); /* NULL: the calling thread's
-                                                        last snapshot error */
+                                                        last error of a call without
+                                                        a context (snapshot I/O,
+                                                        effpot_renormalize_noalloc) */
 /* Device bytes held by a context with these parameters (admission control). */
 int64_t effpot_footprint_bytes(const effpot_params* p);
 
@@ -397,6 +399,23 @@ int effpot_energy(effpot_ctx* ctx, effpot_energies* out);  /* energy.f90:12, out
 int effpot_step_rk(effpot_ctx* ctx, double dt);            /* steprk.f90:1  */
 int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he); /* steppc.f90:1, err_he[batch] or NULL */
 int effpot_renormalize(effpot_ctx* ctx);                   /* psi *= sqrt(n4/N) */
+/* effpot_renormalize on caller-owned device memory that allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  For each of `nslots` slots of `npoints` points
+ * (slot-major; psi interleaved complex when psi_complex, else real):
+ * psi *= sqrt(n4[s] / (sum |psi|^2 dxyz)) in place, den = |psi|^2 (den may be
+ * NULL), natoms_out[s] = sum |psi|^2 dxyz before the scaling (NULL: not
+ * written).  psi, den and natoms_out are device pointers; n4 is host memory
+ * read during the call; `stream` is a hipStream_t (NULL: the null stream) and
+ * the call only enqueues kernels on it.  Limits: nslots <= 64, npoints <=
+ * 256^3; beyond them EFFPOT_ERR_ARG (effpot_last_error(NULL) says why).  The
+ * norm's partial sums live in a static 264 KiB device array per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race.  The first call on a device looks up that array's address;
+ * make it outside graph capture. */
+int effpot_renormalize_noalloc(double* psi, double* den, int64_t npoints, int nslots, int psi_complex, double dxyz,
+                               const double* n4, double* natoms_out, void* stream);
 /* One 13-point derivative (derivnd, init_deriv_parallel.f90:1-46) of a real
  * host field f along axis iv (1..3), order id (1..2), into df (host). */
 int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df);

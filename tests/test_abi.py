@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import eff_pot_amd as ep
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -68,3 +70,18 @@ def test_header_compiles_as_c11_and_links(tmp_path):
            "-L", os.path.dirname(ep.LIB_PATH), "-leffpot", "-Wl,--allow-shlib-undefined"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_renormalize_noalloc_refuses_sizes_beyond_its_scratch():
+    """The refusals happen on the host before any device call."""
+    lib = ep.lib()
+    n4 = np.full(65, 100.0)
+    fake = ctypes.c_void_p(4096)  # never dereferenced: every call below is refused first
+    cases = [(fake, 32 ** 3, 65, "nslots"), (fake, 256 ** 3 + 1, 1, "npoints"), (None, 32 ** 3, 1, "null"),
+             (fake, 0, 1, "npoints"), (fake, 32 ** 3, 0, "nslots")]
+    for psi, npts, ns, word in cases:
+        rc = lib.effpot_renormalize_noalloc(psi, None, npts, ns, 1, 0.1, n4, None, None)
+        assert rc == ep.ERR_ARG
+        assert word in lib.effpot_last_error(None).decode()
+    bad_n4 = np.array([100.0, -1.0])
+    assert lib.effpot_renormalize_noalloc(fake, None, 8, 2, 1, 0.1, bad_n4, None, None) == ep.ERR_ARG
