@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--npos", type=int, default=48)
     ap.add_argument("--nreq", type=int, default=32)
     ap.add_argument("--rate", type=float, default=0.5)
+    ap.add_argument("--serve-budget-gb", type=float, default=96.0,
+                    help="HBM the serving workload may reserve for solver state (admission by footprint)")
     ap.add_argument("--stress-steps", type=int, default=300)
     ap.add_argument("--seed", type=int, default=20261203, help="noise seed of the initial droplets")
     a = ap.parse_args()
@@ -210,14 +212,15 @@ def run_scan(a):
 def run_serve(a):
     from eff_pot_amd import serve
     reqs = serve.make_stream(a.nreq, rate=a.rate, seed=7)
-    plan = {64: 8, 96: 6, 128: 4}
-    srv = serve.Server(plan, tol=a.tol, tol_fp32=a.tol_fp32, max_steps=a.max_steps,
+    budget = int(a.serve_budget_gb * 2 ** 30)
+    srv = serve.Server(budget_bytes=budget, tol=a.tol, tol_fp32=a.tol_fp32, max_steps=a.max_steps,
                        precision=1 if a.mode == "mixed" else 0)
     t0 = time.perf_counter()
     try:
         srv.run(reqs, progress=lambda r: print(f"[serve] {time.perf_counter() - t0:.0f} s: request {r.rid} "
                                                f"grid {r.grid} steps {r.steps} failed {r.failed}",
                                                file=sys.stderr, flush=True))
+        stats = srv.stats()
     finally:
         srv.close()
     el = time.perf_counter() - t0
@@ -227,7 +230,8 @@ def run_serve(a):
                        "fp32 fields + f64" if a.mode == "mixed" else "f64",
                        f"{a.nreq} requests, Poisson rate {a.rate}/s, N log-uniform [100,3000], grids 64/96/128 at 0.75 A",
                        p50_latency_s=serve.percentile(lat, 0.5), p99_latency_s=serve.percentile(lat, 0.99),
-                       slots=plan, converged=len(ok), grids={g: sum(1 for r in reqs if r.grid == g) for g in (64, 96, 128)})
+                       admission=stats, converged=len(ok),
+                       grids={g: sum(1 for r in reqs if r.grid == g) for g in (64, 96, 128)})
 
 
 def run_corrector(a):

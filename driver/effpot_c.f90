@@ -132,11 +132,11 @@ module effpot_c
       type(c_ptr), value :: ctx
       real(c_double), value :: dt
     end function
-    integer(c_int) function effpot_step_pc(ctx, dt, err_he) bind(c, name="effpot_step_pc")
+    integer(c_int) function effpot_step_pc(ctx, dt, err_he, err_imp, err_vimp, err_iv) bind(c, name="effpot_step_pc")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: dt
-      real(c_double), intent(out) :: err_he
+      real(c_double), intent(out) :: err_he, err_imp, err_vimp, err_iv
     end function
     integer(c_int) function effpot_renormalize(ctx) bind(c, name="effpot_renormalize")
       import :: c_int, c_ptr

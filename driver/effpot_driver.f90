@@ -41,6 +41,27 @@ program effpot_driver
   real(8) :: z_exciplex_exclusion = 0.d0   ! energy.f90:155 (modules.f90 default 0)
   real(8) :: tol = 1.d-9
   logical :: fast_io = .true.   ! mode 3: the library's reader instead of list-directed READ
+  ! The other keys of the reference's /input/ (DFT4He3d.f90:104-134; types and
+  ! defaults of modules.f90, DFT4He3d.f90:26-99, V_impur.f90:2-4,51-53): read
+  ! and not used by this driver (real-time, surface, constraint and output
+  ! options of the TDDFT code), so that an input.dat written for the reference,
+  ! as prep.sh edits it, reads unchanged.
+  character(len=15) :: fftwplan = 'FFTW_PATIENT'
+  character(len=60) :: fileout = 'DFT4He3d.res', filerimp = 'rimp.out', filevimp = 'vimp.out', &
+                       fileaimp = 'aimp.out', filedenout = 'he4.output.density', fileimpin = 'X.input.wf', &
+                       fileimpout = 'X.output.wf'
+  character(len=80) :: selec_gs = 'Ba_plus_gs'
+  integer :: nsfiles = 1, pchem = 50, irespar = 0, pdenpar = 50, ppot = 1, iron = 1, ironx = 1, norder = 0, &
+             nv_iter = 0, iter0 = 0, pcurr = 0, icurr = 0, ndmax = 2
+  real(8) :: denmin = 1.d-60, psimin = 1.d-30, rimpur = 5.5d0, mimpur = 0.d0, xlamda = 0.d0, xlamdax = 0.d0, &
+             xlamday = 0.d0, xlamdaz = 0.d0, xlamdax_x = 0.d0, xlamdax_y = 0.d0, xlamdax_z = 0.d0, &
+             time0 = -1.d0, zsurf = -25.d0, lambdah = 0.d0, tzmean = 0.d0, tzsurf = 0.d0, vximp = 0.d0, &
+             vyimp = 0.d0, vzimp = 0.d0, r_cutoff_gs = 1.05d0, umax_gs = 6.836177d3, ev0 = 0.d0, &
+             quita_c4_ba_plus_gs_fix_c4 = 0.d0, quita_c4_ba_plus_pi_fix_c4 = 0.d0, &
+             quita_c4_ba_plus_sigma_fix_c4 = 0.d0, z_exciplex_position = -3.5d0
+  logical :: printpot = .false., lrkpc = .true., lrk = .false., lsurf = .false., lsurf3d = .false., &
+             lexcite_state = .false., lexcite_state_external = .false., lfix_lambda = .false., &
+             l_v_aditional = .false.
   real(8), parameter :: ktops = 7.63822291d0
   namelist /input/ title, filedenin, core4, selec_pi, n4, mode, nx, ny, nz, npd, icon, niter, pener, &
                    nthread, precision, seed, xmax, ymax, zmax, xc, yc, zc, eps4, sigma4, h_core, l, &
@@ -50,7 +71,15 @@ program effpot_driver
                    instate, r_cutoff_sigma, umax_sigma, r_cutoff_delta, umax_delta, als_p, als_d, &
                    ldiag_jz, ljz, lexcite_state_fix, laverage_p_value, lmixture, a_pi_3o2, a_sig_1o2, &
                    lexciplex_state_fix, exciplex, lexciplex_axis, r_exc, z_exciplex_exclusion, &
-                   lfilter_exciplex_force
+                   lfilter_exciplex_force, &
+                   fftwplan, nsfiles, fileout, filerimp, filevimp, fileaimp, filedenout, fileimpin, &
+                   fileimpout, denmin, psimin, printpot, pchem, irespar, pdenpar, ppot, iron, ironx, &
+                   rimpur, mimpur, norder, nv_iter, lrkpc, lrk, xlamda, xlamdax, xlamday, xlamdaz, &
+                   xlamdax_x, xlamdax_y, xlamdax_z, iter0, time0, zsurf, pcurr, icurr, lsurf, lsurf3d, &
+                   lambdah, tzmean, tzsurf, vximp, vyimp, vzimp, ndmax, selec_gs, r_cutoff_gs, umax_gs, &
+                   ev0, lexcite_state, quita_c4_ba_plus_gs_fix_c4, quita_c4_ba_plus_pi_fix_c4, &
+                   quita_c4_ba_plus_sigma_fix_c4, lexcite_state_external, lfix_lambda, l_v_aditional, &
+                   z_exciplex_position
 
   type(effpot_params) :: p
   type(effpot_impurity) :: imp

@@ -15,13 +15,14 @@ import numpy as np
 from .abi import (Params, Impurity, Plus, StateImp, STATE_P, STATE_D, Energies, SolveOpts, Result, Slot, Snapshot, OK, FIELDS,  # noqa: F401
                   SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED,
                   make_params, make_morse, CORE_OT, CORE_OTC, PREC_FP64, PREC_MIXED,
-                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, H4OT, ERR_ARG,
+                  ICON_PERIODIC, ICON_ZERO, POT_MORSE, POT_NULL, POT_SERIES, POT_KOUTSELOS, H4OT, ERR_ARG,
                   Select, SELECT_PAIR, SELECT_JZ, SELECT_FIX, SELECT_AVERAGE, SELECT_MIXTURE)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeffpot.so")
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 class EffpotError(RuntimeError):
@@ -49,6 +50,10 @@ def lib():
         L.effpot_download_psi.argtypes = [vp, C.c_int, _dp]
         L.effpot_download_den.argtypes = [vp, C.c_int, _dp]
         L.effpot_download_field.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.effpot_poten_fp32.argtypes = [vp]
+        L.effpot_download_field_fp32.argtypes = [vp, C.c_int, C.c_int, _fp]
+        L.effpot_fft_r2c_fp32.argtypes = [vp, _fp, _fp]
+        L.effpot_fft_c2r_fp32.argtypes = [vp, _fp, _fp]
         L.effpot_init_fermi.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _dp]
         L.effpot_set_impurity.argtypes = [vp, C.c_int, C.POINTER(Impurity)]
@@ -57,7 +62,7 @@ def lib():
         L.effpot_poten.argtypes = [vp]
         L.effpot_energy.argtypes = [vp, C.POINTER(Energies)]
         L.effpot_step_rk.argtypes = [vp, C.c_double]
-        L.effpot_step_pc.argtypes = [vp, C.c_double, C.POINTER(C.c_double)]
+        L.effpot_step_pc.argtypes = [vp, C.c_double] + [C.POINTER(C.c_double)] * 4
         L.effpot_renormalize.argtypes = [vp]
         L.effpot_renormalize_noalloc.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_int, C.c_double, _dp, vp, vp]
         L.effpot_derivnd.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
@@ -125,7 +130,8 @@ PLUS_DEFAULT = ("Ba_plus_2D2", 1.05, 6.260015e4)
 
 def make_plus(selec=PLUS_DEFAULT[0], r_cutoff=PLUS_DEFAULT[1], umax=PLUS_DEFAULT[2]) -> Plus:
     """X+-He potential by Select_Pot name ('null', 'Ba_plus_gs', 'Ba_plus_2D0',
-    'Ba_plus_2D1', 'Ba_plus_2D2'); an unknown name raises (Select_Pot stops)."""
+    'Ba_plus_2D1', 'Ba_plus_2D2', '<Na|K|Rb|Cs>_plus_Koutselos'); an unknown
+    name raises (Select_Pot stops)."""
     p = Plus()
     rc = lib().effpot_plus_preset(selec.encode(), r_cutoff, umax, C.byref(p))
     if rc != OK:
@@ -282,6 +288,30 @@ class Context:
         self._chk(lib().effpot_download_field(self.h, slot, FIELDS.get(which, which), out.reshape(-1)))
         return out
 
+    def field_fp32(self, which, slot=0):
+        """A field of the last poten_fp32 (mixed contexts); 'corr' is the defect correction."""
+        out = np.zeros(self.shape, dtype=np.float32)
+        w = 13 if which == "corr" else FIELDS.get(which, which)
+        self._chk(lib().effpot_download_field_fp32(self.h, slot, w, out.reshape(-1)))
+        return out
+
+    def poten_fp32(self):
+        """poten on the FP32 fields of a mixed context (from the live psi)."""
+        self._chk(lib().effpot_poten_fp32(self.h))
+
+    def fft_r2c_fp32(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(self.shape)
+        out = np.zeros(self.hshape + (2,), dtype=np.float32)
+        self._chk(lib().effpot_fft_r2c_fp32(self.h, f.reshape(-1), out.reshape(-1)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def fft_c2r_fp32(self, F):
+        F = np.asarray(F).reshape(self.hshape)
+        buf = np.ascontiguousarray(np.stack([F.real, F.imag], axis=-1), dtype=np.float32)
+        out = np.zeros(self.shape, dtype=np.float32)
+        self._chk(lib().effpot_fft_c2r_fp32(self.h, buf.reshape(-1), out.reshape(-1)))
+        return out
+
     def init_fermi(self, rho0, R, afermi, noise, seed, centre=(0.0, 0.0, 0.0), slot=0):
         self._chk(lib().effpot_init_fermi(self.h, slot, rho0, R, afermi, noise, seed,
                                           np.asarray(centre, dtype=np.float64)))
@@ -353,10 +383,17 @@ class Context:
     def step_rk(self, dt):
         self._chk(lib().effpot_step_rk(self.h, dt))
 
-    def step_pc(self, dt):
+    def step_pc(self, dt, impurity_errors=False):
+        """One predictor-corrector step; returns errHe per slot, or with
+        impurity_errors the lists (errHe, errimp, errvimp, erriv) of
+        steppc(deltat, errHe, errimp, errvimp, erriv)."""
         err = (C.c_double * self.B)()
-        self._chk(lib().effpot_step_pc(self.h, dt, err))
-        return list(err)
+        if not impurity_errors:
+            self._chk(lib().effpot_step_pc(self.h, dt, err, None, None, None))
+            return list(err)
+        e_r, e_v, e_i = (C.c_double * self.B)(), (C.c_double * self.B)(), (C.c_double * self.B)()
+        self._chk(lib().effpot_step_pc(self.h, dt, err, e_r, e_v, e_i))
+        return list(err), list(e_r), list(e_v), list(e_i)
 
     def renormalize(self):
         self._chk(lib().effpot_renormalize(self.h))

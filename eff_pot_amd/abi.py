@@ -9,7 +9,7 @@ OK = 0
 ERR_ARG, ERR_GRID, ERR_CORE, ERR_POT, ERR_DEVICE, ERR_NONFINITE, ERR_STATE, ERR_NOMEM, ERR_DT = range(1, 10)
 CORE_OT, CORE_OTC = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-POT_NULL, POT_MORSE, POT_SERIES = 0, 1, 2
+POT_NULL, POT_MORSE, POT_SERIES, POT_KOUTSELOS = 0, 1, 2, 3
 SERIES_MAX = 16
 ICON_PERIODIC, ICON_ZERO = 8, 13
 

@@ -1149,6 +1149,31 @@ int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out) {
   return EFFPOT_OK;
 }
 
+int effpot_poten_fp32(effpot_ctx* ctx) {
+  RC(check_ctx(ctx));
+  if (!ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "FP32 fields exist in a mixed-precision context only");
+  hipSetDevice(ctx->dev);
+  RC(den_from_cur(ctx, ctx->d_active, true));
+  RC(poten_t(ctx, ctx->s32, ctx->d_active));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_download_field_fp32(effpot_ctx* ctx, int slot, int which, float* out) {
+  RC(check_ctx(ctx));
+  if (!ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "FP32 fields exist in a mixed-precision context only");
+  if (!out || slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or pointer");
+  State<float>& s = ctx->s32;
+  float* src[14] = {s.pot4, s.dencg, s.delj4, s.denalf, s.ualphas, s.uimp, s.dx, s.dy, s.dz,
+                    s.sto1, s.intx, s.inty, s.intz, s.corr};
+  if (which < 0 || which > 13) return ctx->fail(EFFPOT_ERR_ARG, "unknown field");
+  HIPCHK(hipMemcpyAsync(out, src[which] + (long)slot * ctx->N, sizeof(float) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (which == 3)  // the device keeps falfs = 1 - denalf/den0s
+    for (long i = 0; i < ctx->N; i++) out[i] = (float)((1.0 - (double)out[i]) * ctx->p.den0s);
+  return EFFPOT_OK;
+}
+
 // Counter-based uniform in [0,1): splitmix64 finaliser of the seed and index.
 static uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ULL;
@@ -1267,7 +1292,7 @@ int effpot_exciplex_moments(const effpot_plus* v_pi, const effpot_plus* v_sigma,
   if (!v_pi || !v_sigma || !moments || (n_he != 2 && n_he != 6) || axis < 0 || axis > 2 || !(r_exc > 0.0))
     return EFFPOT_ERR_ARG;
   for (const effpot_plus* v : {v_pi, v_sigma})
-    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES) ||
+    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES && v->kind != EFFPOT_POT_KOUTSELOS) ||
         (v->kind == EFFPOT_POT_SERIES && (v->nterms < 1 || v->nterms > EFFPOT_SERIES_MAX || v->k0 < 0)))
       return EFFPOT_ERR_POT;
   // the two axes of the ring plane, lower first (instates.f90:72-81)
@@ -1302,7 +1327,7 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
   if (imp->instate < -1 || imp->instate >= n / 2)
     return ctx->fail(EFFPOT_ERR_ARG, "instate out of range (P: 0..2, D: 0..4, or -1 for a given invar)");
   for (const effpot_plus* v : {&imp->v_pi, &imp->v_sigma, &imp->v_delta})
-    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES) ||
+    if ((v->kind != EFFPOT_POT_NULL && v->kind != EFFPOT_POT_SERIES && v->kind != EFFPOT_POT_KOUTSELOS) ||
         (v->kind == EFFPOT_POT_SERIES && (v->nterms < 1 || v->nterms > EFFPOT_SERIES_MAX || v->k0 < 0)))
       return ctx->fail(EFFPOT_ERR_POT, "bad state potential");
   hipSetDevice(ctx->dev);
@@ -1504,13 +1529,13 @@ int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus) {
   if (slot < 0 || slot >= ctx->B) return ctx->fail(EFFPOT_ERR_ARG, "bad slot");
   PlusSlot& ps = ctx->h_plus[slot];
   if (plus) {
-    if (plus->kind != EFFPOT_POT_NULL && plus->kind != EFFPOT_POT_SERIES)
+    if (plus->kind != EFFPOT_POT_NULL && plus->kind != EFFPOT_POT_SERIES && plus->kind != EFFPOT_POT_KOUTSELOS)
       return ctx->fail(EFFPOT_ERR_POT, "unknown X+ potential kind");
     if (plus->kind == EFFPOT_POT_SERIES &&
         (plus->nterms < 1 || plus->nterms > EFFPOT_SERIES_MAX || plus->k0 < 0 || plus->k0 > 64))
       return ctx->fail(EFFPOT_ERR_ARG, "X+ series: nterms in 1..16 and k0 in 0..64");
     ps.p = *plus;
-    ps.on = plus->kind == EFFPOT_POT_SERIES;
+    ps.on = plus->kind != EFFPOT_POT_NULL;
   } else {
     std::memset(&ps.p, 0, sizeof(ps.p));
     ps.on = 0;
@@ -1576,7 +1601,7 @@ int effpot_step_rk(effpot_ctx* ctx, double dt) {
   return step_rk_t(ctx, ctx->s64, ctx->d_active);
 }
 
-int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he) {
+int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he, double* err_imp, double* err_vimp, double* err_iv) {
   RC(check_ctx(ctx));
   if (!(dt > 0)) return ctx->fail(EFFPOT_ERR_DT, "non-positive time step");
   hipSetDevice(ctx->dev);
@@ -1586,6 +1611,31 @@ int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he) {
     HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_red, sizeof(double) * ctx->B * 2, hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     for (int b = 0; b < ctx->B; b++) err_he[b] = ctx->h_pin[2 * b] / (double)ctx->N;
+  }
+  if (err_imp || err_vimp || err_iv) {
+    // steppc.f90:179,193,213-220, from the corrector differences the step left
+    // in the slots' motion and invar records (0 for a fixed impurity, a frozen invar)
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    std::vector<Motion> mo(ctx->B);
+    std::vector<InvarDyn> dy(ctx->B);
+    HIPCHK(hipMemcpy(mo.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dy.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
+    const double c9 = 9.0 / 121.0;
+    for (int b = 0; b < ctx->B; b++) {
+      double er = 0.0, ev = 0.0, ei = 0.0;
+      if (mo[b].on) {
+        for (int c = 0; c < 3; c++) { er += std::fabs(c9 * mo[b].pcr[c]); ev += std::fabs(c9 * mo[b].pcv[c]); }
+        er *= 0.3333333333;
+        ev *= 0.3333333333;
+      }
+      if (dy[b].on) {
+        for (int k = 0; k < dy[b].n; k++) ei += c9 * std::hypot(dy[b].pciv[2 * k], dy[b].pciv[2 * k + 1]);
+        ei *= dy[b].state == EFFPOT_STATE_P ? 0.1666666666666 : 0.1;
+      }
+      if (err_imp) err_imp[b] = er;
+      if (err_vimp) err_vimp[b] = ev;
+      if (err_iv) err_iv[b] = ei;
+    }
   }
   return EFFPOT_OK;
 }
@@ -1626,48 +1676,71 @@ int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df)
   return EFFPOT_OK;
 }
 
-int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out) {
-  RC(check_ctx(ctx));
-  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
-  hipSetDevice(ctx->dev);
-  State<double>& s = ctx->s64;
+// The 3-D transforms of effpot_fft_r2c/c2r in the precision T of state s
+// (slot 0's source field and first spectrum as staging).
+template <typename T> static int fft_r2c_t(effpot_ctx* ctx, State<T>& s, const T* in, T* out) {
   std::vector<int> m(ctx->B, 0);
   m[0] = 1;
   HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipMemcpyAsync(s.src[0], in, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->st));
-  double* r[1] = {s.src[0]};
-  cx<double>* c[1] = {s.spec[0]};
-  if (plane_ok<double>(ctx)) {
+  HIPCHK(hipMemcpyAsync(s.src[0], in, sizeof(T) * ctx->N, hipMemcpyHostToDevice, ctx->st));
+  T* r[1] = {s.src[0]};
+  cx<T>* c[1] = {s.spec[0]};
+  if (plane_ok<T>(ctx)) {
     RC(fft_plane_fwd(ctx, s, r, c, 1, ctx->d_mask_tmp));
   } else {
     RC(fft_x_r2c(ctx, s, r, c, 1, ctx->d_mask_tmp));
     RC(fft_strided(ctx, s, 1, c, 1, -1, ctx->d_mask_tmp));
   }
   RC(fft_strided(ctx, s, 2, c, 1, -1, ctx->d_mask_tmp));
-  HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<double>) * ctx->nxh, s.spec[0], sizeof(cx<double>) * ctx->nxp,
-                          sizeof(cx<double>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<T>) * ctx->nxh, s.spec[0], sizeof(cx<T>) * ctx->nxp,
+                          sizeof(cx<T>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
+}
+template <typename T> static int fft_c2r_t(effpot_ctx* ctx, State<T>& s, const T* in, T* out) {
+  std::vector<int> m(ctx->B, 0);
+  m[0] = 1;
+  HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpy2DAsync(s.spec[0], sizeof(cx<T>) * ctx->nxp, in, sizeof(cx<T>) * ctx->nxh,
+                          sizeof(cx<T>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyHostToDevice, ctx->st));
+  cx<T>* c[1] = {s.spec[0]};
+  T* r[1] = {s.src[0]};
+  RC(fft_strided(ctx, s, 2, c, 1, +1, ctx->d_mask_tmp));
+  RC(fft_strided(ctx, s, 1, c, 1, +1, ctx->d_mask_tmp));
+  RC(fft_x_c2r(ctx, s, c, r, 1, (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz)), ctx->d_mask_tmp));
+  HIPCHK(hipMemcpyAsync(out, s.src[0], sizeof(T) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return EFFPOT_OK;
+}
+
+int effpot_fft_r2c_fp32(effpot_ctx* ctx, const float* in, float* out) {
+  RC(check_ctx(ctx));
+  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  if (!ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "FP32 transforms need a mixed-precision context");
+  hipSetDevice(ctx->dev);
+  return fft_r2c_t(ctx, ctx->s32, in, out);
+}
+
+int effpot_fft_c2r_fp32(effpot_ctx* ctx, const float* in, float* out) {
+  RC(check_ctx(ctx));
+  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  if (!ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "FP32 transforms need a mixed-precision context");
+  hipSetDevice(ctx->dev);
+  return fft_c2r_t(ctx, ctx->s32, in, out);
+}
+
+int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out) {
+  RC(check_ctx(ctx));
+  if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
+  hipSetDevice(ctx->dev);
+  return fft_r2c_t(ctx, ctx->s64, in, out);
 }
 
 int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out) {
   RC(check_ctx(ctx));
   if (!in || !out) return ctx->fail(EFFPOT_ERR_ARG, "null pointer");
   hipSetDevice(ctx->dev);
-  State<double>& s = ctx->s64;
-  std::vector<int> m(ctx->B, 0);
-  m[0] = 1;
-  HIPCHK(hipMemcpyAsync(ctx->d_mask_tmp, m.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipMemcpy2DAsync(s.spec[0], sizeof(cx<double>) * ctx->nxp, in, sizeof(cx<double>) * ctx->nxh,
-                          sizeof(cx<double>) * ctx->nxh, (size_t)ctx->ny * ctx->nz, hipMemcpyHostToDevice, ctx->st));
-  cx<double>* c[1] = {s.spec[0]};
-  double* r[1] = {s.src[0]};
-  RC(fft_strided(ctx, s, 2, c, 1, +1, ctx->d_mask_tmp));
-  RC(fft_strided(ctx, s, 1, c, 1, +1, ctx->d_mask_tmp));
-  RC(fft_x_c2r(ctx, s, c, r, 1, 1.0 / ((double)ctx->nx * ctx->ny * ctx->nz), ctx->d_mask_tmp));
-  HIPCHK(hipMemcpyAsync(out, s.src[0], sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
-  return EFFPOT_OK;
+  return fft_c2r_t(ctx, ctx->s64, in, out);
 }
 
 // ---- continuous-batching engine ---------------------------------------------------

@@ -66,7 +66,7 @@ EP_DEV double fvlj_dev(double p, double eps, double sigma, double hc) {
   double t2 = -(a * bd5 * cy + cd5 * sy / hc) / (24.0 * hc * hc * hc) + pow(a, 4.0) * sint / 24.0;
   return (8.0 * eps * s6 * (s6 * t1 - t2)) / p;
 }
-// initcg.f90:314-331
+// initcg.f90:32-49
 EP_DEV double wcg_dev(double p, double h) {
   if (!(p > 0.0)) return 1.0;
   double arg = 2.0 * 3.14159265358979323846 * p * h;

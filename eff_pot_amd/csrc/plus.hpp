@@ -44,9 +44,41 @@ __host__ __device__ inline DD dd_mul(DD a, DD b) {
   return dd_fast_two_sum(p, e);
 }
 
+// The Koutselos alkali-ion - He form (V_Koutselos_*_plus_gs,
+// V_impur.f90:1704-1883), in atomic units: x = r / a_bohr,
+//   V = [v0 (A e^{-a x/rho} - B e^{-b x/rho})
+//        + (-CC4/x^4 - CC6/x^6 - CC8/x^8) h(x)] * Hartree * eV_K,
+// h = 1 for x >= 1.28 R_m, else exp(-(1.28 R_m / x - 1)^2) (0 at x = 0).
+// Parameters in effpot_plus.pg (EFFPOT_KOUT_*), operations in the reference's
+// order, FP64 as the reference evaluates it.
+__host__ __device__ inline double koutselos_pot(const effpot_plus& p, double r) {
+#pragma clang fp contract(off)
+  const double* c = p.pg;
+  const double x = r / c[EFFPOT_KOUT_ABOHR];
+  const double rs = x / c[EFFPOT_KOUT_RHO];
+  const double vex = c[EFFPOT_KOUT_V0] * (c[EFFPOT_KOUT_A] * exp(-c[EFFPOT_KOUT_ALPHA] * rs) -
+                                          c[EFFPOT_KOUT_B] * exp(-c[EFFPOT_KOUT_BETA] * rs));
+  const double aux = c[EFFPOT_KOUT_RM] * 1.28;
+  double h = 1.0;
+  if (!(x >= aux)) {
+    h = 0.0;
+    if (x != 0.0) {
+      const double t = aux / x - 1.0;
+      h = exp(-(t * t));
+    }
+  }
+  double sto = vex;
+  if (x != 0.0) {
+    const double x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x8 = x6 * x2;
+    sto = sto + (-c[EFFPOT_KOUT_CC4] / x4 - c[EFFPOT_KOUT_CC6] / x6 - c[EFFPOT_KOUT_CC8] / x8) * h;
+  }
+  return sto * c[EFFPOT_KOUT_HARTREE] * c[EFFPOT_KOUT_EVK];
+}
+
 // Select_Pot for an effpot_plus (host and device, FP64 in, FP64 out).
 __host__ __device__ inline double plus_pot(const effpot_plus& p, double r) {
 #pragma clang fp contract(off)
+  if (p.kind == EFFPOT_POT_KOUTSELOS) return r < p.r_cutoff ? p.umax : koutselos_pot(p, r);
   if (p.kind != EFFPOT_POT_SERIES) return 0.0;
   if (r < p.r_cutoff) return p.umax;
   if (r <= p.rcutoff) return p.fcutoff;
@@ -139,6 +171,20 @@ static const Fit kFits[] = {
       "-7681295358440.52888750989831902749",
       "2696390604822.62910359945307774040"}},
 };
+// The Koutselos et al. (J. Chem. Phys. 93, 7125 (1990)) He - alkali-ion
+// parameters v0, rho, R_m, C4, C6, C8 (atomic units) of V_Koutselos_*_plus_gs
+// (Cs :1704-1749, Na :1750-1795, Rb :1796-1839, K :1840-1883), selected as
+// '<Ion>_plus_Koutselos' (V_impur.f90:232-292).
+struct Kout {
+  const char* name;
+  double v0, rho, rm, c4, c6, c8;
+};
+static const Kout kKout[] = {
+    {"Cs_plus_Koutselos", 0.5092, 0.9241, 6.38, 0.0, 13.69, 236.4},
+    {"Na_plus_Koutselos", 0.2063, 0.7640, 4.55, 0.0, 1.424, 13.17},
+    {"Rb_plus_Koutselos", 0.3474, 0.8868, 5.85, 0.0, 8.8, 126.2},
+    {"K_plus_Koutselos", 0.2690, 0.8589, 5.49, 0.0, 5.83, 72.05},
+};
 }  // namespace plus_fits
 
 // A decimal string ("-123.456...", no exponent) to a double-double: the
@@ -182,6 +228,20 @@ inline bool plus_preset(const char* selec, double r_cutoff, double umax, effpot_
   out->umax = umax;
   if (len == 4 && std::strncmp(selec, "null", 4) == 0) {
     out->kind = EFFPOT_POT_NULL;
+    return true;
+  }
+  for (const plus_fits::Kout& f : plus_fits::kKout) {
+    if (std::strlen(f.name) != len || std::strncmp(selec, f.name, len) != 0) continue;
+    out->kind = EFFPOT_POT_KOUTSELOS;
+    double* c = out->pg;
+    c[EFFPOT_KOUT_V0] = f.v0; c[EFFPOT_KOUT_RHO] = f.rho; c[EFFPOT_KOUT_RM] = f.rm;
+    // CC_n = C_n + alpha_n / 2 with the He polarisabilities (V_impur.f90:1824)
+    c[EFFPOT_KOUT_CC4] = f.c4 + 1.3831 / 2.0;
+    c[EFFPOT_KOUT_CC6] = f.c6 + 2.4434 / 2.0;
+    c[EFFPOT_KOUT_CC8] = f.c8 + 10.614 / 2.0;
+    c[EFFPOT_KOUT_A] = 146.98; c[EFFPOT_KOUT_ALPHA] = 1.5024;
+    c[EFFPOT_KOUT_B] = 70.198; c[EFFPOT_KOUT_BETA] = 1.4041;
+    c[EFFPOT_KOUT_ABOHR] = 0.529177; c[EFFPOT_KOUT_HARTREE] = 27.2114; c[EFFPOT_KOUT_EVK] = 11604.448;
     return true;
   }
   for (const plus_fits::Fit& f : plus_fits::kFits) {

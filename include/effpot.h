@@ -48,6 +48,7 @@ extern "C" {
 #define EFFPOT_POT_NULL 0  /* Select_Pot 'null' (V_impur.f90:171-176): U = 0, r_cutoff ignored */
 #define EFFPOT_POT_MORSE 1 /* isotropic Morse D[e^{-2a(r-re)} - 2e^{-a(r-re)}] */
 #define EFFPOT_POT_SERIES 2 /* inverse-power fit (X+-He potentials, see effpot_plus) */
+#define EFFPOT_POT_KOUTSELOS 3 /* Koutselos alkali-ion form (X+-He, see effpot_plus) */
 
 #define EFFPOT_ICON_PERIODIC 8 /* derivnd icon=8  (init_deriv_parallel.f90:529-634) */
 #define EFFPOT_ICON_ZERO 13    /* derivnd icon=13 (redef.f:133-170): f=0 outside  */
@@ -103,10 +104,31 @@ typedef struct effpot_impurity {
  *   otherwise                sum_{k=1..nterms} pg[k-1] r^-(k+k0)
  * The fits' coefficients carry 33 digits; pg + pg_lo holds them as a
  * double-double pair and the device sums the series in double-double, so the
- * cancellation between terms of 1e10 K near r0 costs no precision. */
+ * cancellation between terms of 1e10 K near r0 costs no precision.
+ * Kind EFFPOT_POT_KOUTSELOS is the form of V_Koutselos_*_plus_gs
+ * (V_impur.f90:1704-1883): r < r_cutoff -> umax, otherwise, with
+ * x = r / a_bohr, E_h = Hartree * eV_K,
+ *   [v0 (A e^{-alpha x/rho} - B e^{-beta x/rho})
+ *    + (-CC4/x^4 - CC6/x^6 - CC8/x^8) h(x)] E_h,
+ *   h = 1 for x >= 1.28 R_m, else exp(-(1.28 R_m/x - 1)^2),
+ * its parameters in pg[EFFPOT_KOUT_*] (nterms, k0 and the plateau and
+ * parabola fields unused). */
 #define EFFPOT_SERIES_MAX 16
+#define EFFPOT_KOUT_V0 0
+#define EFFPOT_KOUT_RHO 1
+#define EFFPOT_KOUT_RM 2
+#define EFFPOT_KOUT_CC4 3
+#define EFFPOT_KOUT_CC6 4
+#define EFFPOT_KOUT_CC8 5
+#define EFFPOT_KOUT_A 6
+#define EFFPOT_KOUT_ALPHA 7
+#define EFFPOT_KOUT_B 8
+#define EFFPOT_KOUT_BETA 9
+#define EFFPOT_KOUT_ABOHR 10   /* a_bohr in A                              */
+#define EFFPOT_KOUT_HARTREE 11 /* Hartree in eV                            */
+#define EFFPOT_KOUT_EVK 12     /* eV in K                                  */
 typedef struct effpot_plus {
-  int32_t kind;       /* EFFPOT_POT_NULL | EFFPOT_POT_SERIES               */
+  int32_t kind;       /* EFFPOT_POT_NULL | EFFPOT_POT_SERIES | EFFPOT_POT_KOUTSELOS */
   int32_t nterms;     /* series terms, 1..EFFPOT_SERIES_MAX                */
   int32_t k0;         /* power offset, >= 0                                */
   int32_t pad_;
@@ -272,6 +294,16 @@ int effpot_download_den(effpot_ctx* ctx, int slot, double* den);
  * 4 ualphas, 5 uimp, 6 dxden, 7 dyden, 8 dzden, 9 sto1 (correlation
  * convolution), 10 intxalf, 11 intyalf, 12 intzalf. */
 int effpot_download_field(effpot_ctx* ctx, int slot, int which, double* out);
+/* The FP32 mean field of a mixed-precision context (no reference
+ * counterpart: the reference is FP64 throughout): effpot_poten_fp32 runs
+ * poten (poten.f90:5-131, term_alfa.f90:5-109) on the FP32 fields from the
+ * live psi of every active slot, as the FP32 phase of a mixed solve does;
+ * its pot4 includes the slot's defect correction, which is zero until an
+ * energy check of effpot_tick has set it.  effpot_download_field_fp32 reads
+ * a field of it: `which` as in effpot_download_field, plus 13 = the defect
+ * correction itself.  Other contexts get EFFPOT_ERR_ARG. */
+int effpot_poten_fp32(effpot_ctx* ctx);
+int effpot_download_field_fp32(effpot_ctx* ctx, int slot, int which, float* out);
 /* Fermi droplet rho0/(1+exp((r-R)/afermi)) + noise*u(seed,i), psi=sqrt(rho)
  * normalised to n4, centred at (xc,yc,zc).  The reference promises this
  * start (readenc.f90:15-17) but never builds it.  Computed on the host so
@@ -369,7 +401,10 @@ int effpot_set_plus(effpot_ctx* ctx, int slot, const effpot_plus* plus);
 /* Fills `out` for a Select_Pot name (trailing blanks ignored): 'null',
  * 'Ba_plus_gs', 'Ba_plus_2D0', 'Ba_plus_2D1', 'Ba_plus_2D2' (the fits of
  * V_impur.f90:1218-1396; the reference's default selec_plus is 'Ba_plus_2D2'
- * with r_cutoff_plus 1.05, umax_plus 6.260015e4, V_impur.f90:14-16).  An
+ * with r_cutoff_plus 1.05, umax_plus 6.260015e4, V_impur.f90:14-16),
+ * 'Na_plus_Koutselos', 'K_plus_Koutselos', 'Rb_plus_Koutselos' (the one
+ * prep.sh:5-7 selects, with r_cutoff_plus 2.0, umax_plus 6.1105922888e3) and
+ * 'Cs_plus_Koutselos' (V_impur.f90:232-292, 1704-1883).  An
  * unknown name gives EFFPOT_ERR_POT, where Select_Pot stops
  * (V_impur.f90:927-931).  Host only; needs no context. */
 int effpot_plus_preset(const char* selec, double r_cutoff, double umax, effpot_plus* out);
@@ -397,7 +432,13 @@ double effpot_plus_eval(const effpot_plus* plus, double r);
 int effpot_poten(effpot_ctx* ctx);                         /* poten.f90:5   */
 int effpot_energy(effpot_ctx* ctx, effpot_energies* out);  /* energy.f90:12, out[batch] */
 int effpot_step_rk(effpot_ctx* ctx, double dt);            /* steprk.f90:1  */
-int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he); /* steppc.f90:1, err_he[batch] or NULL */
+/* STEPPC(deltat, errHe, errimp, errvimp, erriv) (steppc.f90:1): each output
+ * is [batch] or NULL.  errHe = 9/121 sum|pc| / nxyz (:149-150); errimp and
+ * errvimp = sum|9/121 pcr| / 3 and sum|9/121 pcv| / 3 of a moving impurity's
+ * position and velocity (:179, :193); erriv = sum|9/121 pciv| / 6 (P) or / 10
+ * (D) of an evolving invar (:213-218).  A fixed impurity or a frozen invar
+ * gives 0, as Lfix_lambda does for erriv (:220). */
+int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he, double* err_imp, double* err_vimp, double* err_iv);
 int effpot_renormalize(effpot_ctx* ctx);                   /* psi *= sqrt(n4/N) */
 /* effpot_renormalize on caller-owned device memory that allocates nothing.
  * No counterpart in the reference: it is for callers that cannot allocate
@@ -423,6 +464,10 @@ int effpot_derivnd(effpot_ctx* ctx, int id, int iv, const double* f, double* df)
  * half spectrum (nx/2+1,ny,nz) interleaved; c2r scaled by 1/(nx ny nz). */
 int effpot_fft_r2c(effpot_ctx* ctx, const double* in, double* out);
 int effpot_fft_c2r(effpot_ctx* ctx, const double* in, double* out);
+/* The same transforms with the FP32 kernels of a mixed-precision context
+ * (the ones its FP32 poten runs); float buffers, half spectrum interleaved. */
+int effpot_fft_r2c_fp32(effpot_ctx* ctx, const float* in, float* out);
+int effpot_fft_c2r_fp32(effpot_ctx* ctx, const float* in, float* out);
 /* Slots taking part in the operators above (1 active, 0 frozen). */
 int effpot_set_active(effpot_ctx* ctx, int slot, int active);
 

@@ -272,7 +272,7 @@ double oracle_fvlj(double p, double eps, double sigma, double hc) {
   double t1 = sinx11(a, hc), t2 = sinx5(a, hc);
   return (8.0 * eps * s6 * (s6 * t1 - t2)) / p;
 }
-/* initcg.f90:314-331 */
+/* initcg.f90:32-49 */
 double oracle_wcg(double p, double h) {
   if (!(p > 0.0)) return 1.0;
   double arg = 2.0 * M_PI * p * h;
@@ -555,11 +555,34 @@ int oracle_set_impurity(oracle_t* o, const effpot_impurity* imp) {
 }
 
 /* Select_Pot(selec_plus, r, r_cutoff_plus, umax_plus) for the uplus term
- * (V_impur.f90:87): 'null' (:171-176) or an inverse-power fit of the
+ * (V_impur.f90:87): 'null' (:171-176), the Koutselos ion form (:258-262,
+ * :1796-1839, FP64 as the reference evaluates it) or an inverse-power fit of the
  * V_Ba_plus_* form (:1238-1261): plateau, parabola, then the series summed
  * as the reference does, Aux = xq^(k0+1), Sto += pg(k)*Aux, Aux *= xq, in
  * REAL*16 (__float128 here; the coefficient is pg + pg_lo). */
+static double koutselos(const double* c, double xx) {
+  /* V_Koutselos_*_plus_gs (V_impur.f90:1796-1839 for Rb+): atomic units inside */
+  double x = xx / c[EFFPOT_KOUT_ABOHR];
+  double rs = x / c[EFFPOT_KOUT_RHO];
+  double v_ex = c[EFFPOT_KOUT_V0] * (c[EFFPOT_KOUT_A] * exp(-c[EFFPOT_KOUT_ALPHA] * rs) -
+                                     c[EFFPOT_KOUT_B] * exp(-c[EFFPOT_KOUT_BETA] * rs));
+  double aux = c[EFFPOT_KOUT_RM] * 1.28, h_r;
+  if (x >= aux) {
+    h_r = 1.0;
+  } else {
+    h_r = 0.0;
+    if (x != 0.0) h_r = exp(-pow(aux / x - 1.0, 2));
+  }
+  double sto = v_ex;
+  if (x != 0.0) {
+    double x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x8 = x6 * x2;
+    sto = sto + (-c[EFFPOT_KOUT_CC4] / x4 - c[EFFPOT_KOUT_CC6] / x6 - c[EFFPOT_KOUT_CC8] / x8) * h_r;
+  }
+  return sto * c[EFFPOT_KOUT_HARTREE] * c[EFFPOT_KOUT_EVK];
+}
+
 double oracle_plus_pot(const effpot_plus* p, double r) {
+  if (p->kind == EFFPOT_POT_KOUTSELOS) return r < p->r_cutoff ? p->umax : koutselos(p->pg, r);
   if (p->kind != EFFPOT_POT_SERIES) return 0.0;
   if (r < p->r_cutoff) return p->umax;
   if (r <= p->rcutoff) return p->fcutoff;
@@ -942,7 +965,7 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
           double r = sqrt(rx * rx + ry * ry + rz * rz);
           if (r >= o->excl) {
             ehex += a3 * o->uimp[i];
-            if (o->plus.kind == EFFPOT_POT_SERIES) upl += a3 * oracle_plus_pot(&o->plus, r);
+            if (o->plus.kind != EFFPOT_POT_NULL) upl += a3 * oracle_plus_pot(&o->plus, r);
           }
         }
         nat += a3;
@@ -1268,6 +1291,28 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
   if (err_he) *err_he = err / (double)o->N;
   if (!den_ok(o)) return EFFPOT_ERR_NONFINITE;
   return EFFPOT_OK;
+}
+
+/* The impurity's error estimates of the last steppc (steppc.f90:179,193,
+ * 213-220): errimp = sum |c9 pcr| * 0.3333333333, errvimp = sum |c9 pcv| *
+ * 0.3333333333, erriv = sum |c9 pciv| * 0.1666666666666 (P, components 1-6)
+ * or * 0.1 (D); 0 for a fixed impurity or a frozen invar (Lfix_lambda). */
+void oracle_step_errors(const oracle_t* o, double* out3) {
+  const double c9 = 9.0 / 121.0;
+  out3[0] = out3[1] = out3[2] = 0.0;
+  if (o->moving) {
+    for (int c = 0; c < 3; c++) {
+      out3[0] += fabs(c9 * o->pcr[c]);
+      out3[1] += fabs(c9 * o->pcv[c]);
+    }
+    out3[0] *= 0.3333333333;
+    out3[1] *= 0.3333333333;
+  }
+  if (o->dyn) {
+    const int n = o->simp.state == EFFPOT_STATE_P ? 6 : 10;
+    for (int k = 0; k < n; k++) out3[2] += c9 * sqrt(o->pciv[2 * k] * o->pciv[2 * k] + o->pciv[2 * k + 1] * o->pciv[2 * k + 1]);
+    out3[2] *= o->simp.state == EFFPOT_STATE_P ? 0.1666666666666 : 0.1;
+  }
 }
 
 /* psi *= sqrt(n4 / (sum |psi|^2 dxyz)); den = |psi|^2. */

@@ -71,6 +71,7 @@ def lib():
         L.oracle_step_rk.argtypes = [C.c_void_p, C.c_double]
         L.oracle_step_pc.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double)]
         L.oracle_renormalize.argtypes = [C.c_void_p]
+        L.oracle_step_errors.argtypes = [C.c_void_p, _dp]
         L.oracle_uniform.argtypes = [C.c_uint64, C.c_uint64]
         L.oracle_uniform.restype = C.c_double
         L.oracle_init_fermi.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
@@ -280,6 +281,12 @@ class Oracle:
         if rc != OK:
             raise FloatingPointError(f"oracle_step_pc: status {rc}")
         return err.value
+
+    def step_errors(self):
+        """(errimp, errvimp, erriv) of the last step_pc (steppc.f90:179,193,213-220)."""
+        e = np.zeros(3)
+        lib().oracle_step_errors(self.h, e)
+        return tuple(e)
 
     def renormalize(self):
         lib().oracle_renormalize(self.h)

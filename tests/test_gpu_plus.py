@@ -1,7 +1,8 @@
 """uplus (X+-He interaction, energy.f90:146-163) on the GPU against the oracle.
 
-Three slots of one context with different X+ potentials (the reference's
-default Ba_plus_2D2, Ba_plus_gs with its own cutoff, none) and impurities on
+Four slots of one context with different X+ potentials (the reference's
+default Ba_plus_2D2, Ba_plus_gs with its own cutoff, none, and the
+Rb_plus_Koutselos of the reference's snapshot workflow) and impurities on
 grid nodes, so that points sit exactly on the r = 2 A boundary; uplus within
 1e-12 relative of the oracle (double-double series on the device, REAL*16-style
 sum in the oracle), the other energies unchanged by the X+ term."""
@@ -14,17 +15,19 @@ from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-PLUS = [("Ba_plus_2D2", 1.05, 6.260015e4), ("Ba_plus_gs", 1.05, 6.836177e3), None]
-RIMP = [(0.0, 0.0, 0.5), (1.0, -0.5, 2.0), (-1.5, 0.0, 0.0)]
+# the fourth is the X+ potential of the reference's snapshot workflow (prep.sh:5-7)
+PLUS = [("Ba_plus_2D2", 1.05, 6.260015e4), ("Ba_plus_gs", 1.05, 6.836177e3), None,
+        ("Rb_plus_Koutselos", 2.0, 6.1105922888e3)]
+RIMP = [(0.0, 0.0, 0.5), (1.0, -0.5, 2.0), (-1.5, 0.0, 0.0), (0.5, 0.5, -1.0)]
 
 
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
 def test_uplus_matches_oracle(gpu_available, precision):
-    print(f"[plus] 32^3 x 3 slots precision={precision}", flush=True)
-    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=3, precision=precision)
+    print(f"[plus] 32^3 x 4 slots precision={precision}", flush=True)
+    p = make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=4, precision=precision)
     with ep.Context(p) as ctx:
         psis = []
-        for s in range(3):
+        for s in range(4):
             ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, 70 + s, slot=s)
             ctx.set_impurity(make_morse(rimp=RIMP[s]), slot=s)
             ctx.set_plus(ep.make_plus(*PLUS[s]) if PLUS[s] else None, slot=s)
@@ -37,7 +40,7 @@ def test_uplus_matches_oracle(gpu_available, precision):
     assert off[0]["uplus"] == 0.0
     for k in ("etot", "eHeX", "ekin4", "natoms"):
         assert off[0][k] == got[0][k]
-    for s in range(3):
+    for s in range(4):
         o = Oracle(make_params(32, xmax=8.0, n4=100.0, core=CORE_OTC, h_core=2.377))
         o.set_psi(psis[s])
         o.set_impurity(make_morse(rimp=RIMP[s]))

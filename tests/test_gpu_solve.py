@@ -246,6 +246,115 @@ def test_fortran_driver_reads_snapshots(gpu_available, tmp_path):
     assert got[0] == got[1] == got[2]
 
 
+# An input.dat as written for the reference's eff_pot run: keys of its /input/
+# namelist (DFT4He3d.f90:104-134) that this driver reads and does not use,
+# plus the Morse impurity and core radius keys of this driver.
+REF_INPUT = """&input
+ title = 'He4 + Rb+ snapshots',
+ fftwplan = 'FFTW_PATIENT',
+ nthread = 20,
+ filedenin = 'he4.input.density',
+ fileout = 'DFT4He3d.res',
+ mode = 0,
+ n4 = 100,
+ nx = 32, ny = 32, nz = 32,
+ xmax = 12.d0, ymax = 12.d0, zmax = 12.d0,
+ core4 = 'OTC', h_core = 2.377d0,
+ selec_pi = 'Morse',
+ npd = 13, icon = 13, niter = 0, pener = 50,
+ lrkpc = .true., lrk = .false., deltat = 1.d-4, mimpur = 85.47d0,
+ vximp = 0.d0, vyimp = 0.d0, vzimp = 0.d0,
+ lsurf = .false., irespar = 0, pdenpar = 50, iter0 = 0, time0 = 0.d0,
+ selec_gs = 'Ba_plus_gs', r_cutoff_gs = 1.05d0, umax_gs = 6.836177d3,
+ L_v_aditional = .true.,
+ Lfix_lambda = .false., Ev0 = 0.d0,
+ z_exciplex_position = -3.5d0
+/
+"""
+
+
+def _prep(text, x_plus, r_cutoff_plus, umax_plus):
+    """The edits prep.sh:9-15 makes to input.dat: the filedenin line and the
+    mode line replaced (djogger.dat, mode 3), the X+ potential keys added
+    after &input, and the L_v_aditional line replaced (.false.)."""
+    out = []
+    for line in text.splitlines():
+        if "filedenin" in line:
+            line = " filedenin\t\t= 'djogger.dat'"
+        elif "mode" in line:
+            line = " mode\t\t\t= 3"
+        elif "L_v_aditional" in line:
+            line = " L_v_aditional\t\t= .false."
+        out.append(line)
+        if "&input" in line:
+            out += [f" selec_plus\t\t= '{x_plus}'", f" r_cutoff_plus\t\t= {r_cutoff_plus}",
+                    f" umax_plus\t\t= {umax_plus}"]
+    return "\n".join(out) + "\n"
+
+
+def _awk(text, pattern, field):
+    """awk '/pattern/{print $field}' on text (one matching line expected)."""
+    hits = [ln.split()[field - 1] for ln in text.splitlines() if re.search(pattern, ln)]
+    assert len(hits) == 1, (pattern, hits)
+    return hits[0]
+
+
+def test_reference_snapshot_workflow(gpu_available, tmp_path):
+    """prep.sh -> loop.sh -> get_impurity_energy.sh on this driver: an
+    input.dat with the reference's keys, edited as prep.sh edits it
+    (Rb_plus_Koutselos, r_cutoff_plus 2.d0, umax_plus 6.1105922888E+03), runs
+    on two formatted snapshots linked to djogger.dat, and every field the
+    scripts' awk lines pick (time, X-He CoM distance, Ekin, U*, U+, E_SO)
+    parses as a number equal to what the Python binding computes."""
+    exe = os.path.join(ROOT, "driver", "effpot_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    print("[driver] reference workflow, Rb_plus_Koutselos", flush=True)
+    (tmp_path / "input.dat").write_text(_prep(REF_INPUT, "Rb_plus_Koutselos", "2.d0", "6.1105922888E+03"))
+    p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377)
+    plus = ep.make_plus("Rb_plus_Koutselos", 2.0, 6.1105922888e3)
+    for k, (rimp, seed, t) in enumerate((((0.25, -0.5, 1.75), 31, 1.5), ((0.0, 0.75, -2.25), 32, 3.0))):
+        with ep.Context(p) as ctx:
+            ctx.init_fermi(0.0218, 2.22 * 100 ** (1 / 3), 0.6, 1e-6, seed)
+            psi = ctx.psi()
+            ctx.set_impurity(make_morse(rimp=rimp))
+            ctx.set_plus(plus)
+            ctx.poten()
+            e = ctx.energy()[0]
+        h = ep.Snapshot()
+        h.nx = h.ny = h.nz = 32
+        h.xmax = h.ymax = h.zmax = 12.0
+        h.hx = h.hy = h.hz = 0.75
+        h.rimp[:] = rimp
+        h.ninvar = 1
+        snap = tmp_path / f"density.{k:03d}.dat"
+        ep.write_snapshot(str(snap), h, psi, binary=False)
+        # the comment lines printoutc writes first (printoutc.f90:92-94)
+        body = snap.read_text()
+        snap.write_text(f"# Iteration {k}\n#  Total evolution time(ps)...:{t:20.12E}\n"
+                        f"#  xcm,ycm,zcm:{0.0:16.8E}{0.0:16.8E}{0.0:16.8E}\n" + body)
+        link = tmp_path / "djogger.dat"
+        if link.exists() or link.is_symlink():
+            link.unlink()
+        link.symlink_to(snap.name)
+        res = subprocess.run([exe], stdin=open(tmp_path / "input.dat"), capture_output=True, text=True,
+                             cwd=tmp_path, timeout=120)
+        assert res.returncode == 0, res.stderr
+        head = "\n".join(link.read_text().splitlines()[:3])
+        time_ps = float(_awk(head, r"Total evolution time", 5))
+        rcm = float(_awk(res.stdout, r"X-HE CoM distance", 4))
+        ekin = float(_awk(res.stdout, r"Kinetic energy \(X", 5))
+        eint = float(_awk(res.stdout, r"Interaction energy \(X\*", 5))
+        eplus = float(_awk(res.stdout, r"Interaction energy \(X\+", 5))
+        eso = float(_awk(res.stdout, r"Spin-Orbit energy", 5))
+        assert time_ps == t
+        d = np.sqrt((rimp[0] - e["xcm"]) ** 2 + (rimp[1] - e["ycm"]) ** 2 + (rimp[2] - e["zcm"]) ** 2)
+        assert abs(rcm - d) < 1e-5
+        assert ekin == 0.0 and eso == 0.0
+        assert abs(eint - e["eHeX"]) < 1e-5
+        assert e["uplus"] != 0.0 and abs(eplus - e["uplus"]) < 1e-5, (eplus, e["uplus"])
+
+
 @pytest.mark.parametrize("precision,cplx", [(0, 1), (PREC_MIXED, 0)])
 def test_graph_replay_matches_eager(precision, cplx, gpu_available, monkeypatch):
     """Every replay of the captured predictor-corrector step, with eager energy

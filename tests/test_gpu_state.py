@@ -196,31 +196,35 @@ def test_state_dynamics_matches_oracle(gpu_available):
             assert abs(es[s][k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (s, k, es[s][k], e[k])
 
 
-def test_moving_state_matches_oracle(gpu_available):
-    """A D-state impurity that moves while its invar evolves (real time),
-    against the oracle: r, v, a, invar and the energies after 3 RK + 40 PC."""
+@pytest.mark.parametrize("state", [ep.STATE_D, ep.STATE_P])
+def test_moving_state_matches_oracle(gpu_available, state):
+    """A P- or D-state impurity that moves while its invar evolves (real
+    time), against the oracle: r, v, a, invar and the energies after 3 RK +
+    40 PC, and the error estimates errHe, errimp, errvimp, erriv of the last
+    step (steppc.f90:149-150,179,193,213-218)."""
     from test_state_impurity import _series
-    print("[state] moving D state 32^3", flush=True)
+    print(f"[state] moving state {state} 32^3", flush=True)
     soft = dict(v_pi=_series(-300.0, 3, 2.0), v_sigma=_series(-600.0, 3, 2.0), v_delta=_series(-150.0, 3, 2.0))
     p = make_params(32, xmax=12.0, n4=20.0, core=CORE_OTC, h_core=2.377, psi_complex=1)
     rimp, dt = (0.0, 0.5, 7.0), 5e-5
     with ep.Context(p) as ctx:
         ctx.init_fermi(0.0218, 2.22 * 20 ** (1 / 3), 0.6, 1e-6, 9)
-        st = ctx.set_impurity_state(ep.make_state_imp(ep.STATE_D, rimp=rimp, instate=1, als=3.0, **soft))
+        st = ctx.set_impurity_state(ep.make_state_imp(state, rimp=rimp, instate=1, als=3.0, **soft))
         psi0, iv0 = ctx.psi(), st.invar_c()
         ctx.set_state_dynamics(True)
         ctx.set_real_time(True)
         ctx.set_impurity_motion([0.0, 0.0, -2.0], 40.0)
         for _ in range(3):
             ctx.step_rk(dt)
-        for _ in range(40):
+        for _ in range(39):
             ctx.step_pc(dt)
+        errs_gpu = [x[0] for x in ctx.step_pc(dt, impurity_errors=True)]
         got = ctx.impurity() + (ctx.invar(),)
         ctx.poten()
         e_gpu = ctx.energy()[0]
     o = O.Oracle(make_params(32, xmax=12.0, n4=20.0, core=CORE_OTC, h_core=2.377))
     o.set_psi(psi0)
-    imp = ep.make_state_imp(ep.STATE_D, rimp=rimp, instate=-1, als=3.0, invar=iv0, **soft)
+    imp = ep.make_state_imp(state, rimp=rimp, instate=-1, als=3.0, invar=iv0, **soft)
     o.set_impurity_state(imp)
     o.poten()
     o.set_state_dynamics(imp, ev0=0.0)
@@ -229,7 +233,8 @@ def test_moving_state_matches_oracle(gpu_available):
     for _ in range(3):
         o.step_rk(dt)
     for _ in range(40):
-        o.step_pc(dt)
+        err_he = o.step_pc(dt)
+    errs = (err_he,) + o.step_errors()
     want = o.impurity() + (o.invar(),)
     o.poten()
     e = o.energy()
@@ -240,6 +245,14 @@ def test_moving_state_matches_oracle(gpu_available):
     assert got[0][2] < 7.0
     for k in ("eso", "eHeX", "ekinx", "eimpu", "etot"):
         assert abs(e_gpu[k] - e[k]) <= 1e-9 * max(1.0, abs(e[k])), (k, e_gpu[k], e[k])
+    # the estimates are differences of nearly equal predictor and corrector
+    # values: compared relative to their own size, with an absolute floor of
+    # the rounding of the quantities they are differences of
+    print("errHe, errimp, errvimp, erriv", errs_gpu, errs)
+    assert errs[0] > 0.0 and errs[3] > 0.0
+    for name, x, y, floor in zip(("errHe", "errimp", "errvimp", "erriv"), errs_gpu, errs,
+                                 (1e-14, 1e-15 * 7.0, 1e-15 * 2.0, 1e-15)):
+        assert abs(x - y) <= 1e-6 * y + 64 * floor, (name, x, y)
 
 
 SELECT_CASES = [  # (state, instate, select, rimp)

@@ -47,6 +47,39 @@ def test_select_pot_cutoff_and_null():
     assert e.value.code == 4  # EFFPOT_ERR_POT
 
 
+KOUT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "plus_koutselos.json")))
+IONS = ["Na_plus_Koutselos", "K_plus_Koutselos", "Rb_plus_Koutselos", "Cs_plus_Koutselos"]
+
+
+@pytest.mark.parametrize("name", IONS)
+def test_koutselos_matches_exact_values(name):
+    """The Koutselos ion potentials (V_impur.f90:1704-1883) against their
+    closed form in 50-digit arithmetic (plus_koutselos.json, written by
+    make_koutselos_vectors.py from the reference's parameters): FP64 with the
+    reference's operations, so within a few ulps of the larger of the exchange
+    and dispersion terms (they cancel near the well's zero crossing; the
+    exponentials, with arguments up to ~8, lose up to that many ulps), on both
+    sides of the damping switch at x = 1.28 R_m."""
+    p = ep.make_plus(name, 0.0, 0.0)
+    assert p.kind == ep.POT_KOUTSELOS
+    g = KOUT[name]
+    for r, v, sc in zip(g["radii"], g["V"], g["scale"]):
+        for got in (ep.plus_eval(p, r), O.plus_pot(p, r)):
+            assert abs(got - v) <= 4e-15 * sc, (name, r, got, v, sc)
+
+
+def test_rb_koutselos_as_prep_sh_selects_it():
+    """prep.sh:5-7 runs the snapshot workflow with selec_plus =
+    'Rb_plus_Koutselos', r_cutoff_plus = 2.d0, umax_plus = 6.1105922888E+03:
+    that umax is the potential at the cutoff to its 11 digits, so uplus is
+    continuous there; inside the cutoff Select_Pot returns umax."""
+    p = ep.make_plus("Rb_plus_Koutselos", 2.0, 6.1105922888e3)
+    for r in (0.0, 1.0, 1.9999999999999998):
+        assert ep.plus_eval(p, r) == 6.1105922888e3 and O.plus_pot(p, r) == 6.1105922888e3
+    assert abs(ep.plus_eval(p, 2.0) - 6.1105922888e3) < 5e-8
+    assert ep.plus_eval(p, 3.0) < -190.0  # the well of He-Rb+ near 3 A (about -200 K)
+
+
 def _exact_v(p, r):
     x = Fraction(r)
     if r < p.r_cutoff:

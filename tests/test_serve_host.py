@@ -35,3 +35,27 @@ def test_stream_statistics():
         lim = serve.SPACING * serve.grid_for(r.natoms) / 2 - 8.0
         assert all(abs(c) <= lim + 1e-12 for c in r.pos)
         assert 10 <= r.D <= 60 and 3.5 <= r.re <= 5.0 and 0.8 <= r.a <= 1.4
+
+
+def test_budget_reserves_and_marks_starved_classes():
+    """Admission accounting: a block's footprint is reserved only while the
+    budget holds it; a class refused is marked waiting until it gets budget,
+    which is what makes other classes hand back their idle blocks."""
+    b = serve.Budget(10)
+    assert b.take(6, 64) and b.used == 6
+    assert not b.take(6, 128) and b.used == 6
+    assert b.others_waiting(64) and not b.others_waiting(128)
+    b.give(6)
+    assert b.take(6, 128) and not b.others_waiting(64)
+    assert b.take(4, 96) and b.used == 10 and b.peak == 10
+    b.not_waiting(96)
+    assert not b.take(1, 96) and b.others_waiting(128)
+    b.not_waiting(96)
+    assert not b.others_waiting(128)
+
+
+def test_block_footprint_is_what_a_block_reserves():
+    for n, k in serve.BLOCK_SLOTS.items():
+        one = serve.footprint_bytes(serve.class_params(n, 1))
+        blk = serve.footprint_bytes(serve.class_params(n, k))
+        assert one < blk <= k * one
