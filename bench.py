@@ -554,7 +554,9 @@ def main():
                                f"Morse D=30 re=4 a=1 at z0~0.5R, to |dE/E|<1e-9/step",
                    "mode": a.mode, "batch": B, "dt": dt, "converged": conv, "stencil_tile": tile,
                    "steps_per_solve": steps_per_solve, "steps": [r.steps for r in results], "E_imp_K": eimp,
-                   "E_tot_K": [r.e.etot for r in results]},
+                   "E_tot_K": [r.e.etot for r in results],
+                   # |H psi - mu psi| / (|mu| |psi|) at the stop: how far from an eigenstate
+                   "residual": [r.e.resid for r in results]},
         # every tick of the timed batches is one graph-replayed PC step of all
         # running slots (plus the energy checks every 50 steps)
         "ticks": ticks,
@@ -603,6 +605,7 @@ def main():
             (r64,), _ = solve_batch(cr, pr, R, zs[:1], opts, seed=a.seed)
             tr = time.perf_counter() - tr0
         line["fp64_reference"] = {"solves_per_s": 1.0 / tr, "E_imp_K": r64.e.eimpu, "steps": r64.steps,
+                                  "residual": r64.e.resid,
                                   "dE_imp_K": abs(results[0].e.eimpu - r64.e.eimpu),
                                   "max_dE_imp_K_all_slots": max(abs(r.e.eimpu - r64.e.eimpu) for r in results),
                                   "speedup": line["value"] * tr}

@@ -24,7 +24,7 @@ module effpot_c
 
   type, bind(c) :: effpot_energies
     real(c_double) :: natoms, etot4, ekin4, elj4, ealphas, esolid, ecor4, eimpu
-    real(c_double) :: ekinx, ehex, uplus, eso, etot, mu, xcm, ycm, zcm
+    real(c_double) :: ekinx, ehex, uplus, eso, etot, mu, xcm, ycm, zcm, resid
   end type
 
   type, bind(c) :: effpot_solve_opts

@@ -72,7 +72,7 @@ class StateImp(C.Structure):
 class Energies(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("natoms", "etot4", "ekin4", "elj4", "ealphas", "esolid", "ecor4", "eimpu",
-                 "ekinx", "eHeX", "uplus", "eso", "etot", "mu", "xcm", "ycm", "zcm")]
+                 "ekinx", "eHeX", "uplus", "eso", "etot", "mu", "xcm", "ycm", "zcm", "resid")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

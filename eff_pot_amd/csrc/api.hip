@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -181,7 +182,7 @@ struct effpot_ctx {
   std::vector<double> h_sals;     // [B] its spin-orbit splitting
   int any_dyn = 0;
   double* d_dynpart = nullptr;    // [B][kStateBlocks][kMomD] moment partials
-  int plane = 1;                  // forward x and y FFT passes fused per z plane where the plane fits in LDS
+  int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
   double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
@@ -233,6 +234,21 @@ struct effpot_ctx {
 
 namespace {
 
+// Device allocation and free, context set-up and tear-down, and every graph
+// capture (with its instantiation) hold this lock: HIP treats allocation
+// during another thread's stream capture as unsafe (the capture is
+// invalidated and the call fails), so a server that creates contexts while
+// other contexts capture their step graphs needs them serialised.
+std::recursive_mutex g_api_mu;
+
+// Host <-> device copies ordered on the context's own stream (never the null
+// stream, which would synchronise with, and break, another thread's capture).
+hipError_t copy_sync(effpot_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->st);
+}
+
 int tstgrid(int n) {
   if (n < 16 || n > 256) return 1;
   int m = n;
@@ -256,6 +272,7 @@ void fd_coeffs(int id, double* c) {  // same closed forms as the Bickley table r
 }
 
 template <typename T> int dalloc(effpot_ctx* ctx, T** p, size_t count) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   void* v = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc(&v, count * sizeof(T));
@@ -298,7 +315,7 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
       double ang = -2.0 * M_PI * (double)k / (double)ns[a];
       tw[k] = cx<T>{(T)std::cos(ang), (T)std::sin(ang)};
     }
-    HIPCHK(hipMemcpy(s.tw[a], tw.data(), sizeof(cx<T>) * ns[a], hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(ctx, s.tw[a], tw.data(), sizeof(cx<T>) * ns[a], hipMemcpyHostToDevice));
   }
   // stencil coefficients: c_m / h^id in FP64, then rounded to T
   double c1[13], c2[13];
@@ -317,7 +334,7 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
     int init[8] = {0, 1, 2, 3, 4, 0, 1, 0};
     for (int k = 0; k < 8; k++) ring[b * 8 + k] = init[k];
   }
-  HIPCHK(hipMemcpy(s.ring, ring.data(), sizeof(int) * B * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, s.ring, ring.data(), sizeof(int) * B * 8, hipMemcpyHostToDevice));
   return EFFPOT_OK;
 }
 
@@ -371,20 +388,32 @@ template <typename T> int fft_x_c2r(effpot_ctx* ctx, State<T>& s, cx<T>* const* 
   LAUNCHCHK();
   return EFFPOT_OK;
 }
-// The fused forward x-y plane transform applies to square planes that fit in LDS.
-template <typename T> bool plane_ok(effpot_ctx* ctx) {
-  if (!ctx->plane || ctx->nx != ctx->ny) return false;
-  // FP64 planes above 64^2 leave one wave per SIMD and measured slower than two passes
-  if (sizeof(T) == 8 && ctx->nx > 64) return false;
-  switch (ctx->nx) {
-    case 16: return PlaneFits<T, 16>::value; case 24: return PlaneFits<T, 24>::value;
-    case 32: return PlaneFits<T, 32>::value; case 48: return PlaneFits<T, 48>::value;
-    case 64: return PlaneFits<T, 64>::value; case 96: return PlaneFits<T, 96>::value;
-    case 128: return PlaneFits<T, 128>::value; case 192: return PlaneFits<T, 192>::value;
-    case 256: return PlaneFits<T, 256>::value;
+// The fused forward x-y plane transform of square planes (ctx->plane): 1 the
+// kernel with per-sub-group scratch (FP64 only up to 64^2, where larger
+// planes leave one wave per SIMD), 2 / 3 the in-place kernel with 4 / 2
+// sub-groups where the plane fits twice per CU (else mode 1 where it fits);
+// 0 separate x and y passes.
+template <typename T, int NN> bool plane_fits(int mode) {
+  switch (mode) {
+    case 1: return PlaneFits<T, NN>::value && !(sizeof(T) == 8 && NN > 64);
+    case 2: return PlaneIP<T, NN, 4>::value;
+    case 3: return PlaneIP<T, NN, 2>::value;
+    case 4: return PlaneIP<T, NN, 4>::value;
   }
   return false;
 }
+template <typename T> int plane_mode(effpot_ctx* ctx) {
+  if (!ctx->plane || ctx->nx != ctx->ny) return 0;
+  const int m = ctx->plane;
+  auto pick = [m](bool want, bool one) { return want ? m : (m >= 2 && one ? 1 : 0); };
+  switch (ctx->nx) {
+#define M_(NN) case NN: return pick(plane_fits<T, NN>(m), plane_fits<T, NN>(1));
+    M_(16) M_(24) M_(32) M_(48) M_(64) M_(96) M_(128) M_(192) M_(256)
+#undef M_
+  }
+  return 0;
+}
+template <typename T> bool plane_ok(effpot_ctx* ctx) { return plane_mode<T>(ctx) != 0; }
 template <typename T>
 PlaneArgs<T> plane_args(effpot_ctx* ctx, State<T>& s, T* const* r, cx<T>* const* c, int nf, const int* act) {
   PlaneArgs<T> a{};
@@ -398,9 +427,24 @@ template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* 
                                         const int* act) {
   PlaneArgs<T> a = plane_args(ctx, s, in, out, nf, act);
   dim3 g(ctx->nz, nf, ctx->B);
-#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c<T, NN>), g, dim3(kThreads * PlaneFits<T, NN>::G), 0, ctx->st, a)
-  FFT_SWITCH(ctx->nx, M_)
+  const int mode = plane_mode<T>(ctx);
+  if (mode == 2) {
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c_ip<T, NN, 4>), g, dim3(kThreads * 4), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nx, M_)
 #undef M_
+  } else if (mode == 3) {
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c_ip<T, NN, 2>), g, dim3(kThreads * 2), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  } else if (mode == 4) {  // 4 sub-groups held to 64 VGPRs: two 1024-thread groups per CU
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c_ip<T, NN, 4, 8>), g, dim3(kThreads * 4), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  } else {
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_r2c<T, NN>), g, dim3(kThreads * PlaneFits<T, NN>::G), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  }
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -521,7 +565,7 @@ int push_dt(effpot_ctx* ctx, double dt) {
   }
   if (!change) return EFFPOT_OK;
   HIPCHK(hipStreamSynchronize(ctx->st));  // kernels in flight read d_dt
-  HIPCHK(hipMemcpy(ctx->d_dt, v.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_dt, v.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice));
   ctx->dt_dev = v;
   return EFFPOT_OK;
 }
@@ -683,7 +727,7 @@ int stop_motion(effpot_ctx* ctx, int slot) {
   if (!ctx->h_motion[slot].on) return EFFPOT_OK;
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->h_motion[slot] = Motion{};
-  HIPCHK(hipMemcpy(ctx->d_motion + slot, &ctx->h_motion[slot], sizeof(Motion), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_motion + slot, &ctx->h_motion[slot], sizeof(Motion), hipMemcpyHostToDevice));
   ctx->any_motion = 0;
   for (const Motion& q : ctx->h_motion) ctx->any_motion |= q.on;
   return EFFPOT_OK;
@@ -695,7 +739,7 @@ int stop_dyn(effpot_ctx* ctx, int slot) {
   if (ctx->h_is_state[slot]) RC(stop_motion(ctx, slot));  // a P/D impurity moves only with its invar
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->h_dyn[slot] = InvarDyn{};
-  HIPCHK(hipMemcpy(ctx->d_dyn + slot, &ctx->h_dyn[slot], sizeof(InvarDyn), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_dyn + slot, &ctx->h_dyn[slot], sizeof(InvarDyn), hipMemcpyHostToDevice));
   ctx->any_dyn = 0;
   for (const InvarDyn& q : ctx->h_dyn) ctx->any_dyn |= q.on;
   return EFFPOT_OK;
@@ -823,11 +867,11 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
   a.psi_all = s.psi; a.ring = s.ring; a.plus = ctx->d_plus;
   if (ctx->any_dyn) {  // current invar for eso
     HIPCHK(hipStreamSynchronize(ctx->st));
-    HIPCHK(hipMemcpy(ctx->h_dyn.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, ctx->h_dyn.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
   }
   if (ctx->any_motion) {  // current positions for uplus, velocities for ekinx
     HIPCHK(hipStreamSynchronize(ctx->st));
-    HIPCHK(hipMemcpy(ctx->h_motion.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, ctx->h_motion.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
     for (int b = 0; b < ctx->B; b++)
       if (ctx->h_motion[b].on)
         for (int c = 0; c < 3; c++) ctx->h_plus[b].rimp[c] = ctx->h_motion[b].r[c];
@@ -869,6 +913,9 @@ template <typename T> int energy_t(effpot_ctx* ctx, State<T>& s, const int* act,
     e.eimpu = e.ekinx + e.eHeX + e.eso;
     e.etot = e.etot4 + e.eimpu;
     e.mu = e.natoms > 0 ? S[10] * dv / e.natoms : 0.0;
+    // |H psi - mu psi|^2 = |H psi|^2 - mu^2 N (both weighted by dV)
+    const double r2 = S[12] * dv - e.mu * e.mu * e.natoms;
+    e.resid = (e.natoms > 0 && e.mu != 0.0) ? std::sqrt(std::max(r2, 0.0)) / (std::fabs(e.mu) * std::sqrt(e.natoms)) : 0.0;
     const double n4 = ctx->h_n4[b];
     e.xcm = S[7] * dv / n4; e.ycm = S[8] * dv / n4; e.zcm = S[9] * dv / n4;
     out[b] = e;
@@ -899,6 +946,7 @@ int64_t effpot_footprint_bytes(const effpot_params* p) {
 
 int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (!p || !out) return EFFPOT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   *out = nullptr;
   if (tstgrid(p->nx) || tstgrid(p->ny) || tstgrid(p->nz)) return EFFPOT_ERR_GRID;
   if (!fft_size_supported(p->nx) || !fft_size_supported(p->ny) || !fft_size_supported(p->nz)) return EFFPOT_ERR_GRID;
@@ -945,7 +993,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_plus.assign(ctx->B, PlusSlot{});
   ctx->h_eso.assign(ctx->B, 0.0);
-  if (const char* e = std::getenv("EFFPOT_PLANE")) ctx->plane = std::atoi(e) != 0;
+  if (const char* e = std::getenv("EFFPOT_PLANE")) ctx->plane = std::atoi(e);
   ctx->h_imp.assign(ctx->B, effpot_impurity{});
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -954,11 +1002,11 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_dyn.assign(ctx->B, InvarDyn{});
   ctx->h_sargs.assign(ctx->B, StateArgs{});
   ctx->h_sals.assign(ctx->B, 0.0);
-  if (hipMemcpy(ctx->d_dyn, ctx->h_dyn.data(), sizeof(InvarDyn) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(ctx, ctx->d_dyn, ctx->h_dyn.data(), sizeof(InvarDyn) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
-  if (hipMemcpy(ctx->d_motion, ctx->h_motion.data(), sizeof(Motion) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(ctx, ctx->d_motion, ctx->h_motion.data(), sizeof(Motion) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
-  if (hipMemcpy(ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(ctx, ctx->d_plus, ctx->h_plus.data(), sizeof(PlusSlot) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_ones, ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_mask_tmp, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -984,7 +1032,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_reltrend.assign(ctx->B, 0.0);
   ctx->dt_dev.assign(ctx->B, -1.0);
   ctx->h_n4.assign(ctx->B, p->n4);
-  if (hipMemcpy(ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(ctx, ctx->d_n4, ctx->h_n4.data(), sizeof(double) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   if ((rc = dalloc(ctx, &ctx->d_bad, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_mask, (size_t)ctx->B * 7)) != EFFPOT_OK) return bail(rc);
@@ -993,8 +1041,8 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * ctx->B * 16 + 64, hipHostMallocDefault) != hipSuccess)
     return bail(EFFPOT_ERR_NOMEM);
   std::vector<int> ones(ctx->B, 1);
-  if (hipMemcpy(ctx->d_active, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(ctx->d_ones, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(ctx, ctx->d_active, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess ||
+      copy_sync(ctx, ctx->d_ones, ones.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice) != hipSuccess)
     return bail(EFFPOT_ERR_DEVICE);
   // one-time k-space kernels in FP64 (fforma / initcg / kalfs)
   KernParams kp{p->nx, p->ny, p->nz, ctx->nxh, ctx->hx, ctx->hy, ctx->hz, p->eps4, p->sigma4, p->h_core, p->l,
@@ -1014,6 +1062,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
 
 int effpot_ctx_destroy(effpot_ctx* ctx) {
   if (!ctx) return EFFPOT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   hipSetDevice(ctx->dev);
   if (ctx->st) hipStreamSynchronize(ctx->st);
   for (int k = 0; k < 2; k++) {
@@ -1086,7 +1135,7 @@ int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi) {
   cx<double>* dst = ctx->s64.psi + (long)slot * 5 * ctx->N;  // buffer 0 = CUR after reset
   const long nval = ctx->cplx ? 2 * ctx->N : ctx->N;        // doubles per state
   if (ctx->cplx) {
-    HIPCHK(hipMemcpy(dst, psi, sizeof(double) * 2 * ctx->N, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(ctx, dst, psi, sizeof(double) * 2 * ctx->N, hipMemcpyHostToDevice));
   } else {
     std::vector<double> re(ctx->N);
     for (long i = 0; i < ctx->N; i++) {
@@ -1094,7 +1143,7 @@ int effpot_upload_psi(effpot_ctx* ctx, int slot, const double* psi) {
         return ctx->fail(EFFPOT_ERR_ARG, "complex wavefunction uploaded to a context with psi_complex = 0");
       re[i] = psi[2 * i];
     }
-    HIPCHK(hipMemcpy(dst, re.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(ctx, dst, re.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice));
   }
   (void)nval;
   return den_of_slot(ctx, slot);
@@ -1119,10 +1168,10 @@ int effpot_download_psi(effpot_ctx* ctx, int slot, double* psi) {
   RC(cur_index(ctx, ctx->s64, slot, &idx));
   const cx<double>* src = ctx->s64.psi + ((long)slot * 5 + idx) * ctx->N;
   if (ctx->cplx) {
-    HIPCHK(hipMemcpy(psi, src, sizeof(double) * 2 * ctx->N, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, psi, src, sizeof(double) * 2 * ctx->N, hipMemcpyDeviceToHost));
   } else {
     std::vector<double> re(ctx->N);
-    HIPCHK(hipMemcpy(re.data(), src, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, re.data(), src, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
     for (long i = 0; i < ctx->N; i++) { psi[2 * i] = re[i]; psi[2 * i + 1] = 0.0; }
   }
   return EFFPOT_OK;
@@ -1219,7 +1268,7 @@ int effpot_set_natoms(effpot_ctx* ctx, int slot, double n4) {
   if (slot < 0 || slot >= ctx->B || !(n4 > 0)) return ctx->fail(EFFPOT_ERR_ARG, "bad slot or atom number");
   ctx->h_n4[slot] = n4;
   HIPCHK(hipStreamSynchronize(ctx->st));  // kernels in flight read d_n4
-  HIPCHK(hipMemcpy(ctx->d_n4 + slot, &n4, sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_n4 + slot, &n4, sizeof(double), hipMemcpyHostToDevice));
   return EFFPOT_OK;
 }
 
@@ -1366,7 +1415,7 @@ int effpot_set_impurity_state(effpot_ctx* ctx, int slot, effpot_state_imp* imp) 
     // the slot's current psi buffer (ring roles live on the device)
     std::vector<int> ring(8);
     HIPCHK(hipStreamSynchronize(ctx->st));
-    HIPCHK(hipMemcpy(ring.data(), ctx->s64.ring + 8 * slot, sizeof(int) * 8, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, ring.data(), ctx->s64.ring + 8 * slot, sizeof(int) * 8, hipMemcpyDeviceToHost));
     const long off = ((long)slot * 5 + ring[R_CUR]) * ctx->N;  // in complex elements, as Ring::p
 #define M_(KK)                                                                                                 \
   if (ctx->cplx)                                                                                             \
@@ -1424,7 +1473,7 @@ int effpot_set_state_dynamics(effpot_ctx* ctx, int slot, int on, double ev0) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   std::vector<int> one(ctx->B, 0);
   one[slot] = 1;
-  HIPCHK(hipMemcpy(ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
   RC(den_from_cur(ctx, ctx->d_mask_tmp, false));
   RC(poten_t(ctx, ctx->s64, ctx->d_mask_tmp));
   if (ev0 == 0.0) {  // Ev0 = eHeX + eso of the starting state (DFT4He3d.f90:444)
@@ -1444,7 +1493,7 @@ int effpot_set_state_dynamics(effpot_ctx* ctx, int slot, int on, double ev0) {
   q.ihiv[0] = 0; q.ihiv[1] = 1;
   q.sa = a;
   ctx->h_dyn[slot] = q;
-  HIPCHK(hipMemcpy(ctx->d_dyn + slot, &q, sizeof(InvarDyn), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_dyn + slot, &q, sizeof(InvarDyn), hipMemcpyHostToDevice));
   ctx->any_dyn = 1;
   RC(dyn_hiv(ctx, ctx->d_mask_tmp));  // the initial Hinvar (DFT4He3d.f90:454)
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1457,7 +1506,7 @@ int effpot_get_invar(effpot_ctx* ctx, int slot, double* invar) {
   hipSetDevice(ctx->dev);
   HIPCHK(hipStreamSynchronize(ctx->st));
   InvarDyn q;
-  HIPCHK(hipMemcpy(&q, ctx->d_dyn + slot, sizeof(InvarDyn), hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(ctx, &q, ctx->d_dyn + slot, sizeof(InvarDyn), hipMemcpyDeviceToHost));
   const double* src = q.on ? q.iv : ctx->h_sargs[slot].inv;
   for (int k = 0; k < 20; k++) invar[k] = src[k];
   return EFFPOT_OK;
@@ -1496,12 +1545,12 @@ int effpot_set_impurity_motion(effpot_ctx* ctx, int slot, const double* vimp, do
   q.ia[0] = 0; q.ia[1] = 1;
   q.ip = ImpParams{im.kind, im.D, im.re, im.a, im.r_cutoff, im.umax, im.cap, im.rimp[0], im.rimp[1], im.rimp[2]};
   ctx->h_motion[slot] = q;
-  HIPCHK(hipMemcpy(ctx->d_motion + slot, &q, sizeof(Motion), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_motion + slot, &q, sizeof(Motion), hipMemcpyHostToDevice));
   ctx->any_motion = 1;
   // the initial acceleration: poten and forceimp (DFT4He3d.f90:437-453)
   std::vector<int> one(ctx->B, 0);
   one[slot] = 1;
-  HIPCHK(hipMemcpy(ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(ctx, ctx->d_mask_tmp, one.data(), sizeof(int) * ctx->B, hipMemcpyHostToDevice));
   RC(den_from_cur(ctx, ctx->d_mask_tmp, false));
   RC(poten_t(ctx, ctx->s64, ctx->d_mask_tmp));
   RC(force_t(ctx, ctx->d_mask_tmp));
@@ -1515,7 +1564,7 @@ int effpot_get_impurity(effpot_ctx* ctx, int slot, double* r, double* v, double*
   hipSetDevice(ctx->dev);
   HIPCHK(hipStreamSynchronize(ctx->st));
   Motion q;
-  HIPCHK(hipMemcpy(&q, ctx->d_motion + slot, sizeof(Motion), hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(ctx, &q, ctx->d_motion + slot, sizeof(Motion), hipMemcpyDeviceToHost));
   for (int c = 0; c < 3; c++) {
     if (r) r[c] = q.on ? q.r[c] : ctx->h_plus[slot].rimp[c];
     if (v) v[c] = q.on ? q.v[c] : 0.0;
@@ -1618,8 +1667,8 @@ int effpot_step_pc(effpot_ctx* ctx, double dt, double* err_he, double* err_imp, 
     HIPCHK(hipStreamSynchronize(ctx->st));
     std::vector<Motion> mo(ctx->B);
     std::vector<InvarDyn> dy(ctx->B);
-    HIPCHK(hipMemcpy(mo.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(dy.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, mo.data(), ctx->d_motion, sizeof(Motion) * ctx->B, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(ctx, dy.data(), ctx->d_dyn, sizeof(InvarDyn) * ctx->B, hipMemcpyDeviceToHost));
     const double c9 = 9.0 / 121.0;
     for (int b = 0; b < ctx->B; b++) {
       double er = 0.0, ev = 0.0, ei = 0.0;
@@ -1760,6 +1809,7 @@ static int push_mask(effpot_ctx* ctx, int k, const std::vector<int>& m) {
 }
 
 static void drop_graphs(effpot_ctx* ctx) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   for (int k = 0; k < 2; k++) {
     if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
     if (ctx->pc_exec2[k]) { hipGraphExecDestroy(ctx->pc_exec2[k]); ctx->pc_exec2[k] = nullptr; }
@@ -1773,6 +1823,7 @@ template <typename T>
 static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA, const int* dB) {
   const int which = f32 ? 1 : 0;
   if (!ctx->pc_exec2[which]) {
+    std::lock_guard<std::recursive_mutex> lk(g_api_mu);
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = hipEventRecord(ctx->ev_fork, ctx->st);
@@ -1816,6 +1867,7 @@ static int pc_step_graph(effpot_ctx* ctx, State<T>& s, bool f32, const int* dP) 
     return renorm_t(ctx, s, dP, f32);
   }
   if (!ctx->pc_exec[which]) {
+    std::lock_guard<std::recursive_mutex> lk(g_api_mu);
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     ctx->corr_den = false;

@@ -634,4 +634,175 @@ __global__ __launch_bounds__((kThreads * PlaneFits<T, N>::G)) void k_fft_plane_r
   }
 }
 
+// ---- the same forward x-y plane transform in place in one plane buffer -----
+// LDS holds only the plane, as N rows of Q = N/2 + 1 complex values, so two
+// work-groups fit per CU (FP32 at N = 128: 66.5 KB) and one group's HBM loads
+// overlap the other's transforms:
+//  1. row pairs packed as z_p = row(2p) + i row(2p+1) into plane[i Q + p]
+//     (pair-major: consecutive pairs on consecutive banks);
+//  2. x FFT of every pair in place: step 1 writes back exactly the positions
+//     the thread read (elements = n2 mod N2), step 3 after a barrier;
+//  3. y FFT of N/2 columns: column k (1 <= k < N/2) of the half spectrum is
+//     formed on load from Z_p[k] and Z_p[N-k] (X[2p] = (Z_p[k] + conj Z_p[N-k])/2,
+//     X[2p+1] = (Z_p[k] - conj Z_p[N-k])/(2i)); the real columns k = 0 and N/2
+//     travel packed as one complex column W = X[.][0] + i X[.][N/2] and are
+//     separated after the transform (Y0 = (W + conj W_-)/2, Y_N/2 =
+//     (W - conj W_-)/(2i)).  Each column works in place in the 2 (N/2) slots
+//     of the two plane rows it reads.
+// The work-group (G x kThreads threads) transforms kThreads G / N1 lines at a time.
+template <typename T, int N, int G> struct PlaneIP {
+  static constexpr int H = N / 2, Q = H + 1;
+  static constexpr long bytes = (long)N * Q * (long)sizeof(cx<T>);
+  static constexpr bool value = bytes <= 80 * 1024;  // two work-groups per CU
+};
+template <typename T, int N, int G, int W = 1>  // W: waves per SIMD the registers must allow
+__global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<T> a) {
+  constexpr bool fits = PlaneIP<T, N, G>::value;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, H = N / 2, Q = H + 1;
+  constexpr int CW = kThreads * G / N1;  // lines (pairs or columns) per round
+  constexpr int R = (H + CW - 1) / CW;   // rounds
+  if (!a.active[blockIdx.z]) return;
+  __shared__ cx<T> plane[fits ? N * Q : 1];
+  if constexpr (fits) {
+    const int f = blockIdx.y, iz = blockIdx.x;
+    const T* src = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
+    cx<T>* dst = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    // 1. pairs of real rows, packed: a lane takes element i of a row pair and
+    //    loads it from both rows (each load coalesced along the row), every
+    //    load of the thread issued before its first LDS store; consecutive
+    //    lanes store at the pitch Q (2Q = 2 mod 64 words: no bank conflicts)
+    {
+      constexpr int NT = kThreads * G, NE = N * H, NL = (NE + NT - 1) / NT;
+      T ra[NL], rb[NL];
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (NE % NT == 0 || e < NE) {
+          const int p = e / N, i = e - p * N;
+          ra[k] = src[(long)(2 * p) * N + i];
+          rb[k] = src[(long)(2 * p + 1) * N + i];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (NE % NT == 0 || e < NE) {
+          const int p = e / N, i = e - p * N;
+          plane[i * Q + p] = cx<T>{ra[k], rb[k]};
+        }
+      }
+    }
+    __syncthreads();
+    // the whole work-group is one group of CW lines (pairs or columns): the
+    // 32 lanes of a wave half then touch 32 consecutive lines with the same
+    // element index, at the pitch Q (2Q = 2 mod 64 words), so no LDS access
+    // of the transforms has a bank conflict
+    const int tt = threadIdx.x, c = tt % CW;
+    // 2. x transforms, in place in pair column p
+    for (int j = 0; j < R; j++) {
+      const int p = j * CW + c;
+      const bool on = p < H;
+      const int n2 = tt / CW;
+      if (on && tt < CW * N2) {
+        cx<T> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) v[n1] = plane[(N2 * n1 + n2) * Q + p];
+        RegFFT<T, N1>::run(v, -1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+          cx<T> w = v[k1];
+          if (k1 > 0 && n2 > 0) w = cmul(w, a.twx[n2 * k1]);
+          plane[(k1 * N2 + n2) * Q + p] = w;  // the positions this thread read
+        }
+      }
+      __syncthreads();
+      const int k1 = tt / CW;
+      cx<T> b[N2];
+      if (on) {
+#pragma unroll
+        for (int m = 0; m < N2; m++) b[m] = plane[(k1 * N2 + m) * Q + p];
+        RegFFT<T, N2>::run(b, -1);
+      }
+      __syncthreads();
+      if (on) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) plane[(k1 + N1 * k2) * Q + p] = b[k2];
+      }
+      __syncthreads();
+    }
+    // 3. y transforms of the half-spectrum columns; column kk owns plane rows
+    //    kk and N - kk (kk = 0: rows 0 and H): element m < H at row ra, else rb
+    for (int j = 0; j < R; j++) {
+      const int kk = j * CW + c;
+      const bool on = kk < H;
+      const int ra = kk, rb = kk == 0 ? H : N - kk;
+      const int n2 = tt / CW;
+      const bool s1 = on && tt < CW * N2;
+      cx<T> v[N1];
+      if (s1) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+          const int y = N2 * n1 + n2, pp = y >> 1;
+          const cx<T> za = plane[ra * Q + pp], zb = plane[rb * Q + pp];
+          cx<T> x;
+          if (kk == 0) {  // W[y] = X[y][0] + i X[y][H]
+            x = (y & 1) ? cx<T>{za.y, zb.y} : cx<T>{za.x, zb.x};
+          } else if (y & 1) {  // (Z[k] - conj Z[N-k]) / (2i)
+            x = cx<T>{T(0.5) * (za.y + zb.y), T(-0.5) * (za.x - zb.x)};
+          } else {  // (Z[k] + conj Z[N-k]) / 2
+            x = cx<T>{T(0.5) * (za.x + zb.x), T(0.5) * (za.y - zb.y)};
+          }
+          v[n1] = x;
+        }
+        RegFFT<T, N1>::run(v, -1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++)
+          if (k1 > 0 && n2 > 0) v[k1] = cmul(v[k1], a.twy[n2 * k1]);
+      }
+      __syncthreads();  // every column's Z values are read before any is overwritten
+      if (s1) {
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+          const int m = k1 * N2 + n2;
+          plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)] = v[k1];
+        }
+      }
+      __syncthreads();
+      const int k1 = tt / CW;
+      cx<T> b[N2];
+      if (on) {
+#pragma unroll
+        for (int m2 = 0; m2 < N2; m2++) {
+          const int m = k1 * N2 + m2;
+          b[m2] = plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)];
+        }
+        RegFFT<T, N2>::run(b, -1);
+        if (kk > 0) {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) dst[kk + (long)(k1 + N1 * k2) * a.pitch] = b[k2];
+        }
+      }
+      __syncthreads();
+      if (on && kk == 0) {  // W[ky] into its own slots
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const int m = k1 + N1 * k2;
+          plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)] = b[k2];
+        }
+      }
+      if (j == 0) {
+        __syncthreads();
+        for (int ky = tt; ky < N; ky += kThreads * G) {  // separate the packed column: Y0 and Y_H
+          const int km = (N - ky) % N;
+          const cx<T> w = plane[(ky < H ? 0 : H) * Q + (ky < H ? ky : ky - H)];
+          const cx<T> wm = plane[(km < H ? 0 : H) * Q + (km < H ? km : km - H)];
+          dst[(long)ky * a.pitch] = cx<T>{T(0.5) * (w.x + wm.x), T(0.5) * (w.y - wm.y)};
+          dst[H + (long)ky * a.pitch] = cx<T>{T(0.5) * (w.y + wm.y), T(-0.5) * (w.x - wm.x)};
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace ep

@@ -508,9 +508,9 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // energy() densities (energy.f90:47-165), the chemical potential, the
-// helium centre of mass (r_cm.f90:38-54) and the X+-He term: 12 block
-// partial sums.
-constexpr int kEnergySums = 12;
+// helium centre of mass (r_cm.f90:38-54), the X+-He term and |H psi|^2 (the
+// residual of the eigenvalue equation): 13 block partial sums.
+constexpr int kEnergySums = 13;
 template <typename T> struct EnergyArgs {
   Grid g;
   StenCoef<T> sc;
@@ -569,6 +569,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
       v[9] += (double)d * (a.z0 + a.hz * z);
       V hp = hpsi_of(p, d2, a.h2o2m4, a.pot4[i], a.uimp[i]);
       v[10] -= (double)dotre(p, hp);
+      v[12] += (double)norm2(hp);
       if (plus_on && keep) v[11] += (double)d * plus_pot(pl.p, r);  // uplus: Select_Pot(selec_plus, r)
     });
   block_sum<kEnergySums>(v, red);

@@ -214,6 +214,10 @@ typedef struct effpot_energies {
   double etot;     /* etot4 + eimpu (:197)                                */
   double mu;       /* chemical potential <psi|H|psi>/N                    */
   double xcm, ycm, zcm; /* helium centre of mass, normalised by n4          */
+  double resid;    /* |H psi - mu psi| / (|mu| |psi|): how far psi is from an
+                      eigenstate of its own mean field (the residual norm
+                      varmu.f90:11-29 keeps); 0 at convergence, independent
+                      of the time step and of the trajectory               */
 } effpot_energies;
 
 /* Options of the relaxation loop (effpot_solve). */

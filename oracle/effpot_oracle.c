@@ -1004,8 +1004,21 @@ void oracle_energy(oracle_t* o, effpot_energies* e) {
       mu -= o->psi[2 * i + c] * h;
     }
   }
-  free(l3);
   e->mu = mu * o->dxyz / e->natoms;
+  /* residual |(-h2o2m4 lap + pot4 + uimp) psi - mu psi| / (|mu| |psi|), summed
+   * directly from the residual at every point (the device forms it from
+   * |H psi|^2 - mu^2 N) */
+  double r2 = 0.0;
+  for (size_t i = 0; i < N; i++) {
+    for (int c = 0; c < 2; c++) {
+      double lp = (o->lap[2 * i + c] + o->hpsi[2 * i + c]) + l3[2 * i + c];
+      double h = lp * p->h2o2m4 - o->pot4[i] * o->psi[2 * i + c] - o->uimp[i] * o->psi[2 * i + c];
+      double r = -h - e->mu * o->psi[2 * i + c];
+      r2 += r * r;
+    }
+  }
+  free(l3);
+  e->resid = e->mu != 0.0 ? sqrt(r2 * o->dxyz) / (fabs(e->mu) * sqrt(e->natoms)) : 0.0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -29,7 +29,7 @@ def test_struct_sizes_match_header():
     # 10 int32 + 16 doubles
     assert ctypes.sizeof(ep.Params) == 10 * 4 + 16 * 8
     assert ctypes.sizeof(ep.Impurity) == 8 + 9 * 8
-    assert ctypes.sizeof(ep.Energies) == 17 * 8
+    assert ctypes.sizeof(ep.Energies) == 18 * 8
     assert ctypes.sizeof(ep.SolveOpts) == 2 * 8 + 4 * 4 + 8 + 8
 
 

@@ -214,3 +214,51 @@ def test_rk_gill_single_step_amplification():
     z = lam * dt
     g = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
     assert np.allclose(o.psi(), g * psi0, rtol=1e-12, atol=1e-14)
+
+
+def test_residual_vanishes_for_an_eigenstate():
+    """The residual |H psi - mu psi| / (|mu| |psi|) of energy() is 0 for an
+    exact eigenstate: a uniform bulk density on a periodic grid (OT, so every
+    gradient term vanishes) with a constant impurity field (a Morse well
+    capped below its own minimum: uimp = cap everywhere), mu = pot4 + cap."""
+    rho = 0.021836
+    p = make_params(16, xmax=6.0, core=CORE_OT, h_core=H4OT, icon=ICON_PERIODIC, n4=1.0)
+    o = Oracle(p)
+    o.set_psi(np.full(o.shape, np.sqrt(rho), dtype=complex))
+    o.set_impurity(make_morse(D=30.0, cap=-40.0))  # V >= -30 K everywhere: uimp = -40 K
+    assert np.all(o.field("uimp") == -40.0)
+    o.poten()
+    e = o.energy()
+    assert abs(e["mu"] - (o.field("pot4")[0, 0, 0] - 40.0)) < 1e-9
+    assert e["resid"] < 1e-12
+    o.close()
+
+
+def test_residual_of_two_plane_waves():
+    """In the linear setting (pot4 = 0, uimp = V) plane waves are exact
+    eigenmodes of the 13-point Laplacian with E_j = -h2o2m4 sigma(k_j) + V,
+    sigma the stencil's symbol; for psi = a cos(k1 x) + b cos(k2 x):
+    mu = (a^2 E1 + b^2 E2) / (a^2 + b^2) and
+    resid = sqrt((a^2 (E1 - mu)^2 + b^2 (E2 - mu)^2) / (a^2 + b^2)) / |mu|.
+    One mode alone gives 0.  A kinetic sign slip, a lost V or a missing dV
+    weight moves it."""
+    o, psi1, lam1 = _linear_setup(V=3.0)
+    n, h = 16, 0.5
+    x = -4.0 + h * np.arange(n)
+    c2 = fd_coeffs(2)
+    k2 = 2 * np.pi * 5 / (n * h)
+    sym2 = (c2 * np.cos(k2 * h * np.arange(-6, 7))).sum() / h ** 2
+    e1, e2 = -lam1, -(6.05969638298 * sym2 - 3.0)  # lam = h2o2m4 sigma - V; E = -lam
+    o.set_psi(psi1)
+    o.poten()
+    assert o.energy()["resid"] < 1e-13
+    a, b = 0.8, 0.35
+    psi = (a * psi1.real + b * np.cos(k2 * x)[None, None, :] * np.ones((n, n, 1))).astype(complex)
+    o.set_psi(psi)
+    o.poten()
+    e = o.energy()
+    mu = (a * a * e1 + b * b * e2) / (a * a + b * b)
+    want = np.sqrt((a * a * (e1 - mu) ** 2 + b * b * (e2 - mu) ** 2) / (a * a + b * b)) / abs(mu)
+    assert abs(e["mu"] - mu) < 1e-11 * abs(mu)
+    assert abs(e["resid"] - want) < 1e-10 * want, (e["resid"], want)
+    o.close()
