@@ -443,8 +443,9 @@ class Context:
         return ms.value
 
     def plane_fused(self, fp32_fields):
-        """True when poten() runs the forward x and y transforms as one launch."""
-        return bool(lib().effpot_plane_fused(self.h, 1 if fp32_fields else 0))
+        """Schedule of poten()'s forward x and y transforms: 0 separate
+        passes, 1 one plane launch with scratch, 3 one plane launch in place."""
+        return int(lib().effpot_plane_fused(self.h, 1 if fp32_fields else 0))
 
     def sync(self):
         self._chk(lib().effpot_sync(self.h))

@@ -183,6 +183,7 @@ struct effpot_ctx {
   int any_dyn = 0;
   double* d_dynpart = nullptr;    // [B][kStateBlocks][kMomD] moment partials
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
+  bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
   double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
@@ -350,7 +351,7 @@ dim3 pw_grid(long n, int nslots) {
     case 16: M_(16); break; case 24: M_(24); break; case 32: M_(32); break; \
     case 48: M_(48); break; case 64: M_(64); break; case 96: M_(96); break; \
     case 128: M_(128); break; case 192: M_(192); break; case 256: M_(256); break; \
-    default: return ctx->fail(EFFPOT_ERR_GRID, "unsupported FFT length"); \
+    default: return ctx->fail(EFFPOT_ERR_GRID, "unsupported FFT length (16 24 32 48 64 96 128 192 256)"); \
   }
 
 int n1_of(int n) {
@@ -948,8 +949,21 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if (!p || !out) return EFFPOT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(g_api_mu);
   *out = nullptr;
-  if (tstgrid(p->nx) || tstgrid(p->ny) || tstgrid(p->nz)) return EFFPOT_ERR_GRID;
-  if (!fft_size_supported(p->nx) || !fft_size_supported(p->ny) || !fft_size_supported(p->nz)) return EFFPOT_ERR_GRID;
+  {
+    const int n3[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < 3; a++) {
+      if (tstgrid(n3[a])) {
+        snap::g_err = "grid: n" + std::string(1, "xyz"[a]) + " = " + std::to_string(n3[a]) +
+                      " is not 2^a 3^b 5^c in 16..256 (tstgrid)";
+        return EFFPOT_ERR_GRID;
+      }
+      if (!fft_size_supported(n3[a])) {
+        snap::g_err = "grid: n" + std::string(1, "xyz"[a]) + " = " + std::to_string(n3[a]) +
+                      " has no FFT here; supported sizes: 16 24 32 48 64 96 128 192 256";
+        return EFFPOT_ERR_GRID;
+      }
+    }
+  }
   if (p->core != EFFPOT_CORE_OT && p->core != EFFPOT_CORE_OTC) return EFFPOT_ERR_CORE;
   if (p->npd != 13 || (p->icon != 13 && p->icon != 8) || p->batch < 1 || p->batch > 4096) return EFFPOT_ERR_ARG;
   if (p->precision != EFFPOT_PREC_FP64 && p->precision != EFFPOT_PREC_MIXED) return EFFPOT_ERR_ARG;
@@ -993,7 +1007,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_plus, ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_plus.assign(ctx->B, PlusSlot{});
   ctx->h_eso.assign(ctx->B, 0.0);
-  if (const char* e = std::getenv("EFFPOT_PLANE")) ctx->plane = std::atoi(e);
+  if (const char* e = std::getenv("EFFPOT_PLANE")) { ctx->plane = std::atoi(e); ctx->plane_fixed = true; }
   ctx->h_imp.assign(ctx->B, effpot_impurity{});
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -2151,6 +2165,24 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     }
   }
   ctx->tx = best_tx; ctx->zc = best_zc;
+  // the forward x-y transform schedule: separate passes (0), the plane kernel
+  // with sub-group scratch (1) or in place (3), timed on whole poten launches
+  // of the live precision (EFFPOT_PLANE set in the environment fixes it)
+  if (!ctx->plane_fixed) {
+    const bool f32 = ctx->mixed;
+    int best_plane = ctx->plane;
+    double best_p = 1e30;
+    for (int mode : {0, 1, 3}) {
+      ctx->plane = mode;
+      const int eff = f32 ? plane_mode<float>(ctx) : plane_mode<double>(ctx);
+      if (eff != mode) continue;  // not applicable at this size and precision
+      double t = 0;
+      RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t, f32));
+      if (ctx->tune_log) std::fprintf(stderr, "autotune plane=%d poten=%.4f ms\n", mode, t);
+      if (t < best_p) { best_p = t; best_plane = mode; }
+    }
+    ctx->plane = best_plane;
+  }
   drop_graphs(ctx);  // captured graphs hold the old launch shapes
   if (tx_out) *tx_out = best_tx;
   if (zc_out) *zc_out = best_zc;
@@ -2179,7 +2211,7 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
 
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields) {
   if (check_ctx(ctx) != EFFPOT_OK) return 0;
-  return fp32_fields ? (plane_ok<float>(ctx) ? 1 : 0) : (plane_ok<double>(ctx) ? 1 : 0);
+  return fp32_fields ? plane_mode<float>(ctx) : plane_mode<double>(ctx);
 }
 
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {

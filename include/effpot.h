@@ -28,7 +28,7 @@ extern "C" {
 /* ---- status codes ------------------------------------------------------ */
 #define EFFPOT_OK 0
 #define EFFPOT_ERR_ARG 1       /* bad argument / NULL pointer                     */
-#define EFFPOT_ERR_GRID 2      /* n not 2^a 3^b 5^c <= 256 (tstgrid.f90:8-12)      */
+#define EFFPOT_ERR_GRID 2      /* grid size not supported: see effpot_params.nx   */
 #define EFFPOT_ERR_CORE 3      /* unknown core4 (DFT4He3d.f90:355-363)            */
 #define EFFPOT_ERR_POT 4       /* unknown impurity potential (V_impur.f90:928-931)*/
 #define EFFPOT_ERR_DEVICE 5    /* HIP runtime failure                             */
@@ -58,7 +58,14 @@ extern "C" {
  * DFT4He3d.f90:104-134 that the hot path reads, plus h_core (the LJ core
  * radius h4, chosen from core4 in DFT4He3d.f90:346-363, here explicit). */
 typedef struct effpot_params {
-  int32_t nx, ny, nz; /* grid points (tstgrid: 2^a 3^b 5^c, 16..256)       */
+  int32_t nx, ny, nz; /* grid points per axis: 16, 24, 32, 48, 64, 96, 128,
+                         192 or 256 (the sizes of the register/LDS FFTs,
+                         N = N1 N2 with N1 a power of two >= N2).  The
+                         reference's tstgrid (tstgrid.f90:8-12) allows any
+                         2^a 3^b 5^c <= 256; the other sizes (e.g. 40, 80,
+                         100, 120, 144, 160, 180, 240) get EFFPOT_ERR_GRID
+                         from effpot_ctx_create, and effpot_last_error(NULL)
+                         says which axis                                    */
   int32_t core;       /* EFFPOT_CORE_OT | EFFPOT_CORE_OTC                   */
   int32_t npd;        /* stencil points; only 13 is supported (modules.f90:15) */
   int32_t icon;       /* EFFPOT_ICON_ZERO (13, default) | EFFPOT_ICON_PERIODIC */
@@ -494,9 +501,10 @@ int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* o
  * overwrite the integrator state: call it after the solves that matter. */
 int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_launch);
 
-/* Benchmark hook (no reference counterpart): 1 when poten() of FP32
- * (fp32_fields = 1) or FP64 fields runs the forward x and y transforms as
- * one fused plane launch (launches 1, 7 and 13 are then empty), else 0. */
+/* Benchmark hook (no reference counterpart): the schedule of the forward x
+ * and y transforms of poten() with FP32 (fp32_fields = 1) or FP64 fields:
+ * 0 separate passes, 1 one plane launch with per-sub-group scratch, 3 one
+ * plane launch in place (launches 1, 7 and 13 are then empty). */
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields);
 
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
@@ -516,8 +524,11 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
 
 /* Autotuner (no reference counterpart): times the H psi + predictor,
  * corrector and poten kernels for every stencil tile (x width 64/32/16/8
- * dividing nx, z chunk 4/8/16/32) on the live state and keeps the fastest.  Overwrites the
- * integrator state like effpot_kernel_time: call it before uploading states. */
+ * dividing nx, z chunk 4/8/16/32) on the live state and keeps the fastest,
+ * then times poten under each forward-transform schedule that applies
+ * (effpot_plane_fused: 0, 1, 3; the environment variable EFFPOT_PLANE fixes
+ * it instead) and keeps the fastest.  Overwrites the integrator state like
+ * effpot_kernel_time: call it before uploading states. */
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
 /* Sets the stencil tile explicitly (x width, z chunk). */
 int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc);

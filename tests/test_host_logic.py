@@ -13,9 +13,15 @@ def create(p):
 
 
 def test_bad_grid_rejected():
+    def msg():
+        return ep.lib().effpot_last_error(None).decode()
     assert create(abi.make_params(100, xmax=10.0)) == abi.ERR_GRID   # allowed by tstgrid, no FFT schedule here
+    assert "nx = 100" in msg() and "supported sizes" in msg()
     assert create(abi.make_params(14, xmax=10.0)) == abi.ERR_GRID
+    assert "tstgrid" in msg()
     assert create(abi.make_params(7 * 16, xmax=10.0)) == abi.ERR_GRID
+    assert create(abi.make_params(32, xmax=10.0, nz=80)) == abi.ERR_GRID
+    assert "nz = 80" in msg()
 
 
 def test_bad_core_and_stencil_rejected():
