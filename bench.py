@@ -269,14 +269,18 @@ POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s), O
     ("C z conv (1 in, 1 out)", 0, 5), ("C y inv (1)", 0, 4), ("C x c2r (1)", 1, 2), ("k_pot4", 8, 0)]
 
 
-def poten_launch_bytes(k, n, mode, B, plane):
+def poten_launch_bytes(k, n, mode, B, plane, iplane=0):
     """Algorithmic bytes of launch k of poten (OTC) for B slots.  k_pot4 reads
     delj4, dencg, sto1, ua, div, den and writes pot4 and ualphas; with FP32
     fields it also reads the defect correction (4 B).  The forward y passes
-    are absent when fused into the plane launch."""
+    are absent when fused into the plane launch, the inverse y passes when
+    fused into the inverse plane launch (which moves what the x c2r pass
+    alone moves: the spectra read, the real fields written)."""
     s = 4 if mode == "mixed" else 8
     N, M = n ** 3, (n // 2 + 1) * n * n
     if plane and k in (1, 7, 13):
+        return 0
+    if iplane and k in (3, 9, 15):
         return 0
     _, cn, cm = POTEN_LAUNCHES[k]
     b = s * (cn * N + cm * M)
@@ -285,7 +289,7 @@ def poten_launch_bytes(k, n, mode, B, plane):
     return B * b
 
 
-def pc_step_bytes(n, mode, B, plane):
+def pc_step_bytes(n, mode, B, plane, iplane=0):
     """Algorithmic bytes of one predictor-corrector step of B slots (real psi,
     OTC): every array a launch reads or writes counted once.  Per point and
     slot, with s the field size (4 B mixed, 8 B FP64) and psi and the
@@ -299,8 +303,22 @@ def pc_step_bytes(n, mode, B, plane):
     N = n ** 3
     # the renormalisation stores the density only for FP64 fields
     psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + (0 if mode == "mixed" else s))
-    poten = sum(poten_launch_bytes(k, n, mode, 1, plane) for k in range(len(POTEN_LAUNCHES)))
+    poten = sum(poten_launch_bytes(k, n, mode, 1, plane, iplane) for k in range(len(POTEN_LAUNCHES)))
     return B * (psi_side * N + poten)
+
+
+def pc_step_fused_bytes(n, mode, B):
+    """Bytes of one PC step of B slots under the fused model: every logical
+    array touched once per fused kernel, a 3-D FFT one read plus one write of
+    its half spectrum, k-space kernels read once.  poten with the OTC terms
+    (derden + term_alfa) is 37 s N + 34 s M per slot (OT part 11 s N + 13 s M,
+    OTC extra 26 s N + 21 s M); the psi side is that of pc_step_bytes (psi and
+    the integrator FP64 in both modes).  Lower than pc_step_bytes, which counts
+    what each of this implementation's launches moves."""
+    s = 4 if mode == "mixed" else 8
+    N, M = n ** 3, (n // 2 + 1) * n * n
+    psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + (0 if mode == "mixed" else s))
+    return B * (psi_side * N + 37 * s * N + 34 * s * M)
 
 
 def run_poten(a):
@@ -319,16 +337,18 @@ def run_poten(a):
         c.kernel_time(2, reps=2)
         tot = 0.0
         plane = c.plane_fused(a.mode == "mixed")
+        iplane = c.plane_inverse_fused(a.mode == "mixed")
         for k, (name, _, _) in enumerate(POTEN_LAUNCHES):
             ms = c.kernel_time(100 + k, reps=20)
-            byt = poten_launch_bytes(k, a.n, a.mode, B, plane)
+            byt = poten_launch_bytes(k, a.n, a.mode, B, plane, iplane)
             out.append({"launch": k, "name": name, "ms": ms, "GB_s": byt / (ms * 1e-3) / 1e9 if byt else 0.0})
             tot += ms
         whole = c.kernel_time(2, reps=10)
     return common_line(a, "mean-field (poten) launches: ms and algorithmic GB/s", whole, "ms", whole * 1e-3,
                        "fp32 fields" if a.mode == "mixed" else "f64",
                        f"poten of the {a.n}^3 N=1000 workload, {B} slots", mode=a.mode, batch=B,
-                       stencil_tile=list(tile), sum_of_launches_ms=tot, launches=out)
+                       stencil_tile=list(tile), plane=plane, inverse_plane=iplane, sum_of_launches_ms=tot,
+                       launches=out)
 
 
 def run_stress(a):
@@ -591,10 +611,14 @@ def main():
         # the whole PC step (predictor, poten, corrector, norm, renormalisation)
         # against its algorithmic bytes: the fraction of HBM peak a step reaches
         ms_step = ctx.kernel_time(4, reps=10)
-        step_b = pc_step_bytes(a.n, a.mode, B, ctx.plane_fused(a.mode == "mixed"))
+        step_b = pc_step_bytes(a.n, a.mode, B, ctx.plane_fused(a.mode == "mixed"),
+                               ctx.plane_inverse_fused(a.mode == "mixed"))
         line["pc_step"] = {"ms": ms_step, "bytes": step_b, "achieved": step_b / (ms_step * 1e-3) / 1e9,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": step_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "slots": B}
+        fused_b = pc_step_fused_bytes(a.n, a.mode, B)
+        line["pc_step"]["fused_model"] = {"bytes": fused_b, "achieved": fused_b / (ms_step * 1e-3) / 1e9,
+                                          "frac": fused_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
     ctx.close()
     # ---- the plain FP64 form on slot 0's input (untimed reference) -----------
     if a.mode == "mixed" and not a.no_ref:

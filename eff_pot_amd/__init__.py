@@ -73,6 +73,7 @@ def lib():
         L.effpot_sync.argtypes = [vp]
         L.effpot_kernel_time.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.effpot_plane_fused.argtypes = [vp, C.c_int]
+        L.effpot_plane_inverse_fused.argtypes = [vp, C.c_int]
         L.effpot_set_exclusion.argtypes = [vp, C.c_int, C.c_double]
         L.effpot_set_force_filter.argtypes = [vp, C.c_int, C.c_int]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
@@ -446,6 +447,11 @@ class Context:
         """Schedule of poten()'s forward x and y transforms: 0 separate
         passes, 1 one plane launch with scratch, 3 one plane launch in place."""
         return int(lib().effpot_plane_fused(self.h, 1 if fp32_fields else 0))
+
+    def plane_inverse_fused(self, fp32_fields):
+        """1 when poten()'s inverse y and x transforms run as one in-place
+        plane launch, 0 for separate passes."""
+        return int(lib().effpot_plane_inverse_fused(self.h, 1 if fp32_fields else 0))
 
     def sync(self):
         self._chk(lib().effpot_sync(self.h))

@@ -184,6 +184,8 @@ struct effpot_ctx {
   double* d_dynpart = nullptr;    // [B][kStateBlocks][kMomD] moment partials
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
+  int iplane = 1;                 // inverse y and x passes fused per z plane where it fits (EFFPOT_IPLANE)
+  bool iplane_fixed = false;      // EFFPOT_IPLANE given: the autotuner keeps it
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
   double* d_smom = nullptr;       // [kStateBlocks][kMomD] moment partials
@@ -450,6 +452,35 @@ template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* 
   return EFFPOT_OK;
 }
 
+// The fused inverse y + x c2r plane transform (ctx->iplane) where its plane
+// fits twice per CU with 32 lines per round (PlaneInv).
+template <typename T> bool iplane_ok(effpot_ctx* ctx) {
+  if (!ctx->iplane || ctx->nx != ctx->ny) return false;
+  switch (ctx->nx) {
+#define M_(NN) case NN: return PlaneInv<T, NN>::value;
+    M_(16) M_(24) M_(32) M_(48) M_(64) M_(96) M_(128) M_(192) M_(256)
+#undef M_
+  }
+  return false;
+}
+// half spectra (after the z transform) -> real fields, inverse y and x in one
+// launch per plane: out = aff0 + scale*aff1*x
+template <typename T> int fft_plane_inv(effpot_ctx* ctx, State<T>& s, cx<T>* const* in, T* const* out, int nf, T scale,
+                                        const int* act, const T* aff0 = nullptr, const T* aff1 = nullptr) {
+  PlaneArgs<T> a = plane_args(ctx, s, out, in, nf, act);
+  a.scale = scale;
+  for (int f = 0; f < nf; f++) {
+    a.aff0[f] = aff0 ? aff0[f] : T(0);
+    a.aff1[f] = aff1 ? aff1[f] : T(1);
+  }
+  dim3 g(ctx->nz, nf, ctx->B);
+#define M_(NN) hipLaunchKernelGGL((k_fft_plane_c2r_ip<T, NN>), g, dim3(kThreads * PlaneInv<T, NN>::G), 0, ctx->st, a)
+  FFT_SWITCH(ctx->nx, M_)
+#undef M_
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
 // y lines: (kx, z) runs of nxp, element stride nxp; z lines: (kx, ky) runs of
 // nxp, element stride nxp*ny.  Padding lines (kx >= nxh) are skipped.
 LineMap ymap(effpot_ctx* ctx) {
@@ -609,6 +640,7 @@ template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring
 template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1) {
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
   const bool pl = plane_ok<T>(ctx);  // forward x and y passes fused: launches 1, 7, 13 are empty
+  const bool ipl = iplane_ok<T>(ctx);  // inverse y and x passes fused: launches 3, 9, 15 are empty
   auto sel = [only](int k) { return only < 0 || only == k; };
   // stage A: FFT(den) -> dencg, delj4 [, denalf]
   {
@@ -624,13 +656,15 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     ca.out[0][2] = s.spec[3]; ca.kern[0][2] = s.kern[2];
     if (sel(2)) RC(fft_zconv(ctx, s, ca, 1, act));
     cx<T>* o[3] = {s.spec[1], s.spec[2], s.spec[3]};
-    if (sel(3)) RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
+    if (sel(3) && !ipl) RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
     // the alpha_s convolution is stored as falfs = 1 - denalf/den0s (term_alfa.f90:157-159),
     // the only form its consumers use
     T* r[3] = {s.dencg, s.delj4, s.denalf};
     const T a0[3] = {T(0), T(0), T(1)};
     const T a1[3] = {T(1), T(1), T(-1.0 / ctx->p.den0s)};
-    if (sel(4)) RC(fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
+    if (sel(4))
+      RC(ipl ? fft_plane_inv(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1)
+             : fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
   }
   // derden + FFT sources
   if (sel(5)) {
@@ -661,9 +695,9 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     }
     if (sel(8)) RC(fft_zconv(ctx, s, ca, nf, act));
     cx<T>* o[4] = {s.spec[0], s.spec[1], s.spec[2], s.spec[3]};
-    if (sel(9)) RC(fft_strided(ctx, s, 1, o, nf, +1, act));
+    if (sel(9) && !ipl) RC(fft_strided(ctx, s, 1, o, nf, +1, act));
     T* r[4] = {s.sto1, s.intx, s.inty, s.intz};
-    if (sel(10)) RC(fft_x_c2r(ctx, s, o, r, nf, T(1), act));
+    if (sel(10)) RC(ipl ? fft_plane_inv(ctx, s, o, r, nf, T(1), act) : fft_x_c2r(ctx, s, o, r, nf, T(1), act));
   }
   if (otc) {
     DivArgs<T> a{};
@@ -686,9 +720,9 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     ca.in[0] = s.spec[4]; ca.nout[0] = 1; ca.out[0][0] = s.spec[0]; ca.kern[0][0] = s.kern[2];
     if (sel(14)) RC(fft_zconv(ctx, s, ca, 1, act));
     cx<T>* o[1] = {s.spec[0]};
-    if (sel(15)) RC(fft_strided(ctx, s, 1, o, 1, +1, act));
+    if (sel(15) && !ipl) RC(fft_strided(ctx, s, 1, o, 1, +1, act));
     T* r[1] = {s.ua};
-    if (sel(16)) RC(fft_x_c2r(ctx, s, o, r, 1, T(1), act));
+    if (sel(16)) RC(ipl ? fft_plane_inv(ctx, s, o, r, 1, T(1), act) : fft_x_c2r(ctx, s, o, r, 1, T(1), act));
   }
   if (sel(17)) {
     Pot4Args<T> a{};
@@ -1008,6 +1042,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_plus.assign(ctx->B, PlusSlot{});
   ctx->h_eso.assign(ctx->B, 0.0);
   if (const char* e = std::getenv("EFFPOT_PLANE")) { ctx->plane = std::atoi(e); ctx->plane_fixed = true; }
+  if (const char* e = std::getenv("EFFPOT_IPLANE")) { ctx->iplane = std::atoi(e) != 0; ctx->iplane_fixed = true; }
   ctx->h_imp.assign(ctx->B, effpot_impurity{});
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -1769,8 +1804,13 @@ template <typename T> static int fft_c2r_t(effpot_ctx* ctx, State<T>& s, const T
   cx<T>* c[1] = {s.spec[0]};
   T* r[1] = {s.src[0]};
   RC(fft_strided(ctx, s, 2, c, 1, +1, ctx->d_mask_tmp));
-  RC(fft_strided(ctx, s, 1, c, 1, +1, ctx->d_mask_tmp));
-  RC(fft_x_c2r(ctx, s, c, r, 1, (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz)), ctx->d_mask_tmp));
+  const T scale = (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz));
+  if (iplane_ok<T>(ctx)) {
+    RC(fft_plane_inv(ctx, s, c, r, 1, scale, ctx->d_mask_tmp));
+  } else {
+    RC(fft_strided(ctx, s, 1, c, 1, +1, ctx->d_mask_tmp));
+    RC(fft_x_c2r(ctx, s, c, r, 1, scale, ctx->d_mask_tmp));
+  }
   HIPCHK(hipMemcpyAsync(out, s.src[0], sizeof(T) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
@@ -2183,6 +2223,21 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     }
     ctx->plane = best_plane;
   }
+  // the inverse y + x transform: separate passes (0) or the in-place plane kernel (1)
+  if (!ctx->iplane_fixed) {
+    const bool f32 = ctx->mixed;
+    int best_i = ctx->iplane;
+    double best_p = 1e30;
+    for (int on : {0, 1}) {
+      ctx->iplane = on;
+      if (on && !(f32 ? iplane_ok<float>(ctx) : iplane_ok<double>(ctx))) continue;
+      double t = 0;
+      RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t, f32));
+      if (ctx->tune_log) std::fprintf(stderr, "autotune iplane=%d poten=%.4f ms\n", on, t);
+      if (t < best_p) { best_p = t; best_i = on; }
+    }
+    ctx->iplane = best_i;
+  }
   drop_graphs(ctx);  // captured graphs hold the old launch shapes
   if (tx_out) *tx_out = best_tx;
   if (zc_out) *zc_out = best_zc;
@@ -2212,6 +2267,11 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields) {
   if (check_ctx(ctx) != EFFPOT_OK) return 0;
   return fp32_fields ? plane_mode<float>(ctx) : plane_mode<double>(ctx);
+}
+
+int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields) {
+  if (check_ctx(ctx) != EFFPOT_OK) return 0;
+  return (fp32_fields ? iplane_ok<float>(ctx) : iplane_ok<double>(ctx)) ? 1 : 0;
 }
 
 int effpot_solve(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_result* out) {

@@ -507,6 +507,8 @@ template <typename T> struct PlaneArgs {
   const cx<T>* twx;
   const cx<T>* twy;
   const int* active;
+  T scale;            // inverse only: out = aff0[f] + (scale*aff1[f]) * x
+  T aff0[4], aff1[4];
 };
 
 // fft_step1 / fft_step3 for sub-group thread t (0 <= t < kThreads).
@@ -698,12 +700,15 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
     // element index, at the pitch Q (2Q = 2 mod 64 words), so no LDS access
     // of the transforms has a bank conflict
     const int tt = threadIdx.x, c = tt % CW;
+    // the N1-point first steps of all H lines run at once (NT / N2 lines per
+    // round keep every thread busy); the N2-point second steps then run in R
+    // rounds of CW lines, which touch disjoint lines
+    constexpr int CW1 = kThreads * G / N2;
+    static_assert(!fits || CW1 >= H, "the first steps of all lines in one round");
     // 2. x transforms, in place in pair column p
-    for (int j = 0; j < R; j++) {
-      const int p = j * CW + c;
-      const bool on = p < H;
-      const int n2 = tt / CW;
-      if (on && tt < CW * N2) {
+    {
+      const int p = tt % CW1, n2 = tt / CW1;
+      if (p < H && n2 < N2) {
         cx<T> v[N1];
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) v[n1] = plane[(N2 * n1 + n2) * Q + p];
@@ -715,7 +720,11 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           plane[(k1 * N2 + n2) * Q + p] = w;  // the positions this thread read
         }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    for (int j = 0; j < R; j++) {
+      const int p = j * CW + c;
+      const bool on = p < H;
       const int k1 = tt / CW;
       cx<T> b[N2];
       if (on) {
@@ -728,16 +737,14 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) plane[(k1 + N1 * k2) * Q + p] = b[k2];
       }
-      __syncthreads();
     }
+    __syncthreads();
     // 3. y transforms of the half-spectrum columns; column kk owns plane rows
     //    kk and N - kk (kk = 0: rows 0 and H): element m < H at row ra, else rb
-    for (int j = 0; j < R; j++) {
-      const int kk = j * CW + c;
-      const bool on = kk < H;
+    {
+      const int kk = tt % CW1, n2 = tt / CW1;
+      const bool s1 = kk < H && n2 < N2;
       const int ra = kk, rb = kk == 0 ? H : N - kk;
-      const int n2 = tt / CW;
-      const bool s1 = on && tt < CW * N2;
       cx<T> v[N1];
       if (s1) {
 #pragma unroll
@@ -767,7 +774,12 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)] = v[k1];
         }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    for (int j = 0; j < R; j++) {
+      const int kk = j * CW + c;
+      const bool on = kk < H;
+      const int ra = kk, rb = kk == 0 ? H : N - kk;
       const int k1 = tt / CW;
       cx<T> b[N2];
       if (on) {
@@ -782,15 +794,15 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           for (int k2 = 0; k2 < N2; k2++) dst[kk + (long)(k1 + N1 * k2) * a.pitch] = b[k2];
         }
       }
-      __syncthreads();
-      if (on && kk == 0) {  // W[ky] into its own slots
+      if (j == 0) {  // column 0, the packed W, lives in round 0
+        __syncthreads();
+        if (on && kk == 0) {  // W[ky] into its own slots
 #pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) {
-          const int m = k1 + N1 * k2;
-          plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)] = b[k2];
+          for (int k2 = 0; k2 < N2; k2++) {
+            const int m = k1 + N1 * k2;
+            plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)] = b[k2];
+          }
         }
-      }
-      if (j == 0) {
         __syncthreads();
         for (int ky = tt; ky < N; ky += kThreads * G) {  // separate the packed column: Y0 and Y_H
           const int km = (N - ky) % N;
@@ -800,7 +812,193 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           dst[H + (long)ky * a.pitch] = cx<T>{T(0.5) * (w.y + wm.y), T(-0.5) * (w.x - wm.x)};
         }
       }
+    }
+  }
+}
+
+// ---- inverse y and x transforms of one z plane, in place ------------------
+// The mirror of k_fft_plane_r2c_ip for the half spectra that the z
+// convolutions leave: half spectrum -> inverse y FFT of every column ->
+// inverse x FFT of row pairs -> real rows, with one HBM read and one write
+// (the separate passes, k_fft_strided + k_fft_x_c2r, make two of each).
+// LDS holds the plane as N rows of Q = N/2 + 1 complex values:
+//  1. column kk (1 <= kk < H) of the half spectrum goes to plane rows kk
+//     (elements ky < H, slot ky) and N - kk (slot ky - H); the columns kx = 0
+//     and H, whose inverse a c2r transform keeps only the real part of, travel
+//     as one complex column W = S0h + i SHh of their Hermitian parts
+//     (S0h[ky] = (S0[ky] + conj S0[N-ky])/2): the inverse of W is then
+//     Re x0 + i Re xH, in rows 0 and H;
+//  2. inverse y FFT of each column in place in its two rows;
+//  3. the rows 2p and 2p+1 of a column come out of the y FFT in lanes l and
+//     l ^ 32 of one wave, which swap them and write the packed x input of row
+//     pair p into the column's own rows: z_p[kk] = x[2p][kk] + i x[2p+1][kk]
+//     at row kk, z_p[N-kk] = conj x[2p][kk] + i conj x[2p+1][kk] at row
+//     N - kk, slot p (column 0: z_p[0] = Re w[2p] + i Re w[2p+1] at row 0,
+//     z_p[H] = Im w[2p] + i Im w[2p+1] at row H);
+//  4. inverse x FFT of each pair column p in place;
+//  5. rows 2p and 2p+1 = Re, Im of z_p, stored as aff0 + scale aff1 x.
+// The work-group (G x kThreads threads) works on 32 lines at a time, so the
+// lines of a wave are 32 columns in each of its halves (step 3 needs that):
+// G = N1 / 8 (N >= 24), and the plane fits twice per CU (PlaneIP).
+template <typename T, int N> struct PlaneInv {
+  static constexpr int G = Shape<N>::N1 >= 8 ? Shape<N>::N1 / 8 : 1;
+  static constexpr bool value = PlaneIP<T, N, G>::value && kThreads * G / Shape<N>::N1 == 32;
+};
+template <typename T, int N>
+__global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2r_ip(PlaneArgs<T> a) {
+  constexpr bool fits = PlaneInv<T, N>::value;
+  constexpr int G = PlaneInv<T, N>::G;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, H = N / 2, Q = H + 1;
+  constexpr int NT = kThreads * G, CW = NT / N1;  // lines per round
+  constexpr int R = (H + CW - 1) / CW;  // rounds
+  if (!a.active[blockIdx.z]) return;
+  __shared__ cx<T> plane[fits ? N * Q : 1];
+  if constexpr (fits) {
+    const int f = blockIdx.y, iz = blockIdx.x;
+    const cx<T>* src = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    T* dst = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
+    const int tt = threadIdx.x;
+    // element m of column kk: plane row (m < H ? kk : kk == 0 ? H : N - kk), slot m mod H
+    auto pos = [](int kk, int m) {
+      const int row = m < H ? kk : (kk == 0 ? H : N - kk);
+      return row * Q + (m < H ? m : m - H);
+    };
+    // 1. a lane per element (ky, kk), consecutive lanes along a half-spectrum
+    //    row, and lane ky < N for element ky of the packed column W; every
+    //    load issued before the first LDS store
+    {
+      constexpr int NE = N * H, NL = (NE + NT - 1) / NT;
+      static_assert(N <= NT, "one lane per element of W");
+      cx<T> w[4];
+      const int km = (N - tt) % N;
+      if (tt < N) {
+        w[0] = src[(long)tt * a.pitch]; w[1] = src[(long)km * a.pitch];
+        w[2] = src[(long)tt * a.pitch + H]; w[3] = src[(long)km * a.pitch + H];
+      }
+      cx<T> v[NL];
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int e = tt + k * NT;
+        if (NE % NT == 0 || e < NE) {
+          const int ky = e / H, kk = e - ky * H;
+          if (kk > 0) v[k] = src[(long)ky * a.pitch + kk];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int e = tt + k * NT;
+        if (NE % NT == 0 || e < NE) {
+          const int ky = e / H, kk = e - ky * H;
+          if (kk > 0) plane[pos(kk, ky)] = v[k];
+        }
+      }
+      if (tt < N) {
+        const cx<T> s0h{T(0.5) * (w[0].x + w[1].x), T(0.5) * (w[0].y - w[1].y)};
+        const cx<T> shh{T(0.5) * (w[2].x + w[3].x), T(0.5) * (w[2].y - w[3].y)};
+        plane[pos(0, tt)] = cx<T>{s0h.x - shh.y, s0h.y + shh.x};
+      }
+    }
+    __syncthreads();
+    const int c = tt % CW;
+    // the N1-point first steps of all H lines run at once (NT / N2 lines per
+    // round keep every thread busy); the N2-point second steps then run in
+    // rounds of CW = 32 lines
+    constexpr int CW1 = NT / N2, R1 = (H + CW1 - 1) / CW1;
+    // 2-3. inverse y transforms; the rounds of the second step touch disjoint
+    //      rows, so only the steps inside a round are separated by barriers
+    for (int j = 0; j < R1; j++) {
+      const int kk = j * CW1 + tt % CW1, n2 = tt / CW1;
+      if (kk < H && n2 < N2) {
+        cx<T> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) v[n1] = plane[pos(kk, N2 * n1 + n2)];
+        RegFFT<T, N1>::run(v, +1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+          cx<T> w = v[k1];
+          if (k1 > 0 && n2 > 0) w = cmul(w, conjg(a.twy[n2 * k1]));
+          plane[pos(kk, k1 * N2 + n2)] = w;  // the positions this thread read
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j < R; j++) {
+      const int kk = j * CW + c;
+      const bool on = kk < H;
+      const int k1 = tt / CW;
+      cx<T> b[N2];
+      if (on) {
+#pragma unroll
+        for (int m2 = 0; m2 < N2; m2++) b[m2] = plane[pos(kk, k1 * N2 + m2)];
+        RegFFT<T, N2>::run(b, +1);  // b[k2] = x[k1 + N1 k2]
+      } else {
+#pragma unroll
+        for (int m2 = 0; m2 < N2; m2++) b[m2] = cx<T>{T(0), T(0)};
+      }
+      // lane l ^ 32 holds the same column at k1 ^ 1: rows 2p and 2p+1
+      cx<T> o[N2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) o[k2] = cx<T>{__shfl_xor(b[k2].x, 32, 64), __shfl_xor(b[k2].y, 32, 64)};
+      __syncthreads();  // every element of the column is read before its slots are rewritten
+      if (on) {
+        const bool odd = k1 & 1;
+        const int ra = kk, rb = kk == 0 ? H : N - kk;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const int p = (k1 >> 1) + (N1 / 2) * k2;
+          const cx<T> xa = odd ? o[k2] : b[k2], xb = odd ? b[k2] : o[k2];  // x[2p], x[2p+1]
+          cx<T> z;
+          if (kk == 0) z = odd ? cx<T>{xa.y, xb.y} : cx<T>{xa.x, xb.x};
+          else z = odd ? cx<T>{xa.x + xb.y, xb.x - xa.y} : cx<T>{xa.x - xb.y, xa.y + xb.x};
+          plane[(odd ? rb : ra) * Q + p] = z;
+        }
+      }
+    }
+    __syncthreads();
+    // 4. inverse x transforms, in place in pair column p
+    for (int j = 0; j < R1; j++) {
+      const int p = j * CW1 + tt % CW1, n2 = tt / CW1;
+      if (p < H && n2 < N2) {
+        cx<T> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) v[n1] = plane[(N2 * n1 + n2) * Q + p];
+        RegFFT<T, N1>::run(v, +1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+          cx<T> w = v[k1];
+          if (k1 > 0 && n2 > 0) w = cmul(w, conjg(a.twx[n2 * k1]));
+          plane[(k1 * N2 + n2) * Q + p] = w;
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j < R; j++) {
+      const int p = j * CW + c;
+      const bool on = p < H;
+      const int k1 = tt / CW;
+      cx<T> b[N2];
+      if (on) {
+#pragma unroll
+        for (int m = 0; m < N2; m++) b[m] = plane[(k1 * N2 + m) * Q + p];
+        RegFFT<T, N2>::run(b, +1);
+      }
       __syncthreads();
+      if (on) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) plane[(k1 + N1 * k2) * Q + p] = b[k2];
+      }
+    }
+    __syncthreads();
+    // 5. a lane per element i of a row pair: coalesced row stores
+    {
+      const T off = a.aff0[f], sc = a.scale * a.aff1[f];
+      constexpr int NE = N * H;
+      for (int e = tt; e < NE; e += NT) {
+        const int p = e / N, i = e - p * N;
+        const cx<T> z = plane[i * Q + p];
+        dst[(long)(2 * p) * N + i] = off + sc * z.x;
+        dst[(long)(2 * p + 1) * N + i] = off + sc * z.y;
+      }
     }
   }
 }

@@ -507,6 +507,12 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
  * plane launch in place (launches 1, 7 and 13 are then empty). */
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields);
 
+/* Benchmark hook (no reference counterpart): 1 when poten() and
+ * effpot_fft_c2r run the inverse y and x transforms as one in-place plane
+ * launch per z plane (launches 3, 9 and 15 of the sequence are then empty),
+ * 0 for separate passes; with FP32 (fp32_fields = 1) or FP64 fields. */
+int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields);
+
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
  * one step of its own phase (a START slot gets poten + its first RK step).
  * Energies are FP64 in every mode; a slot is DONE at the first energy check
@@ -527,7 +533,8 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
  * dividing nx, z chunk 4/8/16/32) on the live state and keeps the fastest,
  * then times poten under each forward-transform schedule that applies
  * (effpot_plane_fused: 0, 1, 3; the environment variable EFFPOT_PLANE fixes
- * it instead) and keeps the fastest.  Overwrites the integrator state like
+ * it instead) and keeps the fastest, and the same for the inverse schedule
+ * (effpot_plane_inverse_fused; EFFPOT_IPLANE).  Overwrites the integrator state like
  * effpot_kernel_time: call it before uploading states. */
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
 /* Sets the stencil tile explicitly (x width, z chunk). */
