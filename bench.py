@@ -74,6 +74,8 @@ def parse():
                     help="HBM the serving workload may reserve for solver state (admission by footprint)")
     ap.add_argument("--stress-steps", type=int, default=300)
     ap.add_argument("--seed", type=int, default=20261203, help="noise seed of the initial droplets")
+    ap.add_argument("--graph-steps", type=int, default=24,
+                    help="predictor-corrector steps per graph launch between energy checks (1: one per tick)")
     a = ap.parse_args()
     if a.tol_fp32 is None:
         a.tol_fp32 = a.tol
@@ -182,7 +184,7 @@ def run_cfg0(a):
     import eff_pot_amd as ep
     p = ep.make_params(64, xmax=24.0, n4=300.0, core=ep.CORE_OTC, h_core=2.377, psi_complex=0)
     opts = ep.SolveOpts(dt=ep.default_dt(p), tol=0.0, n_rk=3, pener=100, max_steps=2000, polish_steps=0,
-                        tol_fp32=0.0, check_finite=0)
+                        tol_fp32=0.0, check_finite=0, graph_steps=a.graph_steps)
     with ep.Context(p) as c:
         t0 = time.perf_counter()
         for _ in range(a.steps):
@@ -360,7 +362,7 @@ def run_stress(a):
         p = ep.make_params(256, xmax=51.2, n4=5000.0, core=ep.CORE_OTC, h_core=2.377, precision=prec, batch=B,
                            psi_complex=0)
         opts = ep.SolveOpts(dt=ep.default_dt(p), tol=0.0, n_rk=3, pener=100, max_steps=a.stress_steps,
-                            polish_steps=0, tol_fp32=0.0, check_finite=0)
+                            polish_steps=0, tol_fp32=0.0, check_finite=0, graph_steps=a.graph_steps)
         with ep.Context(p) as c:
             for b in range(B):
                 c.init_fermi(0.0218, R, 0.6, 1e-6, 11 + b, slot=b)
@@ -526,7 +528,7 @@ def main():
     p, R = workload(a.n, a.mode, B)
     dt = ep.default_dt(p)
     opts = ep.SolveOpts(dt=dt, tol=a.tol, n_rk=3, pener=50, max_steps=a.max_steps, polish_steps=a.max_steps,
-                        tol_fp32=a.tol_fp32, check_finite=0)
+                        tol_fp32=a.tol_fp32, check_finite=0, graph_steps=a.graph_steps)
     zs = positions(R, B)
     ctx = ep.Context(p)
     if a.tile:

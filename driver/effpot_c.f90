@@ -31,7 +31,7 @@ module effpot_c
     real(c_double) :: dt, tol
     integer(c_int32_t) :: n_rk, pener, max_steps, polish_steps
     real(c_double) :: tol_fp32
-    integer(c_int32_t) :: check_finite, pad_
+    integer(c_int32_t) :: check_finite, graph_steps
   end type
 
   type, bind(c) :: effpot_snapshot

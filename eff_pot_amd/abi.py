@@ -81,7 +81,7 @@ class Energies(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("tol", C.c_double), ("n_rk", C.c_int32),
                 ("pener", C.c_int32), ("max_steps", C.c_int32), ("polish_steps", C.c_int32),
-                ("tol_fp32", C.c_double), ("check_finite", C.c_int32), ("pad_", C.c_int32)]
+                ("tol_fp32", C.c_double), ("check_finite", C.c_int32), ("graph_steps", C.c_int32)]
 
 
 SLOT_IDLE, SLOT_START, SLOT_RK, SLOT_PC, SLOT_DONE, SLOT_FAILED = range(6)

@@ -193,6 +193,7 @@ struct effpot_ctx {
   int nblk_max = 0;
   double* d_red = nullptr;    // [B][16]
   double* d_scale = nullptr;  // [B]
+  double* d_dscale = nullptr; // [B] factor taking a step's density to n4 atoms
   double* d_n4 = nullptr;     // [B] atoms each slot is normalised to
   double* d_dt = nullptr;     // [B] time step each slot is advanced with
   std::vector<double> h_dt;   // per-slot override (0: the solve options' dt)
@@ -211,6 +212,9 @@ struct effpot_ctx {
   bool tune_log = false;            // EFFPOT_TUNE_LOG=1 prints the autotuner's timings
   hipGraphExec_t pc_exec[2] = {nullptr, nullptr};  // [0] FP64, [1] FP32
   hipGraphExec_t pc_exec2[2] = {nullptr, nullptr}; // the same as two concurrent slot groups
+  hipGraphExec_t pck_exec[2] = {nullptr, nullptr};  // pck_k[] consecutive PC steps in one graph
+  hipGraphExec_t pck_exec2[2] = {nullptr, nullptr}; // the same, two concurrent slot groups
+  int pck_k[2] = {0, 0}, pck_k2[2] = {0, 0};       // steps captured in pck_exec / pck_exec2
   hipStream_t st2 = nullptr;                       // second stream of the two-group graph
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool split = true;                               // two slot groups when B >= 2 (EFFPOT_SPLIT=0 disables)
@@ -625,7 +629,8 @@ int den_from_cur(effpot_ctx* ctx, const int* act, bool f32) {
 template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring = nullptr, int op1 = -1,
                             int op2 = -1) {
   hipLaunchKernelGGL((k_reduce<K>), dim3(ctx->B), dim3(kThreads), 0, ctx->st, ctx->d_partial, nblk(ctx), ctx->d_red,
-                     act, mode, (const double*)ctx->d_n4, ctx->dxyz, ctx->d_scale, ring, op1, op2);
+                     act, mode, (const double*)ctx->d_n4, ctx->dxyz, mode == 2 ? ctx->d_dscale : ctx->d_scale,
+                     ring, op1, op2);
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -821,6 +826,26 @@ int force_t(effpot_ctx* ctx, const int* act) {
   return EFFPOT_OK;
 }
 
+// Imaginary time: the density of the stage or predicted state (written with
+// its block partial sums by the step kernel) scaled to n4 atoms before the
+// step's poten.  The step multiplies psi by about 1 - mu dt and only the
+// renormalisation after the step restores n4, so the field of the unscaled
+// density would be that of ~(1 - 2 mu dt) n4 atoms and the relaxation's fixed
+// point would miss the eigenvalue equation by O(dt).  Real time keeps
+// den = |psi|^2 as the reference's stepper has it.
+int step_den_scale(effpot_ctx* ctx, const int* act, bool f32) {
+  if (ctx->rt) return EFFPOT_OK;
+  RC(reduce<2>(ctx, 2, act));
+  if (f32)
+    hipLaunchKernelGGL((k_scale_den<float>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, ctx->s32.den,
+                       (const double*)ctx->d_dscale, act, ctx->N);
+  else
+    hipLaunchKernelGGL((k_scale_den<double>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, ctx->s64.den,
+                       (const double*)ctx->d_dscale, act, ctx->N);
+  LAUNCHCHK();
+  return EFFPOT_OK;
+}
+
 // RK-Gill step (steprk.f90:286-405).  Leaves the norm reduction in d_red/d_scale.
 template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act, bool f32 = false) {
   const double r2 = std::sqrt(2.0);
@@ -845,6 +870,7 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
       LAUNCHCHK();
       if (j <= 2) RC(dyn_field(ctx, act));
     }
+    if (j <= 2) RC(step_den_scale(ctx, act, f32));
     if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
     if (ctx->any_motion && j <= 2) RC(force_t(ctx, act));
     if (ctx->any_dyn && j <= 2) RC(dyn_hiv(ctx, act));
@@ -868,6 +894,7 @@ template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act
     LAUNCHCHK();
     RC(dyn_field(ctx, act));
   }
+  RC(step_den_scale(ctx, act, f32));
   RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
   if (ctx->any_motion) RC(force_t(ctx, act));
   if (ctx->any_dyn) RC(dyn_hiv(ctx, act));  // potinvar
@@ -1073,6 +1100,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   if ((rc = dalloc(ctx, &ctx->d_partial, (size_t)ctx->B * ctx->nblk_max * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_red, (size_t)ctx->B * 16)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_scale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_dscale, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_n4, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   if ((rc = dalloc(ctx, &ctx->d_dt, (size_t)ctx->B)) != EFFPOT_OK) return bail(rc);
   ctx->h_dt.assign(ctx->B, 0.0);
@@ -1117,6 +1145,8 @@ int effpot_ctx_destroy(effpot_ctx* ctx) {
   for (int k = 0; k < 2; k++) {
     if (ctx->pc_exec[k]) hipGraphExecDestroy(ctx->pc_exec[k]);
     if (ctx->pc_exec2[k]) hipGraphExecDestroy(ctx->pc_exec2[k]);
+    if (ctx->pck_exec[k]) hipGraphExecDestroy(ctx->pck_exec[k]);
+    if (ctx->pck_exec2[k]) hipGraphExecDestroy(ctx->pck_exec2[k]);
   }
   for (void* v : ctx->allocs) hipFree(v);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
@@ -1867,16 +1897,27 @@ static void drop_graphs(effpot_ctx* ctx) {
   for (int k = 0; k < 2; k++) {
     if (ctx->pc_exec[k]) { hipGraphExecDestroy(ctx->pc_exec[k]); ctx->pc_exec[k] = nullptr; }
     if (ctx->pc_exec2[k]) { hipGraphExecDestroy(ctx->pc_exec2[k]); ctx->pc_exec2[k] = nullptr; }
+    if (ctx->pck_exec[k]) { hipGraphExecDestroy(ctx->pck_exec[k]); ctx->pck_exec[k] = nullptr; }
+    if (ctx->pck_exec2[k]) { hipGraphExecDestroy(ctx->pck_exec2[k]); ctx->pck_exec2[k] = nullptr; }
+    ctx->pck_k[k] = ctx->pck_k2[k] = 0;
   }
 }
 
 // The PC step as two independent chains, one per slot group (masks dA, dB),
 // on two streams inside one graph: the groups touch disjoint slot data, and
-// each chain's launch tails and barriers overlap the other's work.
+// each chain's launch tails and barriers overlap the other's work.  k > 1
+// captures k consecutive steps of each chain in the same graph (one launch
+// for k steps; the same kernels in the same order as k single replays).
 template <typename T>
-static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA, const int* dB) {
+static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA, const int* dB, int k = 1) {
   const int which = f32 ? 1 : 0;
-  if (!ctx->pc_exec2[which]) {
+  hipGraphExec_t& ex = k > 1 ? ctx->pck_exec2[which] : ctx->pc_exec2[which];
+  if (ex && k > 1 && ctx->pck_k2[which] != k) {
+    HIPCHK(hipStreamSynchronize(ctx->st));  // a replay of the old length may still run
+    hipGraphExecDestroy(ex);
+    ex = nullptr;
+  }
+  if (!ex) {
     std::lock_guard<std::recursive_mutex> lk(g_api_mu);
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
@@ -1887,8 +1928,10 @@ static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA,
     for (int part = 0; part < 2 && rc == EFFPOT_OK; part++) {
       if (part == 1) std::swap(ctx->st, ctx->st2);  // launches of group B go to the second stream
       const int* d = part == 0 ? dA : dB;
-      rc = step_pc_t(ctx, s, d, f32);
-      if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, d, f32);
+      for (int j = 0; j < k && rc == EFFPOT_OK; j++) {
+        rc = step_pc_t(ctx, s, d, f32);
+        if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, d, f32);
+      }
       if (part == 1) std::swap(ctx->st, ctx->st2);
     }
     ctx->corr_den = true;
@@ -1898,44 +1941,59 @@ static int pc_step_graph2(effpot_ctx* ctx, State<T>& s, bool f32, const int* dA,
     hipError_t e = hipStreamEndCapture(ctx->st, &g);
     if (rc != EFFPOT_OK) return ctx->fail(rc, "two-group graph capture failed");
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&ctx->pc_exec2[which], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    if (k > 1) ctx->pck_k2[which] = k;
   }
-  HIPCHK(hipGraphLaunch(ctx->pc_exec2[which], ctx->st));
+  HIPCHK(hipGraphLaunch(ex, ctx->st));
   return EFFPOT_OK;
 }
 
-// One predictor-corrector step + renormalisation of the slots in mask dP.  The
-// launch sequence only references device memory (rings, masks, fields), so it
-// is captured once per precision and replayed as a HIP graph (the time step of
-// each slot is read from device memory, so changing it needs no re-capture).
+// One predictor-corrector step + renormalisation of the slots in mask dP (k
+// consecutive steps when k > 1).  The launch sequence only references device
+// memory (rings, masks, fields), so it is captured once per precision and
+// replayed as a HIP graph (the time step of each slot is read from device
+// memory, so changing it needs no re-capture).
 template <typename T>
-static int pc_step_graph(effpot_ctx* ctx, State<T>& s, bool f32, const int* dP) {
+static int pc_step_graph(effpot_ctx* ctx, State<T>& s, bool f32, const int* dP, int k = 1) {
   const int which = f32 ? 1 : 0;
   if (!ctx->graphs) {
-    ctx->corr_den = false;
-    int rc = step_pc_t(ctx, s, dP, f32);
-    ctx->corr_den = true;
-    RC(rc);
-    return renorm_t(ctx, s, dP, f32);
+    for (int j = 0; j < k; j++) {
+      ctx->corr_den = false;
+      int rc = step_pc_t(ctx, s, dP, f32);
+      ctx->corr_den = true;
+      RC(rc);
+      RC(renorm_t(ctx, s, dP, f32));
+    }
+    return EFFPOT_OK;
   }
-  if (!ctx->pc_exec[which]) {
+  hipGraphExec_t& ex = k > 1 ? ctx->pck_exec[which] : ctx->pc_exec[which];
+  if (ex && k > 1 && ctx->pck_k[which] != k) {
+    HIPCHK(hipStreamSynchronize(ctx->st));  // a replay of the old length may still run
+    hipGraphExecDestroy(ex);
+    ex = nullptr;
+  }
+  if (!ex) {
     std::lock_guard<std::recursive_mutex> lk(g_api_mu);
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     ctx->corr_den = false;
-    int rc = step_pc_t(ctx, s, dP, f32);
+    int rc = EFFPOT_OK;
+    for (int j = 0; j < k && rc == EFFPOT_OK; j++) {
+      rc = step_pc_t(ctx, s, dP, f32);
+      if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, dP, f32);
+    }
     ctx->corr_den = true;
-    if (rc == EFFPOT_OK) rc = renorm_t(ctx, s, dP, f32);
     hipError_t e = hipStreamEndCapture(ctx->st, &g);
     if (rc != EFFPOT_OK) return rc;
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&ctx->pc_exec[which], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) return ctx->fail(EFFPOT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    if (k > 1) ctx->pck_k[which] = k;
   }
-  HIPCHK(hipGraphLaunch(ctx->pc_exec[which], ctx->st));
+  HIPCHK(hipGraphLaunch(ex, ctx->st));
   return EFFPOT_OK;
 }
 
@@ -2016,19 +2074,27 @@ static int tick_phase(effpot_ctx* ctx, bool f32, const effpot_solve_opts* opt, e
       }
   }
   if (any_p) {
+    // k > 1 PC steps in one graph launch when no slot of this phase reaches an
+    // energy check (hence a phase change) or its step cap within them
+    int k = 1;
+    if (ctx->graphs && opt->graph_steps > 1) {
+      k = opt->graph_steps;
+      for (int b = 0; b < B; b++)
+        if (mp[b] && (sl[b].since + k >= opt->pener || sl[b].steps + k >= (f32 ? cap_f32 : cap))) k = 1;
+    }
     RC(push_mask(ctx, 2, mp));
     if (ctx->graphs && ctx->split && B >= 2) {
       std::vector<int> ma(B, 0), mb(B, 0);
       for (int b = 0; b < B; b++) (b < B / 2 ? ma : mb)[b] = mp[b];
       RC(push_mask(ctx, 5, ma));
       RC(push_mask(ctx, 6, mb));
-      RC(pc_step_graph2(ctx, s, f32, ctx->d_mask + 5 * B, ctx->d_mask + 6 * B));
+      RC(pc_step_graph2(ctx, s, f32, ctx->d_mask + 5 * B, ctx->d_mask + 6 * B, k));
     } else {
-      RC(pc_step_graph(ctx, s, f32, dP));
+      RC(pc_step_graph(ctx, s, f32, dP, k));
     }
     for (int b = 0; b < B; b++)
       if (mp[b]) {
-        sl[b].steps++; sl[b].since++;
+        sl[b].steps += k; sl[b].since += k;
         if (sl[b].since >= opt->pener || sl[b].steps >= cap) { me[b] = 1; any_e = true; }
       }
   }
@@ -2127,6 +2193,7 @@ template <typename T> static int time_corrector_in_steps(effpot_ctx* ctx, State<
   for (int r = 0; r < reps && rc == EFFPOT_OK; r++) {
     if (whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
     rc = launch_step<T, 0>(ctx, s, act, f32);
+    if (rc == EFFPOT_OK) rc = step_den_scale(ctx, act, f32);
     if (rc == EFFPOT_OK) rc = f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act);
     if (rc != EFFPOT_OK) break;
     if (!whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));

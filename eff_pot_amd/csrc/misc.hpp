@@ -273,11 +273,24 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* partial, int 
 #pragma unroll
     for (int k = 0; k < K; k++) out[s * K + k] = v[k];
     if (mode == 1) scale[s] = sqrt(n4[s] / (v[1] * dxyz));
+    if (mode == 2) scale[s] = n4[s] / (v[1] * dxyz);  // density factor to n4 atoms
     if (ring) {  // role rotations of the step that produced these sums
       ring_apply(ring + 8 * s, op1);
       ring_apply(ring + 8 * s, op2);
     }
   }
+}
+
+// den *= f[slot]: the density of a stage or predicted state scaled to n4
+// atoms (f from k_reduce mode 2) before the step's poten.
+template <typename P>
+__global__ void k_scale_den(P* den, const double* f, const int* active, long N) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const double fs = f[s];
+  P* d = den + (long)s * N;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    d[i] = (P)((double)d[i] * fs);
 }
 
 // ---- precision conversion (FP32 phase -> FP64 polish) -----------------------

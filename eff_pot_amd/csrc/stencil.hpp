@@ -242,12 +242,16 @@ template <typename T, typename V, typename P> struct StepArgs {
   int rt;         // real time: H psi -> i H psi (complex psi only)
 };
 
-// Hamming predictor + modifier (steppc.f90:53-64).  out = o3's buffer.
+// Hamming predictor + modifier (steppc.f90:53-64).  out = o3's buffer.  The
+// block partial sums of the modified density feed the scaling of that density
+// to n4 atoms before the step's poten.
 template <typename T, typename V, typename P, int TX>
 EP_DEV void k_pc_predict_body(const StepArgs<T, V, P>& a) {
   const int s = blockIdx.z;
   if (!a.active[s]) return;
   __shared__ V lds[2 * Tile<TX>::PL];
+  __shared__ double red[8];
+  double v[2] = {0.0, 0.0};
   const long so = (long)s * a.slot;
   const V* cur = a.cur + so;
   V* o3 = a.o3 + so; const V* h1 = a.h1 + so; V* h2 = a.h2 + so; V* pc = a.pc + (long)s * a.pcq_slot;
@@ -266,8 +270,16 @@ EP_DEV void k_pc_predict_body(const StepArgs<T, V, P>& a) {
       o3[i] = pn;  // modified psi goes to the retired psi_{n-3} buffer
       pc[i] = pred;
       h2[i] = hp;
-      den[i] = (P)norm2(pn);
+      const T d = norm2(pn);
+      den[i] = (P)d;
+      v[1] += (double)d;
     });
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    double* p = a.partial + ((long)s * a.nblk + b) * 2;
+    p[0] = 0.0; p[1] = v[1];
+  }
 }
 
 // Hamming corrector + final value (steppc.f90:135-151), errHe and norm partials.

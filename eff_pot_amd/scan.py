@@ -20,9 +20,10 @@ def scan_positions(natoms, npos, zmax_frac=1.6):
 
 
 def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, tol=1e-9, tol_fp32=1e-7,
-             max_steps=60000, warm=False, seed=20261203, h_core=2.377, progress=None):
+             max_steps=60000, warm=False, seed=20261203, h_core=2.377, progress=None, graph_steps=24):
     """Relax all positions in one batched context (ticked until every slot is
-    done); `progress(msg)` is called about every 30 s."""
+    done; up to `graph_steps` predictor-corrector steps per graph launch
+    between energy checks); `progress(msg)` is called about every 30 s."""
     xmax = spacing * n / 2
     p = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=npos,
                     psi_complex=0)
@@ -30,7 +31,7 @@ def run_scan(n=96, spacing=0.6, natoms=1000.0, npos=48, precision=PREC_MIXED, to
     R = 2.22 * natoms ** (1.0 / 3.0)
     dt = default_dt(p)
     opts = SolveOpts(dt=dt, tol=tol, n_rk=3, pener=50, max_steps=max_steps, polish_steps=max_steps,
-                     tol_fp32=tol_fp32, check_finite=0)
+                     tol_fp32=tol_fp32, check_finite=0, graph_steps=graph_steps)
     warm_psi = None
     if warm:
         pw = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=h_core, precision=precision, batch=1,

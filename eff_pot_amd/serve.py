@@ -270,9 +270,9 @@ class Server:
     block per class with that many slots, created up front."""
 
     def __init__(self, slots_per_class=None, budget_bytes=None, block_slots=None, tol=1e-9, tol_fp32=1e-7,
-                 max_steps=60000, pener=50, precision=PREC_MIXED, max_retries=2):
+                 max_steps=60000, pener=50, precision=PREC_MIXED, max_retries=2, graph_steps=24):
         kw = dict(tol=tol, n_rk=3, pener=pener, max_steps=max_steps, polish_steps=max_steps,
-                  tol_fp32=tol_fp32, check_finite=0)
+                  tol_fp32=tol_fp32, check_finite=0, graph_steps=graph_steps)
         if slots_per_class is not None:
             self.budget = Budget(float("inf"))
             self.lanes = {n: _Lane(n, k, kw, precision, self.budget, fixed=True, max_retries=max_retries)

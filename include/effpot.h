@@ -238,7 +238,10 @@ typedef struct effpot_solve_opts {
   double tol_fp32;  /* MIXED only: per-step |dE/E| that ends the FP32 phase
                        (>= tol)                                            */
   int32_t check_finite; /* 1: scan the density for NaN/<=0 at every energy */
-  int32_t pad_;
+  int32_t graph_steps;  /* predictor-corrector steps replayed per graph launch
+                           while no slot of a phase is due an energy check,
+                           a phase change or its step cap (0 or 1: one step
+                           per effpot_tick; results are the same bits)      */
 } effpot_solve_opts;
 
 typedef struct effpot_result {

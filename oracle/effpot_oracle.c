@@ -1126,6 +1126,25 @@ static int den_ok(const oracle_t* o) {
 }
 
 /* steprk.f90:261-420 with the imaginary-time H. */
+
+/* The mean field inside an imaginary-time step is evaluated on the density of
+ * the stage or predicted state scaled to n4 atoms.  Each step multiplies psi
+ * by about 1 - mu dt (the renormalisation after the step restores n4), so
+ * a field of the unscaled density would be that of ~(1 - 2 mu dt) n4 atoms:
+ * the iteration's fixed point would then solve H[(1 + O(dt)) rho] psi = mu psi,
+ * not the eigenvalue equation, and the converged energies would depend on dt.
+ * With the scaled density the fixed point is the stationary state of the
+ * functional.  Real time (timec = i) keeps the reference's den = |psi|^2. */
+static void poten_of_step(oracle_t* o) {
+  if (!o->rt) {
+    double s = 0.0;
+    for (size_t i = 0; i < o->N; i++) s += o->den[i];
+    const double f = o->p.n4 / (s * o->dxyz);
+    for (size_t i = 0; i < o->N; i++) o->den[i] *= f;
+  }
+  oracle_poten(o);
+}
+
 int oracle_step_rk(oracle_t* o, double dt) {
   const double s2 = sqrt(2.0);
   const double arun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 1.0 / 6.0};
@@ -1185,7 +1204,7 @@ int oracle_step_rk(oracle_t* o, double dt) {
       }
       if (j <= 2) oracle_set_impurity_state(o, &o->simp); /* potenimp(rimp, invar) */
     }
-    if (j <= 2) oracle_poten(o);
+    if (j <= 2) poten_of_step(o);
     if (o->moving && j <= 2) force_imp(o);
     if (o->dyn && j <= 2) potinvar(o);
   }
@@ -1249,7 +1268,7 @@ int oracle_step_pc(oracle_t* o, double dt, double* err_he) {
     { int t = ih[1]; ih[1] = ih[0]; ih[0] = t; }
     oracle_set_impurity_state(o, &o->simp); /* potenimp(rimp, invar) */
   }
-  oracle_poten(o);
+  poten_of_step(o);
   if (o->moving) force_imp(o);
   if (o->dyn) potinvar(o);
   compute_hpsi(o, hp);

@@ -387,6 +387,43 @@ def test_graph_replay_matches_eager(precision, cplx, gpu_available, monkeypatch)
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision,batch", [(0, 1), (PREC_MIXED, 3)])
+def test_multi_step_graph_matches_single_steps(precision, batch, gpu_available):
+    """graph_steps = 7 replays 7 predictor-corrector steps per graph launch
+    whenever no slot is due an energy check or a phase change: the same
+    kernels in the same order, so every slot's energies, step count and final
+    wavefunction equal the one-step-per-tick run bit for bit.  7 does not
+    divide pener = 25, and one slot is admitted 4 ticks late, so the
+    single-step fallback runs too."""
+    print(f"[graph] {batch} slots precision={precision}: 7-step graphs vs single steps", flush=True)
+    out = {}
+    for gs in (0, 7):
+        opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=25, max_steps=160, polish_steps=60, tol_fp32=0.0,
+                         check_finite=0, graph_steps=gs)
+        p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=batch, precision=precision,
+                        psi_complex=0)
+        with ep.Context(p) as ctx:
+            for s in range(batch):
+                _load(ctx, s, [0.5, -1.0, 1.5][s], 21 + s, 25.0 + 5 * s)
+            slots = ctx.new_slots()
+            ticks = 0
+            while True:
+                for s in range(batch):
+                    if slots[s].phase == ep.SLOT_IDLE and ticks >= 4 * s:
+                        slots[s].phase = ep.SLOT_START
+                        slots[s].fp32 = 1 if precision else 0
+                if all(slots[s].phase in (ep.SLOT_DONE, ep.SLOT_FAILED) for s in range(batch)):
+                    break
+                ctx.tick(opts, slots)
+                ticks += 1
+            out[gs] = ([(slots[s].steps, slots[s].e.etot, slots[s].e.eimpu, slots[s].rel_de) for s in range(batch)],
+                       [ctx.psi(s) for s in range(batch)], ticks)
+    assert out[7][2] < out[0][2]
+    assert out[7][0] == out[0][0]
+    for a, b in zip(out[7][1], out[0][1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
 def test_real_psi_path_is_bit_identical(precision, gpu_available):
     """Imaginary time from a real start keeps Im psi = 0 exactly, so the real
