@@ -185,6 +185,8 @@ struct effpot_ctx {
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
   int iplane = 1;                 // inverse y and x passes fused per z plane where it fits (EFFPOT_IPLANE)
+  int zconv_th = 256;             // threads per z-convolution group: 256 or 512 (EFFPOT_ZCONV_TH)
+  bool zconv_fixed = false;       // EFFPOT_ZCONV_TH given: the autotuner keeps it
   bool iplane_fixed = false;      // EFFPOT_IPLANE given: the autotuner keeps it
   std::vector<double> h_eso;      // [B] spin-orbit energy of a P/D-state impurity
   double* d_stab = nullptr;       // [B][3][npot+2] P/D-state potential tables (on first use)
@@ -509,11 +511,18 @@ template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx
 template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act) {
   a.sstride = ctx->M; a.lm = zmap(ctx); a.twN = s.tw[2]; a.active = act;
   a.scale = (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz));
-  const int C = kThreads / n1_of(ctx->nz);
+  const int th = ctx->zconv_th;
+  const int C = th / n1_of(ctx->nz);
   dim3 g((a.lm.nlines + C - 1) / C, ngroups, ctx->B);
-#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN>), g, dim3(kThreads), 0, ctx->st, a)
-  FFT_SWITCH(ctx->nz, M_)
+  if (th == 512) {
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, 512>), g, dim3(512), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nz, M_)
 #undef M_
+  } else {
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, kThreads>), g, dim3(kThreads), 0, ctx->st, a)
+    FFT_SWITCH(ctx->nz, M_)
+#undef M_
+  }
   LAUNCHCHK();
   return EFFPOT_OK;
 }
@@ -1070,6 +1079,10 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_eso.assign(ctx->B, 0.0);
   if (const char* e = std::getenv("EFFPOT_PLANE")) { ctx->plane = std::atoi(e); ctx->plane_fixed = true; }
   if (const char* e = std::getenv("EFFPOT_IPLANE")) { ctx->iplane = std::atoi(e) != 0; ctx->iplane_fixed = true; }
+  if (const char* e = std::getenv("EFFPOT_ZCONV_TH")) {
+    ctx->zconv_th = std::atoi(e) == 512 ? 512 : 256;
+    ctx->zconv_fixed = true;
+  }
   ctx->h_imp.assign(ctx->B, effpot_impurity{});
   ctx->h_is_state.assign(ctx->B, 0);
   if ((rc = dalloc(ctx, &ctx->d_motion, ctx->B)) != EFFPOT_OK) return bail(rc);
@@ -2305,6 +2318,21 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     }
     ctx->iplane = best_i;
   }
+  // the z-convolution group: 16 or 32 lines (256 or 512 threads), timed on
+  // whole poten launches of the live precision
+  if (!ctx->zconv_fixed) {
+    const bool f32 = ctx->mixed;
+    int best_t = ctx->zconv_th;
+    double best_p = 1e30;
+    for (int th : {256, 512}) {
+      ctx->zconv_th = th;
+      double t = 0;
+      RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t, f32));
+      if (ctx->tune_log) std::fprintf(stderr, "autotune zconv_th=%d poten=%.4f ms\n", th, t);
+      if (t < best_p) { best_p = t; best_t = th; }
+    }
+    ctx->zconv_th = best_t;
+  }
   drop_graphs(ctx);  // captured graphs hold the old launch shapes
   if (tx_out) *tx_out = best_tx;
   if (zc_out) *zc_out = best_zc;
@@ -2334,6 +2362,11 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields) {
   if (check_ctx(ctx) != EFFPOT_OK) return 0;
   return fp32_fields ? plane_mode<float>(ctx) : plane_mode<double>(ctx);
+}
+
+int effpot_zconv_threads(effpot_ctx* ctx) {
+  if (check_ctx(ctx) != EFFPOT_OK) return 0;
+  return ctx->zconv_th;
 }
 
 int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields) {

@@ -172,10 +172,12 @@ struct LineMap {
 // ---- the two-level transform, step by step -------------------------------------
 // LDS layout between the steps: index (k1*N2 + n2)*C + c (c fastest: conflict free).
 
+// The group has TH threads (kThreads unless a kernel says otherwise) and
+// works on C = TH / N1 lines.
 // Step 1+2 for the lines of this group: a[n1] = x[N2 n1 + n2]; FFT_N1; twiddle; to LDS.
-template <typename T, int N, class Load>
+template <typename T, int N, int TH = kThreads, class Load>
 EP_DEV void fft_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load) {
-  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   const int t = threadIdx.x;
   if (t < C * N2) {
     const int c = t % C, n2 = t / C;
@@ -196,9 +198,9 @@ EP_DEV void fft_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load) {
   }
 }
 // Step 3: thread (c,k1) gathers b[n2] from LDS, FFT_N2; X[k1 + N1 k2] = b[k2].
-template <typename T, int N>
+template <typename T, int N, int TH = kThreads>
 EP_DEV void fft_step3(const cx<T>* lds, cx<T> (&b)[Shape<N>::N2], int sign) {
-  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   const int t = threadIdx.x;
   const int c = t % C, k1 = t / C;
 #pragma unroll
@@ -208,9 +210,9 @@ EP_DEV void fft_step3(const cx<T>* lds, cx<T> (&b)[Shape<N>::N2], int sign) {
 // Inverse of the layout above starting from registers: thread (c,k1) holds
 // X[k1 + N1 j] (j < N2).  Decomposition N = N2*N1: FFT_N2 over j, twiddle
 // w^{k1 k1'}, LDS, FFT_N1 -> x[k1' + N2 k2'] delivered to store(c, i, v).
-template <typename T, int N, class Store>
+template <typename T, int N, int TH = kThreads, class Store>
 EP_DEV void fft_from_regs(cx<T> (&a)[Shape<N>::N2], cx<T>* lds, const cx<T>* twN, int sign, Store store) {
-  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   const int t = threadIdx.x;
   {
     const int c = t % C, k1 = t / C;
@@ -286,9 +288,11 @@ template <typename T> struct ConvArgs {
   const cx<T>* twN;
   const int* active;
 };
-template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
-  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = kThreads / N1;
+// TH = 256 or 512 threads: C = 16 or 32 lines per group (128 B or 256 B runs
+// of FP32 complex per z plane); the autotuner picks one per context.
+template <typename T, int N, int TH = kThreads>
+__global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   if (!a.active[blockIdx.z]) return;
   __shared__ cx<T> lds[N * C];
   const int g = blockIdx.y;
@@ -314,10 +318,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
     for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[lm.addr(cc, kk1 + N1 * k2)] : T(0);
   };
   load_k(0);
-  fft_step1<T, N>(lds, a.twN, -1, load);
+  fft_step1<T, N, TH>(lds, a.twN, -1, load);
   __syncthreads();
   cx<T> spec[N2];
-  fft_step3<T, N>(lds, spec, -1);
+  fft_step3<T, N, TH>(lds, spec, -1);
   for (int j = 0; j < nout; j++) {
     cx<T> v[N2];
 #pragma unroll
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_zconv(ConvArgs<T> a) {
       int c2 = c0 + cl;
       if (lm.valid(c2)) out[lm.addr(c2, i)] = val;
     };
-    fft_from_regs<T, N>(v, lds, a.twN, +1, store);
+    fft_from_regs<T, N, TH>(v, lds, a.twN, +1, store);
   }
 }
 

@@ -516,6 +516,11 @@ int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields);
  * 0 for separate passes; with FP32 (fp32_fields = 1) or FP64 fields. */
 int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields);
 
+/* Benchmark hook (no reference counterpart): threads per work-group of the
+ * z convolutions of poten(), 256 (16 lines of the half spectrum per group)
+ * or 512 (32 lines), as the autotuner or EFFPOT_ZCONV_TH left it. */
+int effpot_zconv_threads(effpot_ctx* ctx);
+
 /* Continuous batching: advance every slot whose phase is START, RK or PC by
  * one step of its own phase (a START slot gets poten + its first RK step).
  * Energies are FP64 in every mode; a slot is DONE at the first energy check
@@ -537,8 +542,9 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
  * then times poten under each forward-transform schedule that applies
  * (effpot_plane_fused: 0, 1, 3; the environment variable EFFPOT_PLANE fixes
  * it instead) and keeps the fastest, and the same for the inverse schedule
- * (effpot_plane_inverse_fused; EFFPOT_IPLANE).  Overwrites the integrator state like
- * effpot_kernel_time: call it before uploading states. */
+ * (effpot_plane_inverse_fused; EFFPOT_IPLANE) and the z-convolution
+ * group size (effpot_zconv_threads; EFFPOT_ZCONV_TH).  Overwrites the
+ * integrator state like effpot_kernel_time: call it before uploading states. */
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
 /* Sets the stencil tile explicitly (x width, z chunk). */
 int effpot_set_tiles(effpot_ctx* ctx, int tx, int zc);

@@ -47,14 +47,21 @@ FIELDS32 = ["pot4", "dencg", "delj4", "sto1", "denalf", "ualphas", "dxden", "dyd
             "intyalf", "intzalf"]
 
 
-@pytest.mark.parametrize("n,h,natoms,zr", CASES)
-def test_fp32_fields_match_oracle(n, h, natoms, zr, gpu_available):
-    print(f"[fp32 fields] {n}^3 h={h} N={natoms}", flush=True)
+# every case with the 256-thread z convolution, the smallest and the largest
+# also with the 512-thread one (32 lines per group)
+ZCASES = [c + ("256",) for c in CASES] + [CASES[0] + ("512",), CASES[2] + ("512",)]
+
+
+@pytest.mark.parametrize("n,h,natoms,zr,zth", ZCASES)
+def test_fp32_fields_match_oracle(n, h, natoms, zr, zth, gpu_available, monkeypatch):
+    monkeypatch.setenv("EFFPOT_ZCONV_TH", zth)
+    print(f"[fp32 fields] {n}^3 h={h} N={natoms} z convolution {zth} threads", flush=True)
     xmax = n * h / 2
     R = 2.22 * natoms ** (1 / 3)
     imp = make_morse(rimp=(0.0, 0.0, zr * R))
     p = make_params(n, xmax=xmax, n4=natoms, core=CORE_OTC, h_core=2.377, precision=PREC_MIXED, psi_complex=0)
     with ep.Context(p) as c:
+        assert c.zconv_threads() == int(zth)
         c.init_fermi(0.0218, R, 0.6, 1e-6, 515 + n)
         psi = c.psi()
         c.set_impurity(imp)

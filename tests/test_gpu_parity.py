@@ -53,9 +53,10 @@ def _setup(n, xmax, n4, core, h_core, icon=ICON_ZERO, rimp=(0.0, 0.0, 0.0), batc
     return p, R, imp
 
 
-@pytest.mark.parametrize("core", [CORE_OTC, CORE_OT])
-def test_poten_energy(core, gpu_available):
-    print(f"[poten] core={core}", flush=True)
+@pytest.mark.parametrize("core,zth", [(CORE_OTC, "256"), (CORE_OT, "256"), (CORE_OTC, "512")])
+def test_poten_energy(core, zth, gpu_available, monkeypatch):
+    monkeypatch.setenv("EFFPOT_ZCONV_TH", zth)
+    print(f"[poten] core={core} z convolution {zth} threads", flush=True)
     p, R, imp = _setup(32, 12.0, 100.0, core, 2.377, rimp=(0.0, 0.0, 1.5))
     o = Oracle(p)
     o.init_fermi(0.0218, R, 0.6, 1e-6, 20261203)
