@@ -224,7 +224,7 @@ program effpot_driver
 
   if (niter > 0) then
     opt%dt = deltat; opt%tol = tol; opt%n_rk = 3; opt%pener = pener; opt%max_steps = niter
-    opt%polish_steps = niter; opt%tol_fp32 = 2.d-8; opt%check_finite = 1; opt%graph_steps = 0
+    opt%polish_steps = niter; opt%tol_fp32 = 2.d-8; opt%check_finite = 1; opt%graph_steps = 24
     call check(ctx, effpot_solve(ctx, opt, res), 'effpot_solve')
     e = res%e
     write(6, '(" Relaxation: steps",i8,"  converged",i2,"  |dE/E| per step",1p,e12.4)') res%steps, &
