@@ -13,7 +13,9 @@
  *     steppc.f90:53 / steprk.f90:304-305 with timec=1 and the -i*uimp term
  *     made real (timec is never initialised by the reference, dimen.f90:145);
  *   - psi is renormalised to n4 after every full step (the reference's
- *     integrators have no renormalisation);
+ *     integrators have no renormalisation), and the mean field inside an
+ *     imaginary-time step is that of the stage or predicted density scaled
+ *     to n4 atoms (poten_of_step);
  *   - den = re^2 + im^2 (the reference squares abs());
  *   - the FFT is this file's own mixed-radix (2,3,4,5) Stockham transform
  *     (the reference calls FFTW/MKL, fft.f90:41-55);
@@ -29,7 +31,9 @@
  * and the variational identity sum(field * drho) dV = d/de E(rho + e drho)
  * term by term (elj4/delj4, ecor4/correlation + sto1, ealphas/ualphas,
  * esolid/penalty) on non-uniform droplets, which pins term_alfa and the
- * correlation convolution (tests/test_oracle_variational.py).
+ * correlation convolution (tests/test_oracle_variational.py), and the
+ * dt-independence of the relaxation's trajectory in imaginary time, which
+ * pins the density scaling of poten_of_step (tests/test_oracle_relaxation.py).
  */
 #include <math.h>
 #include <stdint.h>
