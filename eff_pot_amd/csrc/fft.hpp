@@ -167,6 +167,18 @@ struct LineMap {
     return (unsigned)ci + (unsigned)co * (unsigned)s_outer + (unsigned)i * (unsigned)s_elem;
   }
   EP_DEV bool valid(int c) const { return c < nlines && c - (c / ncx) * ncx < nvalid; }
+  // A thread of the strided and z-convolution passes reads and writes one
+  // line only: its base offset (addr(c, 0)) and validity are formed once, so
+  // the runtime division by ncx is not repeated at every access.
+  struct Line {
+    unsigned base, step;
+    bool ok;
+    EP_DEV unsigned at(int i) const { return base + (unsigned)i * step; }
+  };
+  EP_DEV Line line(int c) const {
+    const bool ok = valid(c);
+    return Line{ok ? addr(c, 0) : 0u, (unsigned)s_elem, ok};
+  }
 };
 
 // ---- the two-level transform, step by step -------------------------------------
@@ -258,19 +270,17 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs<T> a) {
   const LineMap lm = a.lm;
   cx<T>* d = a.f[blockIdx.y] + (long)blockIdx.z * a.sstride;
   const int c0 = blockIdx.x * C;
-  auto load = [&](int c, int i) -> cx<T> {
-    int cc = c0 + c;
-    return lm.valid(cc) ? d[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
-  };
+  // every access of this thread (step 1 loads, step 3 stores) is on line c0 + t % C
+  const LineMap::Line ln = lm.line(c0 + (int)(threadIdx.x % C));
+  auto load = [&](int, int i) -> cx<T> { return ln.ok ? d[ln.at(i)] : cx<T>{T(0), T(0)}; };
   fft_step1<T, N>(lds, a.twN, a.sign, load);
   __syncthreads();
   cx<T> b[N2];
   fft_step3<T, N>(lds, b, a.sign);
-  const int c = threadIdx.x % C, k1 = threadIdx.x / C;
-  const int cc = c0 + c;
-  if (lm.valid(cc)) {
+  const int k1 = threadIdx.x / C;
+  if (ln.ok) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) d[lm.addr(cc, k1 + N1 * k2)] = b[k2];
+    for (int k2 = 0; k2 < N2; k2++) d[ln.at(k1 + N1 * k2)] = b[k2];
   }
 }
 
@@ -300,13 +310,11 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   const long so = (long)blockIdx.z * a.sstride;
   const cx<T>* in = a.in[g] + so;
   const int c0 = blockIdx.x * C;
-  auto load = [&](int c, int i) -> cx<T> {
-    int cc = c0 + c;
-    return lm.valid(cc) ? in[lm.addr(cc, i)] : cx<T>{T(0), T(0)};
-  };
-  const int c = threadIdx.x % C, kk1 = threadIdx.x / C;
-  const int cc = c0 + c;
-  const bool valid = lm.valid(cc);
+  // every access of this thread (loads, k-space factors, stores) is on line c0 + t % C
+  const int kk1 = threadIdx.x / C;
+  const LineMap::Line ln = lm.line(c0 + (int)(threadIdx.x % C));
+  const bool valid = ln.ok;
+  auto load = [&](int, int i) -> cx<T> { return valid ? in[ln.at(i)] : cx<T>{T(0), T(0)}; };
   const int nout = a.nout[g];
   // k-space factors of this thread's outputs: the first output's are loaded
   // before the forward transform, each next output's during the previous
@@ -315,7 +323,7 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   auto load_k = [&](int j) {
     const T* K = a.kern[g][j];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[lm.addr(cc, kk1 + N1 * k2)] : T(0);
+    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[ln.at(kk1 + N1 * k2)] : T(0);
   };
   load_k(0);
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
@@ -329,9 +337,8 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
     if (j + 1 < nout) load_k(j + 1);
     __syncthreads();  // LDS reuse
     cx<T>* out = a.out[g][j] + so;
-    auto store = [&](int cl, int i, cx<T> val) {
-      int c2 = c0 + cl;
-      if (lm.valid(c2)) out[lm.addr(c2, i)] = val;
+    auto store = [&](int, int i, cx<T> val) {
+      if (valid) out[ln.at(i)] = val;
     };
     fft_from_regs<T, N, TH>(v, lds, a.twN, +1, store);
   }
