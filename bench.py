@@ -34,6 +34,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -551,6 +553,8 @@ def main():
     ctx.sync()
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    # slot 0's density, before the roofline timings overwrite the states
+    rho0 = np.abs(ctx.psi(0)) ** 2 if (a.mode == "mixed" and not a.no_ref) else None
     nb = max(a.batches, 1)
     nsolves = B * nb
     conv = all(r.converged for r in results)
@@ -631,9 +635,12 @@ def main():
             tr0 = time.perf_counter()
             (r64,), _ = solve_batch(cr, pr, R, zs[:1], opts, seed=a.seed)
             tr = time.perf_counter() - tr0
+            rho64 = np.abs(cr.psi(0)) ** 2
         line["fp64_reference"] = {"solves_per_s": 1.0 / tr, "E_imp_K": r64.e.eimpu, "steps": r64.steps,
                                   "residual": r64.e.resid,
                                   "dE_imp_K": abs(results[0].e.eimpu - r64.e.eimpu),
+                                  # ||rho_mixed - rho_fp64||_2 / ||rho_fp64||_2 of slot 0
+                                  "L2_density": float(np.linalg.norm(rho0 - rho64) / np.linalg.norm(rho64)),
                                   "max_dE_imp_K_all_slots": max(abs(r.e.eimpu - r64.e.eimpu) for r in results),
                                   "speedup": line["value"] * tr}
     # ---- CPU baseline: the FP64 oracle on the same workload, bounded sample ---
