@@ -394,12 +394,16 @@ def test_multi_step_graph_matches_single_steps(precision, batch, gpu_available):
     kernels in the same order, so every slot's energies, step count and final
     wavefunction equal the one-step-per-tick run bit for bit.  7 does not
     divide pener = 25, and one slot is admitted 4 ticks late, so the
-    single-step fallback runs too."""
-    print(f"[graph] {batch} slots precision={precision}: 7-step graphs vs single steps", flush=True)
+    single-step fallback runs too.  The multi-step run alternates 7- and
+    5-step graphs tick by tick, so the graph is re-captured (after the
+    stream drains) while slots are in flight."""
+    print(f"[graph] {batch} slots precision={precision}: 7/5-step graphs vs single steps", flush=True)
     out = {}
     for gs in (0, 7):
-        opts = SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=25, max_steps=160, polish_steps=60, tol_fp32=0.0,
-                         check_finite=0, graph_steps=gs)
+        def opts_of(tick):
+            k = gs if gs == 0 or tick % 2 == 0 else 5
+            return SolveOpts(dt=1.9e-4, tol=0.0, n_rk=3, pener=25, max_steps=160, polish_steps=60, tol_fp32=0.0,
+                             check_finite=0, graph_steps=k)
         p = make_params(32, xmax=12.0, n4=100.0, core=CORE_OTC, h_core=2.377, batch=batch, precision=precision,
                         psi_complex=0)
         with ep.Context(p) as ctx:
@@ -414,7 +418,7 @@ def test_multi_step_graph_matches_single_steps(precision, batch, gpu_available):
                         slots[s].fp32 = 1 if precision else 0
                 if all(slots[s].phase in (ep.SLOT_DONE, ep.SLOT_FAILED) for s in range(batch)):
                     break
-                ctx.tick(opts, slots)
+                ctx.tick(opts_of(ticks), slots)
                 ticks += 1
             out[gs] = ([(slots[s].steps, slots[s].e.etot, slots[s].e.eimpu, slots[s].rel_de) for s in range(batch)],
                        [ctx.psi(s) for s in range(batch)], ticks)
