@@ -185,6 +185,7 @@ struct effpot_ctx {
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
   int iplane = 1;                 // inverse y and x passes fused per z plane where it fits (EFFPOT_IPLANE)
+  bool packed = true;             // packed half-spectrum layout inside poten when both plane kernels run (EFFPOT_PACKED)
   int zconv_th = 256;             // threads per z-convolution group: 256 or 512 (EFFPOT_ZCONV_TH)
   bool zconv_fixed = false;       // EFFPOT_ZCONV_TH given: the autotuner keeps it
   bool iplane_fixed = false;      // EFFPOT_IPLANE given: the autotuner keeps it
@@ -431,10 +432,12 @@ PlaneArgs<T> plane_args(effpot_ctx* ctx, State<T>& s, T* const* r, cx<T>* const*
   a.twx = s.tw[0]; a.twy = s.tw[1]; a.active = act;
   return a;
 }
-// real fields -> half spectra transformed along x and y (one launch per plane)
+// real fields -> half spectra transformed along x and y (one launch per plane);
+// packed: rows of nx/2 with W = S(0) + i S(nx/2) in column 0 (in-place kernel only)
 template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf,
-                                        const int* act) {
+                                        const int* act, bool packed = false) {
   PlaneArgs<T> a = plane_args(ctx, s, in, out, nf, act);
+  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; }
   dim3 g(ctx->nz, nf, ctx->B);
   const int mode = plane_mode<T>(ctx);
   if (mode == 2) {
@@ -472,8 +475,10 @@ template <typename T> bool iplane_ok(effpot_ctx* ctx) {
 // half spectra (after the z transform) -> real fields, inverse y and x in one
 // launch per plane: out = aff0 + scale*aff1*x
 template <typename T> int fft_plane_inv(effpot_ctx* ctx, State<T>& s, cx<T>* const* in, T* const* out, int nf, T scale,
-                                        const int* act, const T* aff0 = nullptr, const T* aff1 = nullptr) {
+                                        const int* act, const T* aff0 = nullptr, const T* aff1 = nullptr,
+                                        bool packed = false) {
   PlaneArgs<T> a = plane_args(ctx, s, out, in, nf, act);
+  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; }
   a.scale = scale;
   for (int f = 0; f < nf; f++) {
     a.aff0[f] = aff0 ? aff0[f] : T(0);
@@ -508,12 +513,29 @@ template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx
   LAUNCHCHK();
   return EFFPOT_OK;
 }
-template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act) {
-  a.sstride = ctx->M; a.lm = zmap(ctx); a.twN = s.tw[2]; a.active = act;
+// The packed layout (both in-place plane kernels in poten): rows of nx/2,
+// column 0 holds W = S(0) + i S(nx/2).  Where it applies (plane schedules 2-4
+// with the in-place inverse) the z convolutions see 64 aligned lines per row at
+// 128^3 instead of 65 in rows padded to 80, and the inverse plane reads whole
+// 512-byte rows.
+template <typename T> bool packed_ok(effpot_ctx* ctx) {
+  return ctx->packed && plane_mode<T>(ctx) >= 2 && iplane_ok<T>(ctx);
+}
+template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act,
+                                    bool packed = false) {
+  a.sstride = ctx->M; a.lm = zmap(ctx); a.km = a.lm; a.twN = s.tw[2]; a.active = act;
   a.scale = (T)(1.0 / ((double)ctx->nx * ctx->ny * ctx->nz));
+  const int P = ctx->nxh - 1;
+  a.nw = 0;
+  if (packed) {  // lines kx' = 1 .. P-1 of rows of P; the factor tables keep their padded rows
+    a.lm = LineMap{P, (long)P, (long)P * ctx->ny, P * ctx->ny, P, 1};
+    a.km = LineMap{P, (long)ctx->nxp, (long)ctx->nxp * ctx->ny, P * ctx->ny, P, 1};
+    a.P = P; a.ny = ctx->ny; a.kpitch = ctx->nxp;
+  }
   const int th = ctx->zconv_th;
   const int C = th / n1_of(ctx->nz);
-  dim3 g((a.lm.nlines + C - 1) / C, ngroups, ctx->B);
+  if (packed) a.nw = (ctx->ny + C - 1) / C;  // groups of the packed column, launched first
+  dim3 g(a.nw + (a.lm.nlines + C - 1) / C, ngroups, ctx->B);
   if (th == 512) {
 #define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, 512>), g, dim3(512), 0, ctx->st, a)
     FFT_SWITCH(ctx->nz, M_)
@@ -655,12 +677,13 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
   const bool pl = plane_ok<T>(ctx);  // forward x and y passes fused: launches 1, 7, 13 are empty
   const bool ipl = iplane_ok<T>(ctx);  // inverse y and x passes fused: launches 3, 9, 15 are empty
+  const bool pk = packed_ok<T>(ctx);   // packed half spectra between the plane kernels (implies pl and ipl)
   auto sel = [only](int k) { return only < 0 || only == k; };
   // stage A: FFT(den) -> dencg, delj4 [, denalf]
   {
     T* in[1] = {s.den};
     cx<T>* sp[1] = {s.spec[0]};
-    if (sel(0)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act) : fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(0)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act, pk) : fft_x_r2c(ctx, s, in, sp, 1, act));
     if (sel(1) && !pl) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[0];
@@ -668,7 +691,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     ca.out[0][0] = s.spec[1]; ca.kern[0][0] = s.kern[0];
     ca.out[0][1] = s.spec[2]; ca.kern[0][1] = s.kern[1];
     ca.out[0][2] = s.spec[3]; ca.kern[0][2] = s.kern[2];
-    if (sel(2)) RC(fft_zconv(ctx, s, ca, 1, act));
+    if (sel(2)) RC(fft_zconv(ctx, s, ca, 1, act, pk));
     cx<T>* o[3] = {s.spec[1], s.spec[2], s.spec[3]};
     if (sel(3) && !ipl) RC(fft_strided(ctx, s, 1, o, ca.nout[0], +1, act));
     // the alpha_s convolution is stored as falfs = 1 - denalf/den0s (term_alfa.f90:157-159),
@@ -677,7 +700,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     const T a0[3] = {T(0), T(0), T(1)};
     const T a1[3] = {T(1), T(1), T(-1.0 / ctx->p.den0s)};
     if (sel(4))
-      RC(ipl ? fft_plane_inv(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1)
+      RC(ipl ? fft_plane_inv(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1, pk)
              : fft_x_c2r(ctx, s, o, r, ca.nout[0], T(1), act, a0, a1));
   }
   // derden + FFT sources
@@ -698,7 +721,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     const int nf = otc ? 4 : 1;
     T* in[4] = {s.src[0], s.src[1], s.src[2], s.src[3]};
     cx<T>* sp[4] = {s.spec[4], s.spec[5], s.spec[6], s.spec[7]};
-    if (sel(6)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, nf, act) : fft_x_r2c(ctx, s, in, sp, nf, act));
+    if (sel(6)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, nf, act, pk) : fft_x_r2c(ctx, s, in, sp, nf, act));
     if (sel(7) && !pl) RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
     ConvArgs<T> ca{};
     for (int g = 0; g < nf; g++) {
@@ -707,11 +730,12 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
       ca.out[g][0] = s.spec[g];
       ca.kern[g][0] = g == 0 ? s.kern[0] : s.kern[2];
     }
-    if (sel(8)) RC(fft_zconv(ctx, s, ca, nf, act));
+    if (sel(8)) RC(fft_zconv(ctx, s, ca, nf, act, pk));
     cx<T>* o[4] = {s.spec[0], s.spec[1], s.spec[2], s.spec[3]};
     if (sel(9) && !ipl) RC(fft_strided(ctx, s, 1, o, nf, +1, act));
     T* r[4] = {s.sto1, s.intx, s.inty, s.intz};
-    if (sel(10)) RC(ipl ? fft_plane_inv(ctx, s, o, r, nf, T(1), act) : fft_x_c2r(ctx, s, o, r, nf, T(1), act));
+    if (sel(10))
+      RC(ipl ? fft_plane_inv(ctx, s, o, r, nf, T(1), act, (const T*)nullptr, (const T*)nullptr, pk) : fft_x_c2r(ctx, s, o, r, nf, T(1), act));
   }
   if (otc) {
     DivArgs<T> a{};
@@ -728,15 +752,16 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     // stage C: FFT(sum_i d_i den int_i) * kalfs
     T* in[1] = {s.src[0]};
     cx<T>* sp[1] = {s.spec[4]};
-    if (sel(12)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act) : fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(12)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act, pk) : fft_x_r2c(ctx, s, in, sp, 1, act));
     if (sel(13) && !pl) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[4]; ca.nout[0] = 1; ca.out[0][0] = s.spec[0]; ca.kern[0][0] = s.kern[2];
-    if (sel(14)) RC(fft_zconv(ctx, s, ca, 1, act));
+    if (sel(14)) RC(fft_zconv(ctx, s, ca, 1, act, pk));
     cx<T>* o[1] = {s.spec[0]};
     if (sel(15) && !ipl) RC(fft_strided(ctx, s, 1, o, 1, +1, act));
     T* r[1] = {s.ua};
-    if (sel(16)) RC(ipl ? fft_plane_inv(ctx, s, o, r, 1, T(1), act) : fft_x_c2r(ctx, s, o, r, 1, T(1), act));
+    if (sel(16))
+      RC(ipl ? fft_plane_inv(ctx, s, o, r, 1, T(1), act, (const T*)nullptr, (const T*)nullptr, pk) : fft_x_c2r(ctx, s, o, r, 1, T(1), act));
   }
   if (sel(17)) {
     Pot4Args<T> a{};
@@ -1079,6 +1104,7 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_eso.assign(ctx->B, 0.0);
   if (const char* e = std::getenv("EFFPOT_PLANE")) { ctx->plane = std::atoi(e); ctx->plane_fixed = true; }
   if (const char* e = std::getenv("EFFPOT_IPLANE")) { ctx->iplane = std::atoi(e) != 0; ctx->iplane_fixed = true; }
+  if (const char* e = std::getenv("EFFPOT_PACKED")) ctx->packed = std::atoi(e) != 0;
   if (const char* e = std::getenv("EFFPOT_ZCONV_TH")) {
     ctx->zconv_th = std::atoi(e) == 512 ? 512 : 256;
     ctx->zconv_fixed = true;

@@ -158,6 +158,7 @@ struct LineMap {
   long s_elem;      // stride between elements of a line
   int nlines;       // lines per field and slot (padding included)
   int nvalid;       // lines of a run that exist (nx/2+1)
+  int first = 0;    // first line of a run that is processed (1: the packed column 0 is skipped)
   // 32-bit element offset from the field's base (a field of one slot holds
   // fewer than 2^32 elements): with a uniform base the loads and stores take
   // the SGPR-base + 32-bit VGPR-offset form, which frees the per-lane 64-bit
@@ -166,7 +167,10 @@ struct LineMap {
     int co = c / ncx, ci = c - co * ncx;
     return (unsigned)ci + (unsigned)co * (unsigned)s_outer + (unsigned)i * (unsigned)s_elem;
   }
-  EP_DEV bool valid(int c) const { return c < nlines && c - (c / ncx) * ncx < nvalid; }
+  EP_DEV bool valid(int c) const {
+    const int ci = c - (c / ncx) * ncx;
+    return c < nlines && ci >= first && ci < nvalid;
+  }
   // A thread of the strided and z-convolution passes reads and writes one
   // line only: its base offset (addr(c, 0)) and validity are formed once, so
   // the runtime division by ncx is not repeated at every access.
@@ -294,25 +298,98 @@ template <typename T> struct ConvArgs {
   int nout[4];
   long sstride;
   LineMap lm;
+  LineMap km;  // the same lines in the k-space factor tables (padded rows of nx/2+1)
+  int nw;      // packed layout: the first nw groups (blockIdx.x) convolve the packed column 0
+  int P, ny;   // packed row length (nx/2), rows per plane
+  int kpitch;  // row length of the factor tables
   T scale;
   const cx<T>* twN;
   const int* active;
 };
+// ---- z convolution of the packed column (packed layout) --------------------
+// In the packed layout column 0 of a half-spectrum row holds W = S(0) + i S(H)
+// (H = nx/2): the y-transformed columns kx = 0 and kx = H, both spectra of real
+// columns, hence Hermitian in (ky, kz).  After the z transform the packed value
+// at k = (ky, kz) is W(k) = S0(k) + i SH(k) with S0(k) = (W(k) + conj W(-k))/2
+// and i SH(k) = (W(k) - conj W(-k))/2, so the products K0 S0 + i KH SH with the
+// (real, even) k-space factors of columns 0 and H are
+//   O(k) = (K0 + KH)/2 W(k) + (K0 - KH)/2 conj W(-k),   -k = (-ky, -kz),
+// the packed form of the two convolved columns.  Group gx transforms C of the
+// ny lines (kx = 0, ky) with their partners (kx = 0, -ky) in the same group:
+// positions 0 and 1 hold ky = 0 and ny/2 (their own partners), then ky = m and
+// ny - m at positions 2m and 2m + 1.  The partner's spectrum comes through LDS
+// (rows of N + 1: no bank conflicts).
+template <typename T, int N, int TH>
+EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
+  constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
+  const int g = blockIdx.y;
+  const long so = (long)blockIdx.z * a.sstride;
+  const cx<T>* in = a.in[g] + so;
+  const int c = threadIdx.x % C, kk1 = threadIdx.x / C;
+  const int ny = a.ny, hy = ny / 2;
+  const int p = gx * C + c;
+  const bool valid = p < ny;
+  const int ky = p == 0 ? 0 : p == 1 ? hy : ((p & 1) ? ny - (p >> 1) : (p >> 1));
+  const int pc = p <= 1 ? c : (c ^ 1);  // the partner's lane in this group
+  const long srow = (long)a.P * ny, krow = (long)a.kpitch * ny;
+  const long sb = (long)ky * a.P, kb = (long)ky * a.kpitch;
+  auto load = [&](int, int i) -> cx<T> { return valid ? in[sb + (long)i * srow] : cx<T>{T(0), T(0)}; };
+  fft_step1<T, N, TH>(lds, a.twN, -1, load);
+  __syncthreads();
+  cx<T> spec[N2];
+  fft_step3<T, N, TH>(lds, spec, -1);  // spec[k2] = W(ky, kz = kk1 + N1 k2)
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < N2; k2++) lds[c * (N + 1) + kk1 + N1 * k2] = spec[k2];
+  __syncthreads();
+  cx<T> part[N2];  // conj W(-ky, -kz)
+#pragma unroll
+  for (int k2 = 0; k2 < N2; k2++) {
+    const int kz = kk1 + N1 * k2, km = kz == 0 ? 0 : N - kz;
+    part[k2] = conjg(lds[pc * (N + 1) + km]);
+  }
+  const int nout = a.nout[g];
+  for (int j = 0; j < nout; j++) {
+    const T* K = a.kern[g][j];
+    cx<T> v[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      const long kk = kb + (long)(kk1 + N1 * k2) * krow;
+      const T k0 = valid ? K[kk] : T(0), kh = valid ? K[kk + a.P] : T(0);
+      const T ap = T(0.5) * (k0 + kh) * a.scale, am = T(0.5) * (k0 - kh) * a.scale;
+      v[k2] = cx<T>{ap * spec[k2].x + am * part[k2].x, ap * spec[k2].y + am * part[k2].y};
+    }
+    __syncthreads();  // LDS reuse
+    cx<T>* out = a.out[g][j] + so;
+    auto store = [&](int, int i, cx<T> val) {
+      if (valid) out[sb + (long)i * srow] = val;
+    };
+    fft_from_regs<T, N, TH>(v, lds, a.twN, +1, store);
+  }
+}
+
 // TH = 256 or 512 threads: C = 16 or 32 lines per group (128 B or 256 B runs
-// of FP32 complex per z plane); the autotuner picks one per context.
+// of FP32 complex per z plane); the autotuner picks one per context.  In the
+// packed layout the first nw groups (blockIdx.x) convolve the packed column,
+// so that small piece of work runs alongside the other lines.
 template <typename T, int N, int TH = kThreads>
 __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   if (!a.active[blockIdx.z]) return;
-  __shared__ cx<T> lds[N * C];
+  __shared__ cx<T> lds[(N + 1) * C];
+  if ((int)blockIdx.x < a.nw) {
+    zconv_packed_column<T, N, TH>(a, lds, blockIdx.x);
+    return;
+  }
   const int g = blockIdx.y;
   const LineMap lm = a.lm;
   const long so = (long)blockIdx.z * a.sstride;
   const cx<T>* in = a.in[g] + so;
-  const int c0 = blockIdx.x * C;
+  const int c0 = (blockIdx.x - a.nw) * C;
   // every access of this thread (loads, k-space factors, stores) is on line c0 + t % C
   const int kk1 = threadIdx.x / C;
   const LineMap::Line ln = lm.line(c0 + (int)(threadIdx.x % C));
+  const LineMap::Line lk = a.km.line(c0 + (int)(threadIdx.x % C));
   const bool valid = ln.ok;
   auto load = [&](int, int i) -> cx<T> { return valid ? in[ln.at(i)] : cx<T>{T(0), T(0)}; };
   const int nout = a.nout[g];
@@ -323,7 +400,7 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   auto load_k = [&](int j) {
     const T* K = a.kern[g][j];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[ln.at(kk1 + N1 * k2)] : T(0);
+    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[lk.at(kk1 + N1 * k2)] : T(0);
   };
   load_k(0);
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
@@ -352,7 +429,8 @@ template <typename T> struct XArgs {
   cx<T>* c[4];
   long rslot, cslot;
   int nrows;  // ny*nz (even)
-  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
+  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded, or nx/2 packed)
+  int packed; // column 0 holds W = S(kx=0) + i S(kx=nx/2); no column nx/2 (the packed layout)
   T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
   T aff0[4], aff1[4];
   const cx<T>* twN;
@@ -514,7 +592,8 @@ template <typename T> struct PlaneArgs {
   T* r[4];
   cx<T>* c[4];
   long rslot, cslot;
-  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
+  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded, or nx/2 packed)
+  int packed; // column 0 holds W = S(kx=0) + i S(kx=nx/2); no column nx/2 (the packed layout)
   const cx<T>* twx;
   const cx<T>* twy;
   const int* active;
@@ -800,12 +879,12 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           b[m2] = plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)];
         }
         RegFFT<T, N2>::run(b, -1);
-        if (kk > 0) {
+        if (kk > 0 || a.packed) {  // packed layout: column 0 keeps W as it is
 #pragma unroll
           for (int k2 = 0; k2 < N2; k2++) dst[kk + (long)(k1 + N1 * k2) * a.pitch] = b[k2];
         }
       }
-      if (j == 0) {  // column 0, the packed W, lives in round 0
+      if (j == 0 && !a.packed) {  // column 0, the packed W, lives in round 0
         __syncthreads();
         if (on && kk == 0) {  // W[ky] into its own slots
 #pragma unroll
@@ -883,8 +962,11 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
       cx<T> w[4];
       const int km = (N - tt) % N;
       if (tt < N) {
-        w[0] = src[(long)tt * a.pitch]; w[1] = src[(long)km * a.pitch];
-        w[2] = src[(long)tt * a.pitch + H]; w[3] = src[(long)km * a.pitch + H];
+        w[0] = src[(long)tt * a.pitch];
+        if (!a.packed) {
+          w[1] = src[(long)km * a.pitch];
+          w[2] = src[(long)tt * a.pitch + H]; w[3] = src[(long)km * a.pitch + H];
+        }
       }
       cx<T> v[NL];
 #pragma unroll
@@ -904,9 +986,13 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
         }
       }
       if (tt < N) {
-        const cx<T> s0h{T(0.5) * (w[0].x + w[1].x), T(0.5) * (w[0].y - w[1].y)};
-        const cx<T> shh{T(0.5) * (w[2].x + w[3].x), T(0.5) * (w[2].y - w[3].y)};
-        plane[pos(0, tt)] = cx<T>{s0h.x - shh.y, s0h.y + shh.x};
+        if (a.packed) {  // W = S0 + i SH as the packed z convolution left it
+          plane[pos(0, tt)] = w[0];
+        } else {
+          const cx<T> s0h{T(0.5) * (w[0].x + w[1].x), T(0.5) * (w[0].y - w[1].y)};
+          const cx<T> shh{T(0.5) * (w[2].x + w[3].x), T(0.5) * (w[2].y - w[3].y)};
+          plane[pos(0, tt)] = cx<T>{s0h.x - shh.y, s0h.y + shh.x};
+        }
       }
     }
     __syncthreads();
