@@ -432,12 +432,17 @@ PlaneArgs<T> plane_args(effpot_ctx* ctx, State<T>& s, T* const* r, cx<T>* const*
   a.twx = s.tw[0]; a.twy = s.tw[1]; a.active = act;
   return a;
 }
+// The packed layout (poten with both in-place plane kernels): rows of nx/2
+// (column 0 unused), then W = S(kx=0) + i S(kx=nx/2) for every (ky, z), ky
+// fastest, at this offset of the slot's spectrum buffer: rows of nx/2 leave at
+// least ny nz of the nxp ny nz elements free, since nxp >= nx/2 + 1.
+long packed_wofs(effpot_ctx* ctx) { return (long)(ctx->nxh - 1) * ctx->ny * ctx->nz; }
 // real fields -> half spectra transformed along x and y (one launch per plane);
 // packed: rows of nx/2 with W = S(0) + i S(nx/2) in column 0 (in-place kernel only)
 template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf,
                                         const int* act, bool packed = false) {
   PlaneArgs<T> a = plane_args(ctx, s, in, out, nf, act);
-  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; }
+  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; a.wofs = packed_wofs(ctx); }
   dim3 g(ctx->nz, nf, ctx->B);
   const int mode = plane_mode<T>(ctx);
   if (mode == 2) {
@@ -478,7 +483,7 @@ template <typename T> int fft_plane_inv(effpot_ctx* ctx, State<T>& s, cx<T>* con
                                         const int* act, const T* aff0 = nullptr, const T* aff1 = nullptr,
                                         bool packed = false) {
   PlaneArgs<T> a = plane_args(ctx, s, out, in, nf, act);
-  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; }
+  if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; a.wofs = packed_wofs(ctx); }
   a.scale = scale;
   for (int f = 0; f < nf; f++) {
     a.aff0[f] = aff0 ? aff0[f] : T(0);
@@ -530,7 +535,7 @@ template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a
   if (packed) {  // lines kx' = 1 .. P-1 of rows of P; the factor tables keep their padded rows
     a.lm = LineMap{P, (long)P, (long)P * ctx->ny, P * ctx->ny, P, 1};
     a.km = LineMap{P, (long)ctx->nxp, (long)ctx->nxp * ctx->ny, P * ctx->ny, P, 1};
-    a.P = P; a.ny = ctx->ny; a.kpitch = ctx->nxp;
+    a.P = P; a.ny = ctx->ny; a.kpitch = ctx->nxp; a.wofs = packed_wofs(ctx);
   }
   const int th = ctx->zconv_th;
   const int C = th / n1_of(ctx->nz);

@@ -299,16 +299,18 @@ template <typename T> struct ConvArgs {
   long sstride;
   LineMap lm;
   LineMap km;  // the same lines in the k-space factor tables (padded rows of nx/2+1)
-  int nw;      // packed layout: the first nw groups (blockIdx.x) convolve the packed column 0
+  int nw;      // packed layout: the first nw groups (blockIdx.x) convolve the packed column W
   int P, ny;   // packed row length (nx/2), rows per plane
+  long wofs;   // W(ky, kz) at the slot's spectrum + wofs + kz ny + ky
   int kpitch;  // row length of the factor tables
   T scale;
   const cx<T>* twN;
   const int* active;
 };
 // ---- z convolution of the packed column (packed layout) --------------------
-// In the packed layout column 0 of a half-spectrum row holds W = S(0) + i S(H)
-// (H = nx/2): the y-transformed columns kx = 0 and kx = H, both spectra of real
+// In the packed layout the rows hold kx = 1 .. H-1 (H = nx/2; their column 0
+// is unused) and a block of its own holds W = S(0) + i S(H) for every (ky, z),
+// ky fastest: the y-transformed columns kx = 0 and kx = H, both spectra of real
 // columns, hence Hermitian in (ky, kz).  After the z transform the packed value
 // at k = (ky, kz) is W(k) = S0(k) + i SH(k) with S0(k) = (W(k) + conj W(-k))/2
 // and i SH(k) = (W(k) - conj W(-k))/2, so the products K0 S0 + i KH SH with the
@@ -331,8 +333,8 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
   const bool valid = p < ny;
   const int ky = p == 0 ? 0 : p == 1 ? hy : ((p & 1) ? ny - (p >> 1) : (p >> 1));
   const int pc = p <= 1 ? c : (c ^ 1);  // the partner's lane in this group
-  const long srow = (long)a.P * ny, krow = (long)a.kpitch * ny;
-  const long sb = (long)ky * a.P, kb = (long)ky * a.kpitch;
+  const long srow = ny, krow = (long)a.kpitch * ny;  // W is stored kz-major, ky fastest
+  const long sb = a.wofs + ky, kb = (long)ky * a.kpitch;
   auto load = [&](int, int i) -> cx<T> { return valid ? in[sb + (long)i * srow] : cx<T>{T(0), T(0)}; };
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
   __syncthreads();
@@ -429,8 +431,7 @@ template <typename T> struct XArgs {
   cx<T>* c[4];
   long rslot, cslot;
   int nrows;  // ny*nz (even)
-  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded, or nx/2 packed)
-  int packed; // column 0 holds W = S(kx=0) + i S(kx=nx/2); no column nx/2 (the packed layout)
+  int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
   T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
   T aff0[4], aff1[4];
   const cx<T>* twN;
@@ -593,7 +594,8 @@ template <typename T> struct PlaneArgs {
   cx<T>* c[4];
   long rslot, cslot;
   int pitch;  // complex elements per half-spectrum row (nx/2+1 padded, or nx/2 packed)
-  int packed; // column 0 holds W = S(kx=0) + i S(kx=nx/2); no column nx/2 (the packed layout)
+  int packed; // packed layout: rows of nx/2 (column 0 unused) and W = S(kx=0) + i S(kx=nx/2)
+  long wofs;  // packed: W of plane z, row ky at the slot's spectrum + wofs + z ny + ky
   const cx<T>* twx;
   const cx<T>* twy;
   const int* active;
@@ -759,6 +761,7 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
     const int f = blockIdx.y, iz = blockIdx.x;
     const T* src = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
     cx<T>* dst = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    cx<T>* wdst = a.c[f] + (long)blockIdx.z * a.cslot + a.wofs + (long)iz * N;  // packed: W of this plane
     // 1. pairs of real rows, packed: a lane takes element i of a row pair and
     //    loads it from both rows (each load coalesced along the row), every
     //    load of the thread issued before its first LDS store; consecutive
@@ -879,9 +882,12 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
           b[m2] = plane[(m < H ? ra : rb) * Q + (m < H ? m : m - H)];
         }
         RegFFT<T, N2>::run(b, -1);
-        if (kk > 0 || a.packed) {  // packed layout: column 0 keeps W as it is
+        if (kk > 0) {
 #pragma unroll
           for (int k2 = 0; k2 < N2; k2++) dst[kk + (long)(k1 + N1 * k2) * a.pitch] = b[k2];
+        } else if (a.packed) {  // packed layout: W kept as it is, in its own block
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) wdst[k1 + N1 * k2] = b[k2];
         }
       }
       if (j == 0 && !a.packed) {  // column 0, the packed W, lives in round 0
@@ -946,6 +952,7 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
   if constexpr (fits) {
     const int f = blockIdx.y, iz = blockIdx.x;
     const cx<T>* src = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    const cx<T>* wsrc = a.c[f] + (long)blockIdx.z * a.cslot + a.wofs + (long)iz * N;  // packed: W of this plane
     T* dst = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
     const int tt = threadIdx.x;
     // element m of column kk: plane row (m < H ? kk : kk == 0 ? H : N - kk), slot m mod H
@@ -962,7 +969,7 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
       cx<T> w[4];
       const int km = (N - tt) % N;
       if (tt < N) {
-        w[0] = src[(long)tt * a.pitch];
+        w[0] = a.packed ? wsrc[tt] : src[(long)tt * a.pitch];
         if (!a.packed) {
           w[1] = src[(long)km * a.pitch];
           w[2] = src[(long)tt * a.pitch + H]; w[3] = src[(long)km * a.pitch + H];
