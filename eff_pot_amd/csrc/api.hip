@@ -518,8 +518,9 @@ template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx
   LAUNCHCHK();
   return EFFPOT_OK;
 }
-// The packed layout (both in-place plane kernels in poten): rows of nx/2,
-// column 0 holds W = S(0) + i S(nx/2).  Where it applies (plane schedules 2-4
+// The packed layout (both in-place plane kernels in poten): rows of nx/2
+// holding kx = 1 .. nx/2-1, and W = S(0) + i S(nx/2) in a block of its own
+// (packed_wofs).  Where it applies (plane schedules 2-4
 // with the in-place inverse) the z convolutions see 64 aligned lines per row at
 // 128^3 instead of 65 in rows padded to 80, and the inverse plane reads whole
 // 512-byte rows.
