@@ -300,14 +300,14 @@ def pc_step_bytes(n, mode, B, plane, iplane=0):
     integrator history FP64: H psi + predictor/modifier reads psi, o3, h1, h2,
     pc and pot4, uimp and writes o3, pc, h2 and the density (5*8 + 2s + 3*8 +
     s); H psi + corrector reads psi, o3, o2, h1, h2, pc, pot4, uimp and writes
-    pc and psi (6*8 + 2s + 2*8); the scaling of the predicted density to n4
-    atoms reads and writes it (2s); the renormalisation reads and writes psi
+    pc and psi (6*8 + 2s + 2*8); the renormalisation reads and writes psi
     and, for FP64 fields, writes the density (2*8 [+ s]); poten is the sum of
-    its launches (poten_launch_bytes)."""
+    its launches (poten_launch_bytes; the predicted density is read through
+    its factor to n4 atoms, no pass of its own)."""
     s = 4 if mode == "mixed" else 8
     N = n ** 3
     # the renormalisation stores the density only for FP64 fields
-    psi_side = (40 + 2 * s + 24 + s) + 2 * s + (48 + 2 * s + 16) + (16 + (0 if mode == "mixed" else s))
+    psi_side = (40 + 2 * s + 24 + s) + (48 + 2 * s + 16) + (16 + (0 if mode == "mixed" else s))
     poten = sum(poten_launch_bytes(k, n, mode, 1, plane, iplane) for k in range(len(POTEN_LAUNCHES)))
     return B * (psi_side * N + poten)
 

@@ -367,10 +367,13 @@ int n1_of(int n) {
   switch (n) { case 16: return 4; case 24: case 32: case 48: case 64: return 8; default: return 16; }
 }
 
-template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf, const int* act) {
+// rscale: [slot] factor the first field is read with (or null)
+template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf, const int* act,
+                                    const double* rscale = nullptr) {
   XArgs<T> a{};
   for (int f = 0; f < nf; f++) { a.r[f] = in[f]; a.c[f] = out[f]; }
   a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = T(1); a.twN = s.tw[0]; a.active = act;
+  a.rscale = rscale;
   a.pitch = ctx->nxp;
   const int C = kThreads / n1_of(ctx->nx);
   dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
@@ -440,8 +443,9 @@ long packed_wofs(effpot_ctx* ctx) { return (long)(ctx->nxh - 1) * ctx->ny * ctx-
 // real fields -> half spectra transformed along x and y (one launch per plane);
 // packed: rows of nx/2 with W = S(0) + i S(nx/2) in column 0 (in-place kernel only)
 template <typename T> int fft_plane_fwd(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf,
-                                        const int* act, bool packed = false) {
+                                        const int* act, bool packed = false, const double* rscale = nullptr) {
   PlaneArgs<T> a = plane_args(ctx, s, in, out, nf, act);
+  a.rscale = rscale;
   if (packed) { a.packed = 1; a.pitch = ctx->nxh - 1; a.wofs = packed_wofs(ctx); }
   dim3 g(ctx->nz, nf, ctx->B);
   const int mode = plane_mode<T>(ctx);
@@ -679,7 +683,10 @@ template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring
 // 0-4 stage A x r2c, y, z convolution, y, x c2r; 5 k_grad; 6-10 stage B;
 // 11 k_div; 12-16 stage C; 17 k_pot4; with the fused plane transforms 0/6/12
 // are the x+y forward transforms and 1/7/13 are empty).
-template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1) {
+// dscale: [slot] factor the density is read with (the step's density scaled to
+// n4 atoms, from step_den_scale), or null.
+template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1,
+                                  const double* dscale = nullptr) {
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
   const bool pl = plane_ok<T>(ctx);  // forward x and y passes fused: launches 1, 7, 13 are empty
   const bool ipl = iplane_ok<T>(ctx);  // inverse y and x passes fused: launches 3, 9, 15 are empty
@@ -689,7 +696,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
   {
     T* in[1] = {s.den};
     cx<T>* sp[1] = {s.spec[0]};
-    if (sel(0)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act, pk) : fft_x_r2c(ctx, s, in, sp, 1, act));
+    if (sel(0)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, 1, act, pk, dscale) : fft_x_r2c(ctx, s, in, sp, 1, act, dscale));
     if (sel(1) && !pl) RC(fft_strided(ctx, s, 1, sp, 1, -1, act));
     ConvArgs<T> ca{};
     ca.in[0] = s.spec[0];
@@ -715,7 +722,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.otc = otc;
     a.cp4 = (T)ctx->p.cp4; a.cpp4 = (T)ctx->p.cpp4; a.den0s = (T)ctx->p.den0s;
     a.den = s.den; a.dencg = s.dencg; a.denalf = s.denalf; a.dx = s.dx; a.dy = s.dy; a.dz = s.dz;
-    a.s0 = s.src[0]; a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3];
+    a.s0 = s.src[0]; a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3]; a.dscale = dscale;
     dim3 g = sgrid(ctx);
 #define M_(TXX) hipLaunchKernelGGL((k_grad<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
     TX_SWITCH(ctx->tx, M_)
@@ -776,7 +783,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     a.h2o2m4 = (T)ctx->p.h2o2m4; a.alphas = (T)ctx->p.alphas; a.C = (T)ctx->p.C; a.beta = (T)ctx->p.beta;
     a.den_m = (T)ctx->p.den_m;
     a.delj4 = s.delj4; a.dencg = s.dencg; a.sto1 = s.sto1; a.ua = s.ua; a.div = s.div; a.den = s.den;
-    a.pot4 = s.pot4; a.ualphas = s.ualphas; a.corr = s.corr;
+    a.pot4 = s.pot4; a.ualphas = s.ualphas; a.corr = s.corr; a.dscale = dscale;
     hipLaunchKernelGGL((k_pot4<T>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, a);
     LAUNCHCHK();
   }
@@ -873,9 +880,15 @@ int force_t(effpot_ctx* ctx, const int* act) {
 // density would be that of ~(1 - 2 mu dt) n4 atoms and the relaxation's fixed
 // point would miss the eigenvalue equation by O(dt).  Real time keeps
 // den = |psi|^2 as the reference's stepper has it.
-int step_den_scale(effpot_ctx* ctx, const int* act, bool f32) {
+int step_den_scale(effpot_ctx* ctx, const int* act, bool f32, const double** dscale) {
+  *dscale = nullptr;
   if (ctx->rt) return EFFPOT_OK;
   RC(reduce<2>(ctx, 2, act));
+  if (!ctx->any_motion && !ctx->any_dyn) {  // poten reads the density through the factor
+    *dscale = ctx->d_dscale;
+    return EFFPOT_OK;
+  }
+  // the impurity's force and the internal-state moments read den after poten: scale it in place
   if (f32)
     hipLaunchKernelGGL((k_scale_den<float>), pw_grid(ctx->N, ctx->B), dim3(256), 0, ctx->st, ctx->s32.den,
                        (const double*)ctx->d_dscale, act, ctx->N);
@@ -910,8 +923,9 @@ template <typename T> int step_rk_t(effpot_ctx* ctx, State<T>& s, const int* act
       LAUNCHCHK();
       if (j <= 2) RC(dyn_field(ctx, act));
     }
-    if (j <= 2) RC(step_den_scale(ctx, act, f32));
-    if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
+    const double* ds = nullptr;
+    if (j <= 2) RC(step_den_scale(ctx, act, f32, &ds));
+    if (j <= 2) RC(f32 ? poten_t(ctx, ctx->s32, act, -1, ds) : poten_t(ctx, s, act, -1, ds));
     if (ctx->any_motion && j <= 2) RC(force_t(ctx, act));
     if (ctx->any_dyn && j <= 2) RC(dyn_hiv(ctx, act));
   }
@@ -934,8 +948,9 @@ template <typename T> int step_pc_t(effpot_ctx* ctx, State<T>& s, const int* act
     LAUNCHCHK();
     RC(dyn_field(ctx, act));
   }
-  RC(step_den_scale(ctx, act, f32));
-  RC(f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act));
+  const double* ds = nullptr;
+  RC(step_den_scale(ctx, act, f32, &ds));
+  RC(f32 ? poten_t(ctx, ctx->s32, act, -1, ds) : poten_t(ctx, s, act, -1, ds));
   if (ctx->any_motion) RC(force_t(ctx, act));
   if (ctx->any_dyn) RC(dyn_hiv(ctx, act));  // potinvar
   RC((launch_step<T, 1>(ctx, s, act, f32)));  // roles as after RING_AFTER_PRED
@@ -2238,8 +2253,9 @@ template <typename T> static int time_corrector_in_steps(effpot_ctx* ctx, State<
   for (int r = 0; r < reps && rc == EFFPOT_OK; r++) {
     if (whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
     rc = launch_step<T, 0>(ctx, s, act, f32);
-    if (rc == EFFPOT_OK) rc = step_den_scale(ctx, act, f32);
-    if (rc == EFFPOT_OK) rc = f32 ? poten_t(ctx, ctx->s32, act) : poten_t(ctx, s, act);
+    const double* ds = nullptr;
+    if (rc == EFFPOT_OK) rc = step_den_scale(ctx, act, f32, &ds);
+    if (rc == EFFPOT_OK) rc = f32 ? poten_t(ctx, ctx->s32, act, -1, ds) : poten_t(ctx, s, act, -1, ds);
     if (rc != EFFPOT_OK) break;
     if (!whole) HIPCHK(hipEventRecord(ev[2 * r], ctx->st));
     ctx->corr_den = false;

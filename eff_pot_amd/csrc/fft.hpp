@@ -433,6 +433,7 @@ template <typename T> struct XArgs {
   int nrows;  // ny*nz (even)
   int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
   T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
+  const double* rscale;  // r2c only: [slot] factor field 0 is read with, or null
   T aff0[4], aff1[4];
   const cx<T>* twN;
   const int* active;
@@ -449,6 +450,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
   cx<T>* dst = rm.c[f] + (long)blockIdx.z * rm.cslot;
   const int pair0 = blockIdx.x * C;  // pairs of rows
   const int npairs = rm.nrows / 2;
+  const T sc = (rm.rscale && f == 0) ? (T)rm.rscale[blockIdx.z] : T(1);
   // stage: z_c[i] = a_c[i] + i b_c[i]
   for (int e = threadIdx.x; e < C * N; e += kThreads) {
     const int c = e / N, i = e - c * N;
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
     cx<T> v{T(0), T(0)};
     if (pr < npairs) {
       const T* ra = src + (long)(2 * pr) * N;
-      v = cx<T>{ra[i], ra[N + i]};
+      v = cx<T>{sc * ra[i], sc * ra[N + i]};
     }
     lds[i * P + c] = v;
   }
@@ -596,6 +598,7 @@ template <typename T> struct PlaneArgs {
   int pitch;  // complex elements per half-spectrum row (nx/2+1 padded, or nx/2 packed)
   int packed; // packed layout: rows of nx/2 (column 0 unused) and W = S(kx=0) + i S(kx=nx/2)
   long wofs;  // packed: W of plane z, row ky at the slot's spectrum + wofs + z ny + ky
+  const double* rscale;  // forward only: [slot] factor field 0 is read with, or null
   const cx<T>* twx;
   const cx<T>* twy;
   const int* active;
@@ -676,6 +679,7 @@ __global__ __launch_bounds__((kThreads * PlaneFits<T, N>::G)) void k_fft_plane_r
     cx<T>* lds = scratch + grp * N * P;
     const T* src = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
     cx<T>* dst = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
+    const T sc = (a.rscale && f == 0) ? (T)a.rscale[blockIdx.z] : T(1);
     constexpr int NB = (N / 2 + C - 1) / C;  // row-pair batches
     for (int j = 0; j < (NB + G - 1) / G; j++) {  // x: pairs of rows
       const int p0 = (j * G + grp) * C;
@@ -686,7 +690,7 @@ __global__ __launch_bounds__((kThreads * PlaneFits<T, N>::G)) void k_fft_plane_r
           cx<T> v{T(0), T(0)};
           if (pr < N / 2) {
             const T* ra = src + (long)(2 * pr) * N;
-            v = cx<T>{ra[i], ra[N + i]};
+            v = cx<T>{sc * ra[i], sc * ra[N + i]};
           }
           lds[i * P + c] = v;
         }
@@ -762,6 +766,7 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
     const T* src = a.r[f] + (long)blockIdx.z * a.rslot + (long)iz * N * N;
     cx<T>* dst = a.c[f] + (long)blockIdx.z * a.cslot + (long)iz * N * a.pitch;
     cx<T>* wdst = a.c[f] + (long)blockIdx.z * a.cslot + a.wofs + (long)iz * N;  // packed: W of this plane
+    const T sc = (a.rscale && f == 0) ? (T)a.rscale[blockIdx.z] : T(1);
     // 1. pairs of real rows, packed: a lane takes element i of a row pair and
     //    loads it from both rows (each load coalesced along the row), every
     //    load of the thread issued before its first LDS store; consecutive
@@ -774,8 +779,8 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
         const int e = threadIdx.x + k * NT;
         if (NE % NT == 0 || e < NE) {
           const int p = e / N, i = e - p * N;
-          ra[k] = src[(long)(2 * p) * N + i];
-          rb[k] = src[(long)(2 * p + 1) * N + i];
+          ra[k] = sc * src[(long)(2 * p) * N + i];
+          rb[k] = sc * src[(long)(2 * p + 1) * N + i];
         }
       }
 #pragma unroll

@@ -160,14 +160,16 @@ template <typename T> struct Pot4Args {
   const T *delj4, *dencg, *sto1, *ua, *div, *den;
   T* pot4; T* ualphas;
   const T* corr;  // defect correction added to pot4 (FP32 fields of a mixed context), or null
+  const double* dscale;  // [slot] factor den is read with (the step's density to n4 atoms), or null
 };
 template <typename T> __global__ void k_pot4(Pot4Args<T> a) {
   const int s = blockIdx.y;
   if (!a.active[s]) return;
   const long so = (long)s * a.slot;
+  const T sc = a.dscale ? (T)a.dscale[s] : T(1);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.N; i += (long)gridDim.x * blockDim.x) {
     const long k = so + i;
-    T cg = a.dencg[k], d = a.den[k];
+    T cg = a.dencg[k], d = sc * a.den[k];
     T p = (a.delj4[k] + cg * cg * (a.a0 + a.a1 * cg)) + a.sto1[k];
     if (a.otc) {
       T ua = ((a.ua[k] / a.den0s + a.div[k]) * a.h2o2m4) * a.alphas;

@@ -383,6 +383,7 @@ template <typename T> struct PotenArgs {
   const T* den; const T* dencg; const T* denalf;
   T* dx; T* dy; T* dz;
   T* s0; T* s1; T* s2; T* s3;  // sources: correlation, falfs*d{x,y,z}den
+  const double* dscale;        // [slot] factor den is read with (the step's density to n4 atoms), or null
 };
 template <typename T, int TX>
 __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
@@ -394,16 +395,18 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
   T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
   T *s0 = a.s0 + so, *s1 = a.s1 + so, *s2 = a.s2 + so, *s3 = a.s3 + so;
   const bool otc = a.otc;
+  const T sc = a.dscale ? (T)a.dscale[s] : T(1);  // the stencil is linear: scale its results
   struct Op { T cg, fa; };
   zmarch<T, T, TX, true, false>(den, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
     [&](long i) { return Op{dencg[i], otc ? denalf[i] : T(0)}; },
     [&](int x, int y, int z, bool in, T d, T(&d1)[3], T(&)[3], const Op& o) {
       if (!in) return;
       const long i = a.g.idx(x, y, z);
-      dx[i] = d1[0]; dy[i] = d1[1]; dz[i] = d1[2];
-      s0[i] = d * o.cg * (a.cp4 + a.cpp4 * o.cg);
+      const T gx = sc * d1[0], gy = sc * d1[1], gz = sc * d1[2];
+      dx[i] = gx; dy[i] = gy; dz[i] = gz;
+      s0[i] = (sc * d) * o.cg * (a.cp4 + a.cpp4 * o.cg);
       if (otc) {  // o.fa holds falfs
-        s1[i] = o.fa * d1[0]; s2[i] = o.fa * d1[1]; s3[i] = o.fa * d1[2];
+        s1[i] = o.fa * gx; s2[i] = o.fa * gy; s3[i] = o.fa * gz;
       }
     });
 }
