@@ -47,15 +47,19 @@ FIELDS32 = ["pot4", "dencg", "delj4", "sto1", "denalf", "ualphas", "dxden", "dyd
             "intyalf", "intzalf"]
 
 
-# every case with the 256-thread z convolution, the smallest and the largest
-# also with the 512-thread one (32 lines per group)
-ZCASES = [c + ("256",) for c in CASES] + [CASES[0] + ("512",), CASES[2] + ("512",)]
+# every case with the 256-thread z convolution and the packed half-spectrum
+# layout between the plane kernels (the default); the smallest and the largest
+# also with the 512-thread z convolution (32 lines per group) and with the
+# padded layout (rows of nx/2+1 padded to a multiple of 16)
+ZCASES = ([c + ("256", "1") for c in CASES] + [CASES[0] + ("512", "1"), CASES[2] + ("512", "1")]
+          + [CASES[0] + ("256", "0"), CASES[2] + ("256", "0")])
 
 
-@pytest.mark.parametrize("n,h,natoms,zr,zth", ZCASES)
-def test_fp32_fields_match_oracle(n, h, natoms, zr, zth, gpu_available, monkeypatch):
+@pytest.mark.parametrize("n,h,natoms,zr,zth,packed", ZCASES)
+def test_fp32_fields_match_oracle(n, h, natoms, zr, zth, packed, gpu_available, monkeypatch):
     monkeypatch.setenv("EFFPOT_ZCONV_TH", zth)
-    print(f"[fp32 fields] {n}^3 h={h} N={natoms} z convolution {zth} threads", flush=True)
+    monkeypatch.setenv("EFFPOT_PACKED", packed)
+    print(f"[fp32 fields] {n}^3 h={h} N={natoms} z convolution {zth} threads, packed {packed}", flush=True)
     xmax = n * h / 2
     R = 2.22 * natoms ** (1 / 3)
     imp = make_morse(rimp=(0.0, 0.0, zr * R))

@@ -53,10 +53,12 @@ def _setup(n, xmax, n4, core, h_core, icon=ICON_ZERO, rimp=(0.0, 0.0, 0.0), batc
     return p, R, imp
 
 
-@pytest.mark.parametrize("core,zth", [(CORE_OTC, "256"), (CORE_OT, "256"), (CORE_OTC, "512")])
-def test_poten_energy(core, zth, gpu_available, monkeypatch):
+@pytest.mark.parametrize("core,zth,packed", [(CORE_OTC, "256", "1"), (CORE_OT, "256", "1"), (CORE_OTC, "512", "1"),
+                                             (CORE_OTC, "256", "0")])
+def test_poten_energy(core, zth, packed, gpu_available, monkeypatch):
     monkeypatch.setenv("EFFPOT_ZCONV_TH", zth)
-    print(f"[poten] core={core} z convolution {zth} threads", flush=True)
+    monkeypatch.setenv("EFFPOT_PACKED", packed)
+    print(f"[poten] core={core} z convolution {zth} threads, packed {packed}", flush=True)
     p, R, imp = _setup(32, 12.0, 100.0, core, 2.377, rimp=(0.0, 0.0, 1.5))
     o = Oracle(p)
     o.init_fermi(0.0218, R, 0.6, 1e-6, 20261203)
