@@ -343,6 +343,8 @@ def run_poten(a):
         tot = 0.0
         plane = c.plane_fused(a.mode == "mixed")
         iplane = c.plane_inverse_fused(a.mode == "mixed")
+        packed = c.packed_layout(a.mode == "mixed")
+        zth = c.zconv_threads()
         for k, (name, _, _) in enumerate(POTEN_LAUNCHES):
             ms = c.kernel_time(100 + k, reps=20)
             byt = poten_launch_bytes(k, a.n, a.mode, B, plane, iplane)
@@ -352,7 +354,8 @@ def run_poten(a):
     return common_line(a, "mean-field (poten) launches: ms and algorithmic GB/s", whole, "ms", whole * 1e-3,
                        "fp32 fields" if a.mode == "mixed" else "f64",
                        f"poten of the {a.n}^3 N=1000 workload, {B} slots", mode=a.mode, batch=B,
-                       stencil_tile=list(tile), plane=plane, inverse_plane=iplane, sum_of_launches_ms=tot,
+                       stencil_tile=list(tile), plane=plane, inverse_plane=iplane, packed=packed,
+                       zconv_threads=zth, sum_of_launches_ms=tot,
                        launches=out)
 
 

@@ -75,6 +75,7 @@ def lib():
         L.effpot_plane_fused.argtypes = [vp, C.c_int]
         L.effpot_plane_inverse_fused.argtypes = [vp, C.c_int]
         L.effpot_zconv_threads.argtypes = [vp]
+        L.effpot_packed_layout.argtypes = [vp, C.c_int]
         L.effpot_set_exclusion.argtypes = [vp, C.c_int, C.c_double]
         L.effpot_set_force_filter.argtypes = [vp, C.c_int, C.c_int]
         L.effpot_tick.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(Slot)]
@@ -453,6 +454,10 @@ class Context:
         """1 when poten()'s inverse y and x transforms run as one in-place
         plane launch, 0 for separate passes."""
         return int(lib().effpot_plane_inverse_fused(self.h, 1 if fp32_fields else 0))
+
+    def packed_layout(self, fp32_fields):
+        """1 when poten()'s half spectra use the packed layout between the plane kernels."""
+        return int(lib().effpot_packed_layout(self.h, 1 if fp32_fields else 0))
 
     def zconv_threads(self):
         """Threads per z-convolution work-group (256 or 512) this context uses."""

@@ -185,7 +185,8 @@ struct effpot_ctx {
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
   int iplane = 1;                 // inverse y and x passes fused per z plane where it fits (EFFPOT_IPLANE)
-  bool packed = true;             // packed half-spectrum layout inside poten when both plane kernels run (EFFPOT_PACKED)
+  int packed = -1;                // packed half spectra inside poten: 1 on, 0 off, -1 when the z convolution fills the device
+  bool packed_fixed = false;      // EFFPOT_PACKED given: the autotuner keeps it
   int zconv_th = 256;             // threads per z-convolution group: 256 or 512 (EFFPOT_ZCONV_TH)
   bool zconv_fixed = false;       // EFFPOT_ZCONV_TH given: the autotuner keeps it
   bool iplane_fixed = false;      // EFFPOT_IPLANE given: the autotuner keeps it
@@ -528,8 +529,17 @@ template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx
 // with the in-place inverse) the z convolutions see 64 aligned lines per row at
 // 128^3 instead of 65 in rows padded to 80, and the inverse plane reads whole
 // 512-byte rows.
+// Its packed-column groups are longer chains than the others, so it pays
+// where the z convolution has many groups to run beside them (measured: 128^3
+// FP32 x 4 slots and 96^3 FP32 x 8, 5-6 % shorter steps) and costs where the
+// launch is one wave of groups (64^3 FP64 single slot: 14 % longer).  By
+// default it is used from 1024 groups per field (256-thread groups); the
+// autotuner times both.
 template <typename T> bool packed_ok(effpot_ctx* ctx) {
-  return ctx->packed && plane_mode<T>(ctx) >= 2 && iplane_ok<T>(ctx);
+  if (ctx->packed == 0 || plane_mode<T>(ctx) < 2 || !iplane_ok<T>(ctx)) return false;
+  if (ctx->packed == 1) return true;
+  const long groups = (long)(ctx->nxh - 1) * ctx->ny / (kThreads / n1_of(ctx->nz)) * ctx->B;
+  return groups >= 1024;
 }
 template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act,
                                     bool packed = false) {
@@ -1125,7 +1135,10 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
   ctx->h_eso.assign(ctx->B, 0.0);
   if (const char* e = std::getenv("EFFPOT_PLANE")) { ctx->plane = std::atoi(e); ctx->plane_fixed = true; }
   if (const char* e = std::getenv("EFFPOT_IPLANE")) { ctx->iplane = std::atoi(e) != 0; ctx->iplane_fixed = true; }
-  if (const char* e = std::getenv("EFFPOT_PACKED")) ctx->packed = std::atoi(e) != 0;
+  if (const char* e = std::getenv("EFFPOT_PACKED")) {
+    ctx->packed = std::atoi(e) != 0 ? 1 : 0;
+    ctx->packed_fixed = true;
+  }
   if (const char* e = std::getenv("EFFPOT_ZCONV_TH")) {
     ctx->zconv_th = std::atoi(e) == 512 ? 512 : 256;
     ctx->zconv_fixed = true;
@@ -2366,6 +2379,22 @@ int effpot_autotune(effpot_ctx* ctx, int reps, int* tx_out, int* zc_out) {
     }
     ctx->iplane = best_i;
   }
+  // the half-spectrum layout inside poten: packed or padded (where the packed
+  // one applies), timed on whole poten launches of the live precision
+  if (!ctx->packed_fixed) {
+    const bool f32 = ctx->mixed;
+    int best_k = ctx->packed;
+    double best_p = 1e30;
+    for (int k : {0, 1}) {
+      ctx->packed = k;
+      if (k == 1 && !(f32 ? packed_ok<float>(ctx) : packed_ok<double>(ctx))) continue;
+      double t = 0;
+      RC(time_kernel_t(ctx, ctx->s64, 2, reps, &t, f32));
+      if (ctx->tune_log) std::fprintf(stderr, "autotune packed=%d poten=%.4f ms\n", k, t);
+      if (t < best_p) { best_p = t; best_k = k; }
+    }
+    ctx->packed = best_k;
+  }
   // the z-convolution group: 16 or 32 lines (256 or 512 threads), timed on
   // whole poten launches of the live precision
   if (!ctx->zconv_fixed) {
@@ -2410,6 +2439,11 @@ int effpot_kernel_time(effpot_ctx* ctx, int which, int reps, double* ms_per_laun
 int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields) {
   if (check_ctx(ctx) != EFFPOT_OK) return 0;
   return fp32_fields ? plane_mode<float>(ctx) : plane_mode<double>(ctx);
+}
+
+int effpot_packed_layout(effpot_ctx* ctx, int fp32_fields) {
+  if (check_ctx(ctx) != EFFPOT_OK) return 0;
+  return (fp32_fields ? packed_ok<float>(ctx) : packed_ok<double>(ctx)) ? 1 : 0;
 }
 
 int effpot_zconv_threads(effpot_ctx* ctx) {

@@ -516,6 +516,14 @@ int effpot_plane_fused(effpot_ctx* ctx, int fp32_fields);
  * 0 for separate passes; with FP32 (fp32_fields = 1) or FP64 fields. */
 int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields);
 
+/* Benchmark hook (no reference counterpart): 1 when poten() keeps its half
+ * spectra in the packed layout between the plane kernels (rows of nx/2 and
+ * W = S(kx=0) + i S(kx=nx/2) in a block of its own), 0 for rows of nx/2+1
+ * padded to a multiple of 16; with FP32 (fp32_fields = 1) or FP64 fields.
+ * Set by the autotuner, by EFFPOT_PACKED, or by default from 1024 z-line
+ * groups per field. */
+int effpot_packed_layout(effpot_ctx* ctx, int fp32_fields);
+
 /* Benchmark hook (no reference counterpart): threads per work-group of the
  * z convolutions of poten(), 256 (16 lines of the half spectrum per group)
  * or 512 (32 lines), as the autotuner or EFFPOT_ZCONV_TH left it. */
@@ -542,8 +550,9 @@ int effpot_tick(effpot_ctx* ctx, const effpot_solve_opts* opt, effpot_slot* slot
  * then times poten under each forward-transform schedule that applies
  * (effpot_plane_fused: 0, 1, 3; the environment variable EFFPOT_PLANE fixes
  * it instead) and keeps the fastest, and the same for the inverse schedule
- * (effpot_plane_inverse_fused; EFFPOT_IPLANE) and the z-convolution
- * group size (effpot_zconv_threads; EFFPOT_ZCONV_TH).  Overwrites the
+ * (effpot_plane_inverse_fused; EFFPOT_IPLANE), the half-spectrum layout
+ * (effpot_packed_layout; EFFPOT_PACKED) and the z-convolution group size
+ * (effpot_zconv_threads; EFFPOT_ZCONV_TH).  Overwrites the
  * integrator state like effpot_kernel_time: call it before uploading states. */
 int effpot_autotune(effpot_ctx* ctx, int reps, int* tx, int* zc);
 /* Sets the stencil tile explicitly (x width, z chunk). */
