@@ -140,6 +140,7 @@ template <typename T> struct State {
   T *intx = nullptr, *inty = nullptr, *intz = nullptr, *div = nullptr, *src[4] = {nullptr, nullptr, nullptr, nullptr};
   cx<T>* spec[8] = {};  // each [B][M]
   T* kern[3] = {};      // wcgk, fvlj4, kalfs: [M]
+  T* kw[3] = {};        // the same at kx = 0 and nx/2 as pairs ((K0+KH)/2, (K0-KH)/2): [nz][ny][2]
   cx<T>* tw[3] = {};    // twiddles per axis
   StenCoef<T> sc;
   int* ring = nullptr;  // [B][8]
@@ -185,7 +186,7 @@ struct effpot_ctx {
   int plane = 3;                  // forward x and y FFT passes fused per z plane (plane_mode; EFFPOT_PLANE)
   bool plane_fixed = false;       // EFFPOT_PLANE given: the autotuner keeps it
   int iplane = 1;                 // inverse y and x passes fused per z plane where it fits (EFFPOT_IPLANE)
-  int packed = -1;                // packed half spectra inside poten: 1 on, 0 off, -1 when the z convolution fills the device
+  int packed = 1;                 // packed half spectra inside poten where both plane kernels run (EFFPOT_PACKED)
   bool packed_fixed = false;      // EFFPOT_PACKED given: the autotuner keeps it
   int zconv_th = 256;             // threads per z-convolution group: 256 or 512 (EFFPOT_ZCONV_TH)
   bool zconv_fixed = false;       // EFFPOT_ZCONV_TH given: the autotuner keeps it
@@ -306,6 +307,7 @@ template <typename T> int alloc_state(effpot_ctx* ctx, State<T>& s, bool fields_
   if (fields_only) A_(s.corr, B * N);
   for (int j = 0; j < 8; j++) A_(s.spec[j], B * M);
   for (int j = 0; j < 3; j++) A_(s.kern[j], M);
+  for (int j = 0; j < 3; j++) A_(s.kw[j], 2L * ctx->ny * ctx->nz);
   A_(s.tw[0], ctx->nx); A_(s.tw[1], ctx->ny); A_(s.tw[2], ctx->nz);
   A_(s.ring, B * 8);
 #undef A_
@@ -529,17 +531,16 @@ template <typename T> int fft_strided(effpot_ctx* ctx, State<T>& s, int axis, cx
 // with the in-place inverse) the z convolutions see 64 aligned lines per row at
 // 128^3 instead of 65 in rows padded to 80, and the inverse plane reads whole
 // 512-byte rows.
-// Its packed-column groups are longer chains than the others, so it pays
-// where the z convolution has many groups to run beside them (measured: 128^3
-// FP32 x 4 slots and 96^3 FP32 x 8, 5-6 % shorter steps) and costs where the
-// launch is one wave of groups (64^3 FP64 single slot: 14 % longer).  By
-// default it is used from 1024 groups per field (256-thread groups); the
-// autotuner times both.
+// The packed-column groups read their factors from precomputed pairs
+// (State::kw, contiguous in ky like W): read from the padded tables instead,
+// one cache line per value, they made those groups the longest chains and the
+// layout slower where a launch is one wave of groups (64^3 FP64 single slot:
+// step 162 -> 186 us); with the pairs it is as fast or faster at every shape
+// measured (64^3 FP64 single 164 / 165 us, 96^3 FP64 single 324 -> 316 us,
+// 96^3 FP32 x 8 950 -> 888 us, 128^3 FP32 x 4 1063 -> 1001 us).  The autotuner
+// still times both.
 template <typename T> bool packed_ok(effpot_ctx* ctx) {
-  if (ctx->packed == 0 || plane_mode<T>(ctx) < 2 || !iplane_ok<T>(ctx)) return false;
-  if (ctx->packed == 1) return true;
-  const long groups = (long)(ctx->nxh - 1) * ctx->ny / (kThreads / n1_of(ctx->nz)) * ctx->B;
-  return groups >= 1024;
+  return ctx->packed != 0 && plane_mode<T>(ctx) >= 2 && iplane_ok<T>(ctx);
 }
 template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a, int ngroups, const int* act,
                                     bool packed = false) {
@@ -550,7 +551,11 @@ template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a
   if (packed) {  // lines kx' = 1 .. P-1 of rows of P; the factor tables keep their padded rows
     a.lm = LineMap{P, (long)P, (long)P * ctx->ny, P * ctx->ny, P, 1};
     a.km = LineMap{P, (long)ctx->nxp, (long)ctx->nxp * ctx->ny, P * ctx->ny, P, 1};
-    a.P = P; a.ny = ctx->ny; a.kpitch = ctx->nxp; a.wofs = packed_wofs(ctx);
+    a.P = P; a.ny = ctx->ny; a.wofs = packed_wofs(ctx);
+    for (int g = 0; g < ngroups; g++)  // each factor table's pairs for the packed column
+      for (int j = 0; j < a.nout[g]; j++)
+        for (int k = 0; k < 3; k++)
+          if (a.kern[g][j] == s.kern[k]) a.kw[g][j] = s.kw[k];
   }
   const int th = ctx->zconv_th;
   const int C = th / n1_of(ctx->nz);
@@ -1068,7 +1073,8 @@ int64_t effpot_footprint_bytes(const effpot_params* p) {
   const int64_t per64 = (2 * 9 + 20) * N * 8 + 8 * M * 16;
   int64_t per = per64;
   if (p->precision == EFFPOT_PREC_MIXED) per += per64 / 2;
-  return per * (int64_t)p->batch + 3 * M * 12 + (int64_t)(100 * 256 + 3) * 8 * p->batch;
+  return per * (int64_t)p->batch + 3 * M * 12 + 3 * 2 * (int64_t)p->ny * p->nz * 12 +
+         (int64_t)(100 * 256 + 3) * 8 * p->batch;
 }
 
 int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
@@ -1205,6 +1211,15 @@ int effpot_ctx_create(const effpot_params* p, int device, effpot_ctx** out) {
       hipLaunchKernelGGL((k_convert<double, float>), dim3(512), dim3(256), 0, ctx->st, ctx->s64.kern[j],
                          ctx->s32.kern[j], ctx->M);
   }
+  // the factor pairs of the packed column, in each precision from that precision's tables
+  for (int j = 0; j < 3; j++) {
+    hipLaunchKernelGGL((k_pair_tables<double>), dim3(256), dim3(256), 0, ctx->st, (const double*)ctx->s64.kern[j],
+                       ctx->s64.kw[j], ctx->nxp, ctx->nxh - 1, ctx->ny, ctx->nz);
+    if (ctx->mixed)
+      hipLaunchKernelGGL((k_pair_tables<float>), dim3(256), dim3(256), 0, ctx->st, (const float*)ctx->s32.kern[j],
+                         ctx->s32.kw[j], ctx->nxp, ctx->nxh - 1, ctx->ny, ctx->nz);
+  }
+  if (hipGetLastError() != hipSuccess) return bail(EFFPOT_ERR_DEVICE);
   if (hipStreamSynchronize(ctx->st) != hipSuccess) return bail(EFFPOT_ERR_DEVICE);
   *out = ctx;
   return EFFPOT_OK;

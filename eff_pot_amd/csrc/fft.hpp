@@ -302,7 +302,7 @@ template <typename T> struct ConvArgs {
   int nw;      // packed layout: the first nw groups (blockIdx.x) convolve the packed column W
   int P, ny;   // packed row length (nx/2), rows per plane
   long wofs;   // W(ky, kz) at the slot's spectrum + wofs + kz ny + ky
-  int kpitch;  // row length of the factor tables
+  const T* kw[4][3];  // factor pairs ((K0+KH)/2, (K0-KH)/2) of kern[g][j] at [kz][ky]
   T scale;
   const cx<T>* twN;
   const int* active;
@@ -333,8 +333,8 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
   const bool valid = p < ny;
   const int ky = p == 0 ? 0 : p == 1 ? hy : ((p & 1) ? ny - (p >> 1) : (p >> 1));
   const int pc = p <= 1 ? c : (c ^ 1);  // the partner's lane in this group
-  const long srow = ny, krow = (long)a.kpitch * ny;  // W is stored kz-major, ky fastest
-  const long sb = a.wofs + ky, kb = (long)ky * a.kpitch;
+  const long srow = ny;  // W is stored kz-major, ky fastest
+  const long sb = a.wofs + ky;
   auto load = [&](int, int i) -> cx<T> { return valid ? in[sb + (long)i * srow] : cx<T>{T(0), T(0)}; };
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
   __syncthreads();
@@ -352,13 +352,12 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
   }
   const int nout = a.nout[g];
   for (int j = 0; j < nout; j++) {
-    const T* K = a.kern[g][j];
+    const T* K = a.kw[g][j];
     cx<T> v[N2];
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
-      const long kk = kb + (long)(kk1 + N1 * k2) * krow;
-      const T k0 = valid ? K[kk] : T(0), kh = valid ? K[kk + a.P] : T(0);
-      const T ap = T(0.5) * (k0 + kh) * a.scale, am = T(0.5) * (k0 - kh) * a.scale;
+      const long kk = 2 * ((long)(kk1 + N1 * k2) * ny + ky);
+      const T ap = (valid ? K[kk] : T(0)) * a.scale, am = (valid ? K[kk + 1] : T(0)) * a.scale;
       v[k2] = cx<T>{ap * spec[k2].x + am * part[k2].x, ap * spec[k2].y + am * part[k2].y};
     }
     __syncthreads();  // LDS reuse

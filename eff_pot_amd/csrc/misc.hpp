@@ -295,6 +295,20 @@ __global__ void k_scale_den(P* den, const double* f, const int* active, long N) 
     d[i] = (P)((double)d[i] * fs);
 }
 
+// Factor pairs of the packed column: kw[kz][ky] = ((K0 + KH)/2, (K0 - KH)/2)
+// with K0, KH the k-space factor at kx = 0 and kx = P (= nx/2) of the padded
+// table k (rows of nxp), in the precision of k.
+template <typename T>
+__global__ void k_pair_tables(const T* k, T* kw, int nxp, int P, int ny, int nz) {
+  const long n = (long)ny * nz;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long kz = e / ny, ky = e - kz * ny;
+    const T k0 = k[ky * nxp + kz * (long)nxp * ny], kh = k[P + ky * nxp + kz * (long)nxp * ny];
+    kw[2 * e] = T(0.5) * (k0 + kh);
+    kw[2 * e + 1] = T(0.5) * (k0 - kh);
+  }
+}
+
 // ---- precision conversion (FP32 phase -> FP64 polish) -----------------------
 template <typename A, typename B>
 __global__ void k_convert(const A* in, B* out, long n) {

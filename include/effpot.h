@@ -520,8 +520,8 @@ int effpot_plane_inverse_fused(effpot_ctx* ctx, int fp32_fields);
  * spectra in the packed layout between the plane kernels (rows of nx/2 and
  * W = S(kx=0) + i S(kx=nx/2) in a block of its own), 0 for rows of nx/2+1
  * padded to a multiple of 16; with FP32 (fp32_fields = 1) or FP64 fields.
- * Set by the autotuner, by EFFPOT_PACKED, or by default from 1024 z-line
- * groups per field. */
+ * On by default where both in-place plane kernels run; the autotuner times
+ * both layouts and EFFPOT_PACKED fixes it. */
 int effpot_packed_layout(effpot_ctx* ctx, int fp32_fields);
 
 /* Benchmark hook (no reference counterpart): threads per work-group of the
