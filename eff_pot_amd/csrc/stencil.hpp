@@ -52,13 +52,22 @@ EP_DEV long colofs(int x, int y, const Grid& g) {
   return (long)x + (long)g.nx * y;
 }
 // f at column offset c (from colofs) and plane z, with the boundary rule in z.
-template <typename V> EP_DEV V ldz(const V* f, long c, int z, const Grid& g) {
-  if (c < 0) return vzero((V*)nullptr);
+// Offset of plane z from a field's base, wrapped for icon 8; -1 when the
+// plane lies outside the grid for icon 13 (its values are zero).  It is
+// uniform over the workgroup: the z-marches form it once per plane and share
+// it between the loads of that plane (formed inside every load, the wrap and
+// the 64-bit product sat under each load's lane branch and were repeated per
+// load: some 30 scalar instructions each).
+EP_DEV long zofs(int z, const Grid& g) {
   if (z < 0 || z >= g.nz) {
-    if (!g.periodic) return vzero((V*)nullptr);
+    if (!g.periodic) return -1;
     z = (z + g.nz) % g.nz;
   }
-  return f[c + (long)g.nx * g.ny * z];
+  return (long)g.nx * g.ny * z;
+}
+// f at column c (colofs) of the plane at offset zo (zofs); zero outside the grid.
+template <typename V> EP_DEV V ldz(const V* f, long c, long zo) {
+  return (c < 0 || zo < 0) ? vzero((V*)nullptr) : f[c + zo];
 }
 
 template <int TX> struct Tile {
@@ -129,9 +138,10 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
   V q[13], hv[NH];
   const V zero = vzero((V*)nullptr);
 #pragma unroll
-  for (int k = 0; k < 13; k++) q[k] = ldz(f, col, z0 - 6 + k, g);
+  for (int k = 0; k < 13; k++) q[k] = ldz(f, col, zofs(z0 - 6 + k, g));
+  const long zo0 = zofs(z0, g);
 #pragma unroll
-  for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], z0, g);
+  for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], zo0);
   using PV = decltype(pre(0L));
   PV p0{};
   if (tp.inside) p0 = pre(col + plane * z0);
@@ -146,9 +156,10 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
     V qn = zero;
     PV p1{};
     if (more) {
-      qn = ldz(f, col, z + 7, g);
+      qn = ldz(f, col, zofs(z + 7, g));
+      const long zo = zofs(z + 1, g);
 #pragma unroll
-      for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], z + 1, g);
+      for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], zo);
       if (tp.inside) p1 = pre(col + plane * (z + 1));
     }
     const V* pl = lds + buf * PL;
@@ -456,16 +467,17 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
     const int e = threadIdx.x + k * kThreads, r = e / TX, c = e - TX * (e / TX);
     cy_[k] = e < NYS ? colofs(tp.bx * TX + c, tp.by * TY + r - 6, g) : -1;
   }
-  auto prod = [&](const T* comp, long c, int z) { return ldz(fa, c, z, g) * ldz(comp, c, z, g); };
+  auto prod = [&](const T* comp, long c, long zo) { return ldz(fa, c, zo) * ldz(comp, c, zo); };
   struct Op { T dx, dy, dz, ix, iy, iz; };
   auto pre = [&](long i) { return Op{dx[i], dy[i], dz[i], ix[i], iy[i], iz[i]}; };
   T q[13], hx[KX], hy[KY];
 #pragma unroll
-  for (int k = 0; k < 13; k++) q[k] = prod(iz, col, tp.z0 - 6 + k);
+  for (int k = 0; k < 13; k++) q[k] = prod(iz, col, zofs(tp.z0 - 6 + k, g));
+  const long zo0 = zofs(tp.z0, g);
 #pragma unroll
-  for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], tp.z0);
+  for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo0);
 #pragma unroll
-  for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], tp.z0);
+  for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo0);
   Op p0{};
   if (tp.inside) p0 = pre(col + plane * tp.z0);
 #pragma unroll
@@ -481,11 +493,12 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
     T qn = T(0);
     Op p1{};
     if (more) {
-      qn = prod(iz, col, z + 7);
+      qn = prod(iz, col, zofs(z + 7, g));
+      const long zo = zofs(z + 1, g);
 #pragma unroll
-      for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], z + 1);
+      for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo);
 #pragma unroll
-      for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], z + 1);
+      for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo);
       if (tp.inside) p1 = pre(col + plane * (z + 1));
     }
     if (tp.inside) {
