@@ -173,23 +173,42 @@ struct LineMap {
   }
   // A thread of the strided and z-convolution passes reads and writes one
   // line only: its base offset (addr(c, 0)) and validity are formed once, so
-  // the runtime division by ncx is not repeated at every access.
+  // the runtime division by ncx is not repeated at every access.  A line that
+  // is skipped (a padding line or the unused packed column) keeps its own
+  // address, which lies in the allocated row next to its neighbours' (lines
+  // past the last are given the last line's), so its loads may be issued
+  // and their values discarded without touching other cache lines.
   struct Line {
     unsigned base, step;
     bool ok;
     EP_DEV unsigned at(int i) const { return base + (unsigned)i * step; }
   };
   EP_DEV Line line(int c) const {
-    const bool ok = valid(c);
-    return Line{ok ? addr(c, 0) : 0u, (unsigned)s_elem, ok};
+    return Line{addr(min(c, nlines - 1), 0), (unsigned)s_elem, valid(c)};
   }
 };
+
+// Element i of an array whose elements all lie within 4 GiB of its base,
+// addressed by a 32-bit byte offset: the loads and stores then take the
+// uniform-base + 32-bit-offset form instead of a 64-bit address per access
+// held in two VGPRs.
+template <typename V> EP_DEV V ld32(const V* base, unsigned i) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(V));
+}
+template <typename V> EP_DEV void st32(V* base, unsigned i, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(V)) = v;
+}
 
 // ---- the two-level transform, step by step -------------------------------------
 // LDS layout between the steps: index (k1*N2 + n2)*C + c (c fastest: conflict free).
 
 // The group has TH threads (kThreads unless a kernel says otherwise) and
 // works on C = TH / N1 lines.
+// Twiddles: every kernel multiplies by twN[a b] whenever the unrolled index a
+// is nonzero, whatever the lane's index b.  twN[0] = (1, -0) and a product by
+// it returns its operand (up to the sign of a zero), so the lanes with b = 0
+// lose nothing; under a lane-dependent branch each twiddle load waited out
+// its whole latency before the next was issued.
 // Step 1+2 for the lines of this group: a[n1] = x[N2 n1 + n2]; FFT_N1; twiddle; to LDS.
 template <typename T, int N, int TH = kThreads, class Load>
 EP_DEV void fft_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load) {
@@ -204,8 +223,8 @@ EP_DEV void fft_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) {
       cx<T> v = a[k1];
-      if (k1 > 0 && n2 > 0) {
-        cx<T> w = twN[n2 * k1];
+      if (k1 > 0) {
+        cx<T> w = ld32(twN, n2 * k1);
         if (sign > 0) w.y = -w.y;
         v = cmul(v, w);
       }
@@ -236,8 +255,8 @@ EP_DEV void fft_from_regs(cx<T> (&a)[Shape<N>::N2], cx<T>* lds, const cx<T>* twN
 #pragma unroll
     for (int kk = 0; kk < N2; kk++) {
       cx<T> v = a[kk];
-      if (kk > 0 && k1 > 0) {
-        cx<T> w = twN[k1 * kk];
+      if (kk > 0) {
+        cx<T> w = ld32(twN, k1 * kk);
         if (sign > 0) w.y = -w.y;
         v = cmul(v, w);
       }
@@ -276,7 +295,12 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs<T> a) {
   const int c0 = blockIdx.x * C;
   // every access of this thread (step 1 loads, step 3 stores) is on line c0 + t % C
   const LineMap::Line ln = lm.line(c0 + (int)(threadIdx.x % C));
-  auto load = [&](int, int i) -> cx<T> { return ln.ok ? d[ln.at(i)] : cx<T>{T(0), T(0)}; };
+  // unconditional loads (a skipped line reads its own, allocated, positions)
+  // and a select: no lane branch around each load
+  auto load = [&](int, int i) -> cx<T> {
+    const cx<T> v = ld32(d, ln.at(i));
+    return ln.ok ? v : cx<T>{T(0), T(0)};
+  };
   fft_step1<T, N>(lds, a.twN, a.sign, load);
   __syncthreads();
   cx<T> b[N2];
@@ -333,9 +357,13 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
   const bool valid = p < ny;
   const int ky = p == 0 ? 0 : p == 1 ? hy : ((p & 1) ? ny - (p >> 1) : (p >> 1));
   const int pc = p <= 1 ? c : (c ^ 1);  // the partner's lane in this group
+  const int kyl = valid ? ky : 0;  // the row the loads use (in range for every lane)
   const long srow = ny;  // W is stored kz-major, ky fastest
-  const long sb = a.wofs + ky;
-  auto load = [&](int, int i) -> cx<T> { return valid ? in[sb + (long)i * srow] : cx<T>{T(0), T(0)}; };
+  const long sb = a.wofs + kyl;
+  auto load = [&](int, int i) -> cx<T> {
+    const cx<T> v = in[sb + (long)i * srow];
+    return valid ? v : cx<T>{T(0), T(0)};
+  };
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
   __syncthreads();
   cx<T> spec[N2];
@@ -356,8 +384,9 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
     cx<T> v[N2];
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
-      const long kk = 2 * ((long)(kk1 + N1 * k2) * ny + ky);
-      const T ap = (valid ? K[kk] : T(0)) * a.scale, am = (valid ? K[kk + 1] : T(0)) * a.scale;
+      const long kk = 2 * ((long)(kk1 + N1 * k2) * ny + kyl);
+      const T kp = K[kk], kq = K[kk + 1];
+      const T ap = (valid ? kp : T(0)) * a.scale, am = (valid ? kq : T(0)) * a.scale;
       v[k2] = cx<T>{ap * spec[k2].x + am * part[k2].x, ap * spec[k2].y + am * part[k2].y};
     }
     __syncthreads();  // LDS reuse
@@ -392,7 +421,11 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   const LineMap::Line ln = lm.line(c0 + (int)(threadIdx.x % C));
   const LineMap::Line lk = a.km.line(c0 + (int)(threadIdx.x % C));
   const bool valid = ln.ok;
-  auto load = [&](int, int i) -> cx<T> { return valid ? in[ln.at(i)] : cx<T>{T(0), T(0)}; };
+  // unconditional loads and selects, as in k_fft_strided
+  auto load = [&](int, int i) -> cx<T> {
+    const cx<T> v = ld32(in, ln.at(i));
+    return valid ? v : cx<T>{T(0), T(0)};
+  };
   const int nout = a.nout[g];
   // k-space factors of this thread's outputs: the first output's are loaded
   // before the forward transform, each next output's during the previous
@@ -401,7 +434,10 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   auto load_k = [&](int j) {
     const T* K = a.kern[g][j];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) kn[k2] = valid ? K[lk.at(kk1 + N1 * k2)] : T(0);
+    for (int k2 = 0; k2 < N2; k2++) {
+      const T k = ld32(K, lk.at(kk1 + N1 * k2));
+      kn[k2] = valid ? k : T(0);
+    }
   };
   load_k(0);
   fft_step1<T, N, TH>(lds, a.twN, -1, load);
@@ -416,7 +452,7 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
     __syncthreads();  // LDS reuse
     cx<T>* out = a.out[g][j] + so;
     auto store = [&](int, int i, cx<T> val) {
-      if (valid) out[ln.at(i)] = val;
+      if (valid) st32(out, ln.at(i), val);
     };
     fft_from_regs<T, N, TH>(v, lds, a.twN, +1, store);
   }
@@ -477,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) {
       cx<T> v = a[k1];
-      if (k1 > 0 && n2 > 0) v = cmul(v, twN[n2 * k1]);
+      if (k1 > 0) v = cmul(v, twN[n2 * k1]);
       lds[(k1 * N2 + n2) * C + c] = v;
     }
   }
@@ -548,7 +584,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_c2r(XArgs<T> rm) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) {
       cx<T> v = a[k1];
-      if (k1 > 0 && n2 > 0) v = cmul(v, conjg(twN[n2 * k1]));
+      if (k1 > 0) v = cmul(v, conjg(twN[n2 * k1]));
       lds[(k1 * N2 + n2) * C + c] = v;
     }
   }
@@ -618,7 +654,7 @@ EP_DEV void sub_step1(cx<T>* lds, const cx<T>* twN, int sign, Load load, int t, 
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) {
       cx<T> v = a[k1];
-      if (k1 > 0 && n2 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
+      if (k1 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
       lds[(k1 * N2 + n2) * C + c] = v;
     }
   }
@@ -649,7 +685,7 @@ EP_DEV void plane_x_fft(cx<T>* lds, const cx<T>* twN, int sign, int t, bool on) 
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) {
       cx<T> v = a[k1];
-      if (k1 > 0 && n2 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
+      if (k1 > 0) v = cmul(v, sign < 0 ? twN[n2 * k1] : conjg(twN[n2 * k1]));
       lds[(k1 * N2 + n2) * C + c] = v;
     }
   }
@@ -813,7 +849,7 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
           cx<T> w = v[k1];
-          if (k1 > 0 && n2 > 0) w = cmul(w, a.twx[n2 * k1]);
+          if (k1 > 0) w = cmul(w, a.twx[n2 * k1]);
           plane[(k1 * N2 + n2) * Q + p] = w;  // the positions this thread read
         }
       }
@@ -861,7 +897,7 @@ __global__ __launch_bounds__(kThreads * G, W) void k_fft_plane_r2c_ip(PlaneArgs<
         RegFFT<T, N1>::run(v, -1);
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++)
-          if (k1 > 0 && n2 > 0) v[k1] = cmul(v[k1], a.twy[n2 * k1]);
+          if (k1 > 0) v[k1] = cmul(v[k1], a.twy[n2 * k1]);
       }
       __syncthreads();  // every column's Z values are read before any is overwritten
       if (s1) {
@@ -1024,7 +1060,7 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
           cx<T> w = v[k1];
-          if (k1 > 0 && n2 > 0) w = cmul(w, conjg(a.twy[n2 * k1]));
+          if (k1 > 0) w = cmul(w, conjg(a.twy[n2 * k1]));
           plane[pos(kk, k1 * N2 + n2)] = w;  // the positions this thread read
         }
       }
@@ -1074,7 +1110,7 @@ __global__ __launch_bounds__((kThreads * PlaneInv<T, N>::G)) void k_fft_plane_c2
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
           cx<T> w = v[k1];
-          if (k1 > 0 && n2 > 0) w = cmul(w, conjg(a.twx[n2 * k1]));
+          if (k1 > 0) w = cmul(w, conjg(a.twx[n2 * k1]));
           plane[(k1 * N2 + n2) * Q + p] = w;
         }
       }
