@@ -122,7 +122,7 @@ __global__ void k_den_from_cur(Ring<T> rg, P* den, const int* active, long N) {
 template <typename T, int TX, int ID>
 __global__ __launch_bounds__(kThreads) void k_deriv1(Grid g, StenCoef<T> sc, int zc, const T* f, T* df, int axis) {
   __shared__ T lds[2 * Tile<TX>::PL];
-  zmarch<T, T, TX, ID == 1, ID == 2>(f, g, sc, TilePos<TX>(g, zc), lds, [](long) { return NoPre{}; },
+  zmarch<T, T, TX, ID == 1, ID == 2>(f, g, sc, TilePos<TX>(g, zc), lds, [](unsigned) { return NoPre{}; },
     [&](int x, int y, int z, bool in, T, T(&d1)[3], T(&d2)[3], const NoPre&) {
       if (!in) return;
       df[g.idx(x, y, z)] = ID == 1 ? d1[axis] : d2[axis];

@@ -24,6 +24,17 @@ template <typename T> EP_HD cx<T> mul_si(cx<T> a, int sign) {
   return sign < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
 }
 
+// Element i of an array whose elements all lie within 4 GiB of its base (one
+// field of one slot: at most 256^3 complex FP64 values, 268 MB), addressed by
+// a 32-bit byte offset: the loads and stores then take the uniform-base +
+// 32-bit-offset form instead of a 64-bit address per access held in two VGPRs.
+template <typename V> EP_DEV V ld32(const V* base, unsigned i) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(V));
+}
+template <typename V> EP_DEV void st32(V* base, unsigned i, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(V)) = v;
+}
+
 // Value helpers so stencil code is generic over real and complex fields.
 template <typename T> EP_HD T vzero(T*) { return T(0); }
 template <typename T> EP_HD cx<T> vzero(cx<T>*) { return cx<T>{T(0), T(0)}; }

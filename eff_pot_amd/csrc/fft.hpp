@@ -188,17 +188,6 @@ struct LineMap {
   }
 };
 
-// Element i of an array whose elements all lie within 4 GiB of its base,
-// addressed by a 32-bit byte offset: the loads and stores then take the
-// uniform-base + 32-bit-offset form instead of a 64-bit address per access
-// held in two VGPRs.
-template <typename V> EP_DEV V ld32(const V* base, unsigned i) {
-  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(V));
-}
-template <typename V> EP_DEV void st32(V* base, unsigned i, V v) {
-  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(V)) = v;
-}
-
 // ---- the two-level transform, step by step -------------------------------------
 // LDS layout between the steps: index (k1*N2 + n2)*C + c (c fastest: conflict free).
 

@@ -28,6 +28,10 @@ struct Grid {
   int nx, ny, nz, periodic;
   long N;
   EP_DEV long idx(int x, int y, int z) const { return (long)x + (long)nx * ((long)y + (long)ny * z); }
+  // the same offset as 32 bits (a field of one slot holds fewer than 2^32 values)
+  EP_DEV unsigned at(int x, int y, int z) const {
+    return (unsigned)x + (unsigned)nx * ((unsigned)y + (unsigned)ny * (unsigned)z);
+  }
 };
 
 // Scaled 13-point coefficients for each axis (c_m / h^id), first and second
@@ -43,31 +47,30 @@ template <typename T> struct StenCoef {
 
 // Offset of column (x, y) within a plane, wrapped for icon 8; -1 outside the
 // grid for icon 13.
-EP_DEV long colofs(int x, int y, const Grid& g) {
+EP_DEV int colofs(int x, int y, const Grid& g) {
   if (g.periodic) {
     x = (x + g.nx) % g.nx; y = (y + g.ny) % g.ny;
   } else if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) {
     return -1;
   }
-  return (long)x + (long)g.nx * y;
+  return x + g.nx * y;
 }
-// f at column offset c (from colofs) and plane z, with the boundary rule in z.
 // Offset of plane z from a field's base, wrapped for icon 8; -1 when the
 // plane lies outside the grid for icon 13 (its values are zero).  It is
 // uniform over the workgroup: the z-marches form it once per plane and share
 // it between the loads of that plane (formed inside every load, the wrap and
 // the 64-bit product sat under each load's lane branch and were repeated per
 // load: some 30 scalar instructions each).
-EP_DEV long zofs(int z, const Grid& g) {
+EP_DEV int zofs(int z, const Grid& g) {
   if (z < 0 || z >= g.nz) {
     if (!g.periodic) return -1;
     z = (z + g.nz) % g.nz;
   }
-  return (long)g.nx * g.ny * z;
+  return g.nx * g.ny * z;
 }
 // f at column c (colofs) of the plane at offset zo (zofs); zero outside the grid.
-template <typename V> EP_DEV V ldz(const V* f, long c, long zo) {
-  return (c < 0 || zo < 0) ? vzero((V*)nullptr) : f[c + zo];
+template <typename V> EP_DEV V ldz(const V* f, int c, int zo) {
+  return (c < 0 || zo < 0) ? vzero((V*)nullptr) : ld32(f, (unsigned)(c + zo));
 }
 
 template <int TX> struct Tile {
@@ -101,7 +104,7 @@ template <int TX> struct TilePos {
 // Halo point h of a tile plane -> LDS offset (lo) and global column (go).
 // Points 0 .. 12 TY-1 are the x strips (6 columns left and right of the
 // tile), the rest the y strips (6 rows below and above).
-template <int TX> EP_DEV void halo_point(int h, int bx, int by, const Grid& g, int& lo, long& go) {
+template <int TX> EP_DEV void halo_point(int h, int bx, int by, const Grid& g, int& lo, int& go) {
   constexpr int TY = Tile<TX>::TY, LX = Tile<TX>::LX;
   if (h >= Tile<TX>::H) { lo = -1; go = -1; return; }
   int lx, ly;
@@ -129,22 +132,21 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
   constexpr int LX = Tile<TX>::LX, PL = Tile<TX>::PL, NH = Tile<TX>::NH;
   const int tx = tp.tx, ty = tp.ty, z0 = tp.z0, z1 = tp.z1;
   if (z0 >= z1) return;  // uniform over the group
-  const long col = colofs(tp.x, tp.y, g);
-  const long plane = (long)g.nx * g.ny;
-  int lo[NH];
-  long go[NH];
+  const int col = colofs(tp.x, tp.y, g);
+  const unsigned plane = (unsigned)(g.nx * g.ny);
+  int lo[NH], go[NH];
 #pragma unroll
   for (int k = 0; k < NH; k++) halo_point<TX>(threadIdx.x + k * kThreads, tp.bx, tp.by, g, lo[k], go[k]);
   V q[13], hv[NH];
   const V zero = vzero((V*)nullptr);
 #pragma unroll
   for (int k = 0; k < 13; k++) q[k] = ldz(f, col, zofs(z0 - 6 + k, g));
-  const long zo0 = zofs(z0, g);
+  const int zo0 = zofs(z0, g);
 #pragma unroll
   for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], zo0);
-  using PV = decltype(pre(0L));
+  using PV = decltype(pre(0u));
   PV p0{};
-  if (tp.inside) p0 = pre(col + plane * z0);
+  if (tp.inside) p0 = pre((unsigned)col + plane * (unsigned)z0);
   lds[(ty + 6) * LX + tx + 6] = q[6];
 #pragma unroll
   for (int k = 0; k < NH; k++)
@@ -157,10 +159,10 @@ EP_DEV void zmarch(const V* __restrict__ f, const Grid& g, const StenCoef<T>& sc
     PV p1{};
     if (more) {
       qn = ldz(f, col, zofs(z + 7, g));
-      const long zo = zofs(z + 1, g);
+      const int zo = zofs(z + 1, g);
 #pragma unroll
       for (int k = 0; k < NH; k++) hv[k] = ldz(f, go[k], zo);
-      if (tp.inside) p1 = pre(col + plane * (z + 1));
+      if (tp.inside) p1 = pre((unsigned)col + plane * (unsigned)(z + 1));
     }
     const V* pl = lds + buf * PL;
     V d1[3], d2[3];
@@ -270,19 +272,21 @@ EP_DEV void k_pc_predict_body(const StepArgs<T, V, P>& a) {
   const T c4o3dt = T(4.0 / 3.0) * (T)a.dtv[s], c112 = T(112.0 / 121.0);
   struct Op { V o3, h1, h2, pc; T p4, u; };
   zmarch<T, V, TX, false, true>(cur, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
-    [&](long i) { return Op{o3[i], h1[i], h2[i], pc[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](unsigned i) {
+      return Op{ld32(o3, i), ld32(h1, i), ld32(h2, i), ld32(pc, i), (T)ld32(pot4, i), (T)ld32(uimp, i)};
+    },
     [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
-      const long i = a.g.idx(x, y, z);
+      const unsigned i = a.g.at(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
       if (a.rt) hp = times_i(hp);
       V pred = o.o3 + c4o3dt * ((T(2) * hp - o.h1) + T(2) * o.h2);
       V pn = pred - c112 * o.pc;
-      o3[i] = pn;  // modified psi goes to the retired psi_{n-3} buffer
-      pc[i] = pred;
-      h2[i] = hp;
+      st32(o3, i, pn);  // modified psi goes to the retired psi_{n-3} buffer
+      st32(pc, i, pred);
+      st32(h2, i, hp);
       const T d = norm2(pn);
-      den[i] = (P)d;
+      st32(den, i, (P)d);
       v[1] += (double)d;
     });
   block_sum<2>(v, red);
@@ -315,19 +319,21 @@ EP_DEV void k_pc_correct_body(const StepArgs<T, V, P>& a) {
   struct Op { V o3, o2, h1, h2, pc; T p4, u; };
   const TilePos<TX> tp(a.g, a.zc);
   zmarch<T, V, TX, false, true>(cur, a.g, a.sc, tp, lds,
-    [&](long i) { return Op{o3[i], o2[i], h1[i], h2[i], pc[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](unsigned i) {
+      return Op{ld32(o3, i), ld32(o2, i), ld32(h1, i), ld32(h2, i), ld32(pc, i), (T)ld32(pot4, i), (T)ld32(uimp, i)};
+    },
     [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
-      const long i = a.g.idx(x, y, z);
+      const unsigned i = a.g.at(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
       if (a.rt) hp = times_i(hp);
       V corr = T(0.125) * ((T(9) * o.o3 - o.o2) + dt3 * ((hp + T(2) * o.h1) - o.h2));
       V pcv = o.pc - corr;
       V pn = corr + c9 * pcv;
-      pc[i] = pcv;
-      out[i] = pn;
+      st32(pc, i, pcv);
+      st32(out, i, pn);
       T d = norm2(pn);
-      if (wden) den[i] = (P)d;
+      if (wden) st32(den, i, (P)d);
       v[0] += (double)(c9 * sqrt(norm2(pcv)));
       v[1] += (double)d;
     });
@@ -356,19 +362,19 @@ EP_DEV void k_rk_stage_body(const StepArgs<T, V, P>& a) {
   double v[2] = {0.0, 0.0};
   struct Op { V q; T p4, u; };
   zmarch<T, V, TX, false, true>(cur, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
-    [&](long i) { return Op{q[i], (T)pot4[i], (T)uimp[i]}; },
+    [&](unsigned i) { return Op{ld32(q, i), (T)ld32(pot4, i), (T)ld32(uimp, i)}; },
     [&](int x, int y, int z, bool in, V psi, V(&)[3], V(&d2)[3], const Op& o) {
       if (!in) return;
-      const long i = a.g.idx(x, y, z);
+      const unsigned i = a.g.at(x, y, z);
       V hp = hpsi_of(psi, d2, a.h2o2m4, o.p4, o.u);
       if (a.rt) hp = times_i(hp);
       V a2 = a.arun * (hp - a.brun * o.q);
-      q[i] = (o.q + T(3) * a2) - a.crun * hp;
-      if (a.first_stage) h1[i] = hp;
+      st32(q, i, (o.q + T(3) * a2) - a.crun * hp);
+      if (a.first_stage) st32(h1, i, hp);
       V pn = psi + dt * a2;
-      out[i] = pn;
+      st32(out, i, pn);
       T d = norm2(pn);
-      den[i] = (P)d;
+      st32(den, i, (P)d);
       v[1] += (double)d;
     });
   block_sum<2>(v, red);
@@ -409,15 +415,15 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
   const T sc = a.dscale ? (T)a.dscale[s] : T(1);  // the stencil is linear: scale its results
   struct Op { T cg, fa; };
   zmarch<T, T, TX, true, false>(den, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds,
-    [&](long i) { return Op{dencg[i], otc ? denalf[i] : T(0)}; },
+    [&](unsigned i) { return Op{ld32(dencg, i), otc ? ld32(denalf, i) : T(0)}; },
     [&](int x, int y, int z, bool in, T d, T(&d1)[3], T(&)[3], const Op& o) {
       if (!in) return;
-      const long i = a.g.idx(x, y, z);
+      const unsigned i = a.g.at(x, y, z);
       const T gx = sc * d1[0], gy = sc * d1[1], gz = sc * d1[2];
-      dx[i] = gx; dy[i] = gy; dz[i] = gz;
-      s0[i] = (sc * d) * o.cg * (a.cp4 + a.cpp4 * o.cg);
+      st32(dx, i, gx); st32(dy, i, gy); st32(dz, i, gz);
+      st32(s0, i, (sc * d) * o.cg * (a.cp4 + a.cpp4 * o.cg));
       if (otc) {  // o.fa holds falfs
-        s1[i] = o.fa * gx; s2[i] = o.fa * gy; s3[i] = o.fa * gz;
+        st32(s1, i, o.fa * gx); st32(s2, i, o.fa * gy); st32(s3, i, o.fa * gz);
       }
     });
 }
@@ -454,9 +460,9 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   const TilePos<TX> tp(g, a.zc);
   if (tp.z0 >= tp.z1) return;
   const int tx = tp.tx, ty = tp.ty;
-  const long plane = (long)g.nx * g.ny;
-  const long col = colofs(tp.x, tp.y, g);
-  long cx_[KX], cy_[KY];
+  const unsigned plane = (unsigned)(g.nx * g.ny);
+  const int col = colofs(tp.x, tp.y, g);
+  int cx_[KX], cy_[KY];
 #pragma unroll
   for (int k = 0; k < KX; k++) {
     const int e = threadIdx.x + k * kThreads, r = e / LX, c = e - LX * (e / LX);
@@ -467,19 +473,21 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
     const int e = threadIdx.x + k * kThreads, r = e / TX, c = e - TX * (e / TX);
     cy_[k] = e < NYS ? colofs(tp.bx * TX + c, tp.by * TY + r - 6, g) : -1;
   }
-  auto prod = [&](const T* comp, long c, long zo) { return ldz(fa, c, zo) * ldz(comp, c, zo); };
+  auto prod = [&](const T* comp, int c, int zo) { return ldz(fa, c, zo) * ldz(comp, c, zo); };
   struct Op { T dx, dy, dz, ix, iy, iz; };
-  auto pre = [&](long i) { return Op{dx[i], dy[i], dz[i], ix[i], iy[i], iz[i]}; };
+  auto pre = [&](unsigned i) {
+    return Op{ld32(dx, i), ld32(dy, i), ld32(dz, i), ld32(ix, i), ld32(iy, i), ld32(iz, i)};
+  };
   T q[13], hx[KX], hy[KY];
 #pragma unroll
   for (int k = 0; k < 13; k++) q[k] = prod(iz, col, zofs(tp.z0 - 6 + k, g));
-  const long zo0 = zofs(tp.z0, g);
+  const int zo0 = zofs(tp.z0, g);
 #pragma unroll
   for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo0);
 #pragma unroll
   for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo0);
   Op p0{};
-  if (tp.inside) p0 = pre(col + plane * tp.z0);
+  if (tp.inside) p0 = pre((unsigned)col + plane * (unsigned)tp.z0);
 #pragma unroll
   for (int k = 0; k < KX; k++)
     if (threadIdx.x + k * kThreads < NXS) lx_[threadIdx.x + k * kThreads] = hx[k];
@@ -494,12 +502,12 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
     Op p1{};
     if (more) {
       qn = prod(iz, col, zofs(z + 7, g));
-      const long zo = zofs(z + 1, g);
+      const int zo = zofs(z + 1, g);
 #pragma unroll
       for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo);
 #pragma unroll
       for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo);
-      if (tp.inside) p1 = pre(col + plane * (z + 1));
+      if (tp.inside) p1 = pre((unsigned)col + plane * (unsigned)(z + 1));
     }
     if (tp.inside) {
       const T* px = lx_ + buf * NXS;
@@ -511,9 +519,9 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
         ay += a.sc.d1(1, m) * py[(ty + m) * TX + tx];
         az += a.sc.d1(2, m) * q[m];
       }
-      const long i = g.idx(tp.x, tp.y, z);
-      div[i] = (ax + ay) + az;
-      src[i] = (p0.dx * p0.ix + p0.dy * p0.iy) + p0.dz * p0.iz;
+      const unsigned i = g.at(tp.x, tp.y, z);
+      st32(div, i, (ax + ay) + az);
+      st32(src, i, (p0.dx * p0.ix + p0.dy * p0.iy) + p0.dz * p0.iz);
     }
     if (more) {
       T* nx_ = lx_ + (buf ^ 1) * NXS;
@@ -568,7 +576,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(EnergyArgs<T> a) {
   double v[kEnergySums];
 #pragma unroll
   for (int k = 0; k < kEnergySums; k++) v[k] = 0.0;
-  zmarch<T, V, TX, true, true>(psi, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds, [](long) { return NoPre{}; },
+  zmarch<T, V, TX, true, true>(psi, a.g, a.sc, TilePos<TX>(a.g, a.zc), lds, [](unsigned) { return NoPre{}; },
     [&](int x, int y, int z, bool in, V p, V(&d1)[3], V(&d2)[3], const NoPre&) {
       if (!in) return;
       const long i = so + a.g.idx(x, y, z);
