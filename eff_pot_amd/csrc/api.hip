@@ -561,12 +561,33 @@ template <typename T> int fft_zconv(effpot_ctx* ctx, State<T>& s, ConvArgs<T>& a
   const int C = th / n1_of(ctx->nz);
   if (packed) a.nw = (ctx->ny + C - 1) / C;  // groups of the packed column, launched first
   dim3 g(a.nw + (a.lm.nlines + C - 1) / C, ngroups, ctx->B);
+  // One output per input (stages B and C): the instance that keeps no spectrum
+  // or next factors between outputs holds 91 VGPRs instead of 162 at 128 FP32
+  // (occupancy 5 instead of 3; stage B 81 -> 77 us, C 22 -> 19 us at 128^3 x 4).
+  // In FP64 it measured slower at 128^3 (step 544 -> 547 us) and is not used.
+  bool one = true;
+  for (int k = 0; k < ngroups; k++) one = one && a.nout[k] == 1;
+  if constexpr (sizeof(T) == 4) {
+    if (one) {
+      if (th == 512) {
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, 512, true>), g, dim3(512), 0, ctx->st, a)
+        FFT_SWITCH(ctx->nz, M_)
+#undef M_
+      } else {
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, kThreads, true>), g, dim3(kThreads), 0, ctx->st, a)
+        FFT_SWITCH(ctx->nz, M_)
+#undef M_
+      }
+      LAUNCHCHK();
+      return EFFPOT_OK;
+    }
+  }
   if (th == 512) {
-#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, 512>), g, dim3(512), 0, ctx->st, a)
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, 512, false>), g, dim3(512), 0, ctx->st, a)
     FFT_SWITCH(ctx->nz, M_)
 #undef M_
   } else {
-#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, kThreads>), g, dim3(kThreads), 0, ctx->st, a)
+#define M_(NN) hipLaunchKernelGGL((k_fft_zconv<T, NN, kThreads, false>), g, dim3(kThreads), 0, ctx->st, a)
     FFT_SWITCH(ctx->nz, M_)
 #undef M_
   }

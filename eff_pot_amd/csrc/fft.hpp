@@ -334,7 +334,7 @@ template <typename T> struct ConvArgs {
 // positions 0 and 1 hold ky = 0 and ny/2 (their own partners), then ky = m and
 // ny - m at positions 2m and 2m + 1.  The partner's spectrum comes through LDS
 // (rows of N + 1: no bank conflicts).
-template <typename T, int N, int TH>
+template <typename T, int N, int TH, bool ONE>
 EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
   constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   const int g = blockIdx.y;
@@ -367,7 +367,7 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
     const int kz = kk1 + N1 * k2, km = kz == 0 ? 0 : N - kz;
     part[k2] = conjg(lds[pc * (N + 1) + km]);
   }
-  const int nout = a.nout[g];
+  const int nout = ONE ? 1 : a.nout[g];
   for (int j = 0; j < nout; j++) {
     const T* K = a.kw[g][j];
     cx<T> v[N2];
@@ -388,16 +388,18 @@ EP_DEV void zconv_packed_column(const ConvArgs<T>& a, cx<T>* lds, int gx) {
 }
 
 // TH = 256 or 512 threads: C = 16 or 32 lines per group (128 B or 256 B runs
-// of FP32 complex per z plane); the autotuner picks one per context.  In the
+// of FP32 complex per z plane); the autotuner picks one per context.  ONE:
+// every input has one output, so nothing is kept between outputs (a separate
+// instance because the register allocation follows the longest path).  In the
 // packed layout the first nw groups (blockIdx.x) convolve the packed column,
 // so that small piece of work runs alongside the other lines.
-template <typename T, int N, int TH = kThreads>
+template <typename T, int N, int TH = kThreads, bool ONE = false>
 __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
   constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, C = TH / N1;
   if (!a.active[blockIdx.z]) return;
   __shared__ cx<T> lds[(N + 1) * C];
   if ((int)blockIdx.x < a.nw) {
-    zconv_packed_column<T, N, TH>(a, lds, blockIdx.x);
+    zconv_packed_column<T, N, TH, ONE>(a, lds, blockIdx.x);
     return;
   }
   const int g = blockIdx.y;
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(TH) void k_fft_zconv(ConvArgs<T> a) {
     const cx<T> v = ld32(in, ln.at(i));
     return valid ? v : cx<T>{T(0), T(0)};
   };
-  const int nout = a.nout[g];
+  const int nout = ONE ? 1 : a.nout[g];
   // k-space factors of this thread's outputs: the first output's are loaded
   // before the forward transform, each next output's during the previous
   // inverse transform, so their latency overlaps arithmetic
