@@ -265,7 +265,7 @@ def run_corrector(a):
                        mode=a.mode, batch=B, stencil_tile=list(tile), ms_per_launch=ms, bytes_per_launch=byt)
 
 
-POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s), OTC
+POTEN_LAUNCHES = [  # name, algorithmic bytes per slot in units of (N s, M s), OTC, plane transforms
     ("A x r2c (1 field)", 1, 2), ("A y fwd (1)", 0, 4), ("A z conv (1 in, 3 out)", 0, 2 + 6 + 3),
     ("A y inv (3)", 0, 12), ("A x c2r (3)", 3, 6), ("k_grad", 10, 0),
     ("B x r2c (4)", 4, 8), ("B y fwd (4)", 0, 16), ("B z conv (4 in, 4 out)", 0, 20), ("B y inv (4)", 0, 16),
@@ -287,6 +287,11 @@ def poten_launch_bytes(k, n, mode, B, plane, iplane=0):
     if iplane and k in (3, 9, 15):
         return 0
     _, cn, cm = POTEN_LAUNCHES[k]
+    if not plane and k in (5, 6):
+        # separate x pass: stage B's x transform reads dx, dy, dz and falfs
+        # (once, as an array) and forms falfs * d{x,y,z}den as it loads, so
+        # k_grad stores 4 fields instead of 7
+        cn = 7 if k == 5 else 5
     b = s * (cn * N + cm * M)
     if k == 17 and mode == "mixed":
         b += 4 * N

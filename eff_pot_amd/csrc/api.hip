@@ -372,11 +372,11 @@ int n1_of(int n) {
 
 // rscale: [slot] factor the first field is read with (or null)
 template <typename T> int fft_x_r2c(effpot_ctx* ctx, State<T>& s, T* const* in, cx<T>* const* out, int nf, const int* act,
-                                    const double* rscale = nullptr) {
+                                    const double* rscale = nullptr, const T* rmul = nullptr) {
   XArgs<T> a{};
   for (int f = 0; f < nf; f++) { a.r[f] = in[f]; a.c[f] = out[f]; }
   a.rslot = ctx->N; a.cslot = ctx->M; a.nrows = ctx->ny * ctx->nz; a.scale = T(1); a.twN = s.tw[0]; a.active = act;
-  a.rscale = rscale;
+  a.rscale = rscale; a.rmul = rmul;
   a.pitch = ctx->nxp;
   const int C = kThreads / n1_of(ctx->nx);
   dim3 g((ctx->ny * ctx->nz / 2 + C - 1) / C, nf, ctx->B);
@@ -758,7 +758,10 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.otc = otc;
     a.cp4 = (T)ctx->p.cp4; a.cpp4 = (T)ctx->p.cpp4; a.den0s = (T)ctx->p.den0s;
     a.den = s.den; a.dencg = s.dencg; a.denalf = s.denalf; a.dx = s.dx; a.dy = s.dy; a.dz = s.dz;
-    a.s0 = s.src[0]; a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3]; a.dscale = dscale;
+    // with separate x and y passes falfs * d{x,y,z}den are formed by stage B's
+    // x transform from dx, dy, dz and not stored
+    a.s0 = s.src[0]; a.dscale = dscale;
+    if (pl) { a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3]; }
     dim3 g = sgrid(ctx);
 #define M_(TXX) hipLaunchKernelGGL((k_grad<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
     TX_SWITCH(ctx->tx, M_)
@@ -768,9 +771,18 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
   // stage B: correlation convolution [+ alpha_s integrals]
   {
     const int nf = otc ? 4 : 1;
+    // sources: the correlation source and falfs * d{x,y,z}den.  The x pass
+    // forms the products as it reads dx, dy, dz (k_grad then stores 4 fields
+    // instead of 7: 128^3 FP64 step 545 -> 538 us); the plane kernel, whose
+    // loads are its latency chain, reads the stored products (with the
+    // products formed on load its stage-B launch took 99-120 us against 67 at
+    // 128^3 x 4 FP32, more than k_grad saved)
     T* in[4] = {s.src[0], s.src[1], s.src[2], s.src[3]};
+    if (!pl) { in[1] = s.dx; in[2] = s.dy; in[3] = s.dz; }
     cx<T>* sp[4] = {s.spec[4], s.spec[5], s.spec[6], s.spec[7]};
-    if (sel(6)) RC(pl ? fft_plane_fwd(ctx, s, in, sp, nf, act, pk) : fft_x_r2c(ctx, s, in, sp, nf, act));
+    const T* fa = otc ? s.denalf : nullptr;  // denalf holds falfs
+    if (sel(6))
+      RC(pl ? fft_plane_fwd(ctx, s, in, sp, nf, act, pk) : fft_x_r2c(ctx, s, in, sp, nf, act, (const double*)nullptr, fa));
     if (sel(7) && !pl) RC(fft_strided(ctx, s, 1, sp, nf, -1, act));
     ConvArgs<T> ca{};
     for (int g = 0; g < nf; g++) {

@@ -460,6 +460,7 @@ template <typename T> struct XArgs {
   int pitch;  // complex elements per half-spectrum row (nx/2+1 padded)
   T scale;    // c2r only: out = aff0[f] + (scale*aff1[f]) * x
   const double* rscale;  // r2c only: [slot] factor field 0 is read with, or null
+  const T* rmul;         // r2c only: fields 1.. are read times rmul (a field laid out like r), or null
   T aff0[4], aff1[4];
   const cx<T>* twN;
   const int* active;
@@ -477,6 +478,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
   const int pair0 = blockIdx.x * C;  // pairs of rows
   const int npairs = rm.nrows / 2;
   const T sc = (rm.rscale && f == 0) ? (T)rm.rscale[blockIdx.z] : T(1);
+  const T* mul = (rm.rmul && f > 0) ? rm.rmul + (long)blockIdx.z * rm.rslot : nullptr;
   // stage: z_c[i] = a_c[i] + i b_c[i]
   for (int e = threadIdx.x; e < C * N; e += kThreads) {
     const int c = e / N, i = e - c * N;
@@ -485,6 +487,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_r2c(XArgs<T> rm) {
     if (pr < npairs) {
       const T* ra = src + (long)(2 * pr) * N;
       v = cx<T>{sc * ra[i], sc * ra[N + i]};
+      if (mul) {
+        const T* ma = mul + (long)(2 * pr) * N;
+        v = cx<T>{v.x * ma[i], v.y * ma[N + i]};
+      }
     }
     lds[i * P + c] = v;
   }

@@ -399,7 +399,7 @@ template <typename T> struct PotenArgs {
   T cp4, cpp4, den0s;
   const T* den; const T* dencg; const T* denalf;
   T* dx; T* dy; T* dz;
-  T* s0; T* s1; T* s2; T* s3;  // sources: correlation, falfs*d{x,y,z}den
+  T* s0; T* s1; T* s2; T* s3;  // sources: correlation, falfs*d{x,y,z}den (s1..s3 null: not stored)
   const double* dscale;        // [slot] factor den is read with (the step's density to n4 atoms), or null
 };
 template <typename T, int TX>
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
       const T gx = sc * d1[0], gy = sc * d1[1], gz = sc * d1[2];
       st32(dx, i, gx); st32(dy, i, gy); st32(dz, i, gz);
       st32(s0, i, (sc * d) * o.cg * (a.cp4 + a.cpp4 * o.cg));
-      if (otc) {  // o.fa holds falfs
+      if (otc && s1) {  // o.fa holds falfs; with s1 null the forward transform forms these products
         st32(s1, i, o.fa * gx); st32(s2, i, o.fa * gy); st32(s3, i, o.fa * gz);
       }
     });
