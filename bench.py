@@ -287,6 +287,11 @@ def poten_launch_bytes(k, n, mode, B, plane, iplane=0):
     if iplane and k in (3, 9, 15):
         return 0
     _, cn, cm = POTEN_LAUNCHES[k]
+    if plane and mode == "mixed" and k in (5, 11):
+        # FP32 fields with the plane transforms: k_div forms the density
+        # gradient from den (reads falfs, int x/y/z, den; writes 2), so k_grad
+        # does not store dx, dy, dz (writes 4)
+        cn = 7
     if not plane and k in (5, 6):
         # separate x pass: stage B's x transform reads dx, dy, dz and falfs
         # (once, as an array) and forms falfs * d{x,y,z}den as it loads, so

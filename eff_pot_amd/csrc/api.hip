@@ -720,11 +720,17 @@ template <int K> int reduce(effpot_ctx* ctx, int mode, const int* act, int* ring
 // 11 k_div; 12-16 stage C; 17 k_pot4; with the fused plane transforms 0/6/12
 // are the x+y forward transforms and 1/7/13 are empty).
 // dscale: [slot] factor the density is read with (the step's density scaled to
-// n4 atoms, from step_den_scale), or null.
+// n4 atoms, from step_den_scale), or null.  grads: store the density gradient
+// (dx, dy, dz) also where k_div forms its own (FP32 field downloads).
 template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, int only = -1,
-                                  const double* dscale = nullptr) {
+                                  const double* dscale = nullptr, bool grads = false) {
   const bool otc = ctx->p.core == EFFPOT_CORE_OTC;
   const bool pl = plane_ok<T>(ctx);  // forward x and y passes fused: launches 1, 7, 13 are empty
+  // FP32 fields with the plane transforms: k_div forms the density gradient
+  // from den and k_grad stores 4 fields instead of 7 (the FP64 gradient is
+  // kept: the energy and the impurity force read it; with separate x passes
+  // stage B's x transform reads it)
+  const bool regrad = sizeof(T) == 4 && otc && pl && !grads;
   const bool ipl = iplane_ok<T>(ctx);  // inverse y and x passes fused: launches 3, 9, 15 are empty
   const bool pk = packed_ok<T>(ctx);   // packed half spectra between the plane kernels (implies pl and ipl)
   auto sel = [only](int k) { return only < 0 || only == k; };
@@ -762,6 +768,7 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     // x transform from dx, dy, dz and not stored
     a.s0 = s.src[0]; a.dscale = dscale;
     if (pl) { a.s1 = s.src[1]; a.s2 = s.src[2]; a.s3 = s.src[3]; }
+    if (regrad) a.dx = a.dy = a.dz = nullptr;
     dim3 g = sgrid(ctx);
 #define M_(TXX) hipLaunchKernelGGL((k_grad<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
     TX_SWITCH(ctx->tx, M_)
@@ -802,14 +809,25 @@ template <typename T> int poten_t(effpot_ctx* ctx, State<T>& s, const int* act, 
     DivArgs<T> a{};
     a.g = grid_of(ctx); a.sc = s.sc; a.zc = ctx->zc; a.active = act; a.slot = ctx->N; a.den0s = (T)ctx->p.den0s;
     a.denalf = s.denalf; a.ix = s.intx; a.iy = s.inty; a.iz = s.intz; a.dx = s.dx; a.dy = s.dy; a.dz = s.dz;
-    a.src = s.src[0]; a.div = s.div;
+    a.src = s.src[0]; a.div = s.div; a.den = s.den; a.dscale = dscale;
     dim3 g = sgrid(ctx);
-#define M_(TXX) hipLaunchKernelGGL((k_div<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
     if (sel(11)) {
-      TX_SWITCH(ctx->tx, M_)
+      bool done = false;
+      if constexpr (sizeof(T) == 4) {
+        if (regrad) {
+#define M_(TXX) hipLaunchKernelGGL((k_div<T, TXX, true>), g, dim3(kThreads), 0, ctx->st, a)
+          TX_SWITCH(ctx->tx, M_)
+#undef M_
+          done = true;
+        }
+      }
+      if (!done) {
+#define M_(TXX) hipLaunchKernelGGL((k_div<T, TXX>), g, dim3(kThreads), 0, ctx->st, a)
+        TX_SWITCH(ctx->tx, M_)
+#undef M_
+      }
       LAUNCHCHK();
     }
-#undef M_
     // stage C: FFT(sum_i d_i den int_i) * kalfs
     T* in[1] = {s.src[0]};
     cx<T>* sp[1] = {s.spec[4]};
@@ -1403,7 +1421,7 @@ int effpot_poten_fp32(effpot_ctx* ctx) {
   if (!ctx->mixed) return ctx->fail(EFFPOT_ERR_ARG, "FP32 fields exist in a mixed-precision context only");
   hipSetDevice(ctx->dev);
   RC(den_from_cur(ctx, ctx->d_active, true));
-  RC(poten_t(ctx, ctx->s32, ctx->d_active));
+  RC(poten_t(ctx, ctx->s32, ctx->d_active, -1, nullptr, true));  // the gradient stored for downloads
   HIPCHK(hipStreamSynchronize(ctx->st));
   return EFFPOT_OK;
 }

@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
   __shared__ T lds[2 * Tile<TX>::PL];
   const long so = (long)s * a.slot;
   const T* den = a.den + so; const T* dencg = a.dencg + so; const T* denalf = a.denalf + so;
-  T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
+  T *dx = a.dx ? a.dx + so : nullptr, *dy = a.dy ? a.dy + so : nullptr, *dz = a.dz ? a.dz + so : nullptr;
   T *s0 = a.s0 + so, *s1 = a.s1 + so, *s2 = a.s2 + so, *s3 = a.s3 + so;
   const bool otc = a.otc;
   const T sc = a.dscale ? (T)a.dscale[s] : T(1);  // the stencil is linear: scale its results
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(kThreads) void k_grad(PotenArgs<T> a) {
       if (!in) return;
       const unsigned i = a.g.at(x, y, z);
       const T gx = sc * d1[0], gy = sc * d1[1], gz = sc * d1[2];
-      st32(dx, i, gx); st32(dy, i, gy); st32(dz, i, gz);
+      if (dx) { st32(dx, i, gx); st32(dy, i, gy); st32(dz, i, gz); }  // null: k_div forms them (GRAD)
       st32(s0, i, (sc * d) * o.cg * (a.cp4 + a.cpp4 * o.cg));
       if (otc && s1) {  // o.fa holds falfs; with s1 null the forward transform forms these products
         st32(s1, i, o.fa * gx); st32(s2, i, o.fa * gy); st32(s3, i, o.fa * gz);
@@ -440,9 +440,15 @@ template <typename T> struct DivArgs {
   T den0s;
   const T* denalf; const T* ix; const T* iy; const T* iz;
   const T* dx; const T* dy; const T* dz;
+  const T* den; const double* dscale;  // GRAD: the density and its [slot] factor (or null)
   T* src; T* div;
 };
-template <typename T, int TX>
+// GRAD: the density gradient of the source is formed here from den (k_grad's
+// 13-point sums in k_grad's order; equal to FP32 rounding, as the compiler
+// contracts the two kernels' sums into FMAs its own way) instead of read from
+// dx, dy, dz, which k_grad then does not store: k_grad is bound by its stores
+// (7 fields written; 4 without dx, dy, dz).
+template <typename T, int TX, bool GRAD = false>
 __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   constexpr int TY = Tile<TX>::TY, LX = Tile<TX>::LX, LY = Tile<TX>::LY;
   constexpr int NXS = TY * LX, NYS = LY * TX;  // x strip (x halo), y strip (y halo) of a plane
@@ -451,11 +457,15 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   if (!a.active[s]) return;
   __shared__ T lx_[2 * NXS];  // falfs*intx
   __shared__ T ly_[2 * NYS];  // falfs*inty
+  __shared__ T gx_[GRAD ? 2 * NXS : 1];  // den, x strip
+  __shared__ T gy_[GRAD ? 2 * NYS : 1];  // den, y strip
   const long so = (long)s * a.slot;
   const Grid& g = a.g;
   const T* fa = a.denalf + so;  // denalf holds falfs
   const T *ix = a.ix + so, *iy = a.iy + so, *iz = a.iz + so;
-  const T *dx = a.dx + so, *dy = a.dy + so, *dz = a.dz + so;
+  const T *dx = GRAD ? nullptr : a.dx + so, *dy = GRAD ? nullptr : a.dy + so, *dz = GRAD ? nullptr : a.dz + so;
+  const T* den = GRAD ? a.den + so : nullptr;
+  const T dsc = (GRAD && a.dscale) ? (T)a.dscale[s] : T(1);
   T *src = a.src + so, *div = a.div + so;
   const TilePos<TX> tp(g, a.zc);
   if (tp.z0 >= tp.z1) return;
@@ -476,9 +486,11 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   auto prod = [&](const T* comp, int c, int zo) { return ldz(fa, c, zo) * ldz(comp, c, zo); };
   struct Op { T dx, dy, dz, ix, iy, iz; };
   auto pre = [&](unsigned i) {
-    return Op{ld32(dx, i), ld32(dy, i), ld32(dz, i), ld32(ix, i), ld32(iy, i), ld32(iz, i)};
+    if constexpr (GRAD) return Op{T(0), T(0), T(0), ld32(ix, i), ld32(iy, i), ld32(iz, i)};
+    else return Op{ld32(dx, i), ld32(dy, i), ld32(dz, i), ld32(ix, i), ld32(iy, i), ld32(iz, i)};
   };
   T q[13], hx[KX], hy[KY];
+  T qd[GRAD ? 13 : 1], hdx[GRAD ? KX : 1], hdy[GRAD ? KY : 1];  // den: z column, strips
 #pragma unroll
   for (int k = 0; k < 13; k++) q[k] = prod(iz, col, zofs(tp.z0 - 6 + k, g));
   const int zo0 = zofs(tp.z0, g);
@@ -486,6 +498,14 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
   for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo0);
 #pragma unroll
   for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo0);
+  if constexpr (GRAD) {
+#pragma unroll
+    for (int k = 0; k < 13; k++) qd[k] = ldz(den, col, zofs(tp.z0 - 6 + k, g));
+#pragma unroll
+    for (int k = 0; k < KX; k++) hdx[k] = ldz(den, cx_[k], zo0);
+#pragma unroll
+    for (int k = 0; k < KY; k++) hdy[k] = ldz(den, cy_[k], zo0);
+  }
   Op p0{};
   if (tp.inside) p0 = pre((unsigned)col + plane * (unsigned)tp.z0);
 #pragma unroll
@@ -494,11 +514,19 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
 #pragma unroll
   for (int k = 0; k < KY; k++)
     if (threadIdx.x + k * kThreads < NYS) ly_[threadIdx.x + k * kThreads] = hy[k];
+  if constexpr (GRAD) {
+#pragma unroll
+    for (int k = 0; k < KX; k++)
+      if (threadIdx.x + k * kThreads < NXS) gx_[threadIdx.x + k * kThreads] = hdx[k];
+#pragma unroll
+    for (int k = 0; k < KY; k++)
+      if (threadIdx.x + k * kThreads < NYS) gy_[threadIdx.x + k * kThreads] = hdy[k];
+  }
   __syncthreads();
   int buf = 0;
   for (int z = tp.z0; z < tp.z1; z++) {
     const bool more = z + 1 < tp.z1;
-    T qn = T(0);
+    T qn = T(0), qdn = T(0);
     Op p1{};
     if (more) {
       qn = prod(iz, col, zofs(z + 7, g));
@@ -507,6 +535,13 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
       for (int k = 0; k < KX; k++) hx[k] = prod(ix, cx_[k], zo);
 #pragma unroll
       for (int k = 0; k < KY; k++) hy[k] = prod(iy, cy_[k], zo);
+      if constexpr (GRAD) {
+        qdn = ldz(den, col, zofs(z + 7, g));
+#pragma unroll
+        for (int k = 0; k < KX; k++) hdx[k] = ldz(den, cx_[k], zo);
+#pragma unroll
+        for (int k = 0; k < KY; k++) hdy[k] = ldz(den, cy_[k], zo);
+      }
       if (tp.inside) p1 = pre((unsigned)col + plane * (unsigned)(z + 1));
     }
     if (tp.inside) {
@@ -519,9 +554,22 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
         ay += a.sc.d1(1, m) * py[(ty + m) * TX + tx];
         az += a.sc.d1(2, m) * q[m];
       }
+      T gxv = p0.dx, gyv = p0.dy, gzv = p0.dz;
+      if constexpr (GRAD) {  // k_grad's sums (zmarch, D1) on the same values
+        const T* gxs = gx_ + buf * NXS;
+        const T* gys = gy_ + buf * NYS;
+        T bx = T(0), by = T(0), bz = T(0);
+#pragma unroll
+        for (int m = 0; m < 13; m++) {
+          bx = vfma(a.sc.d1(0, m), gxs[ty * LX + tx + m], bx);
+          by = vfma(a.sc.d1(1, m), gys[(ty + m) * TX + tx], by);
+          bz = vfma(a.sc.d1(2, m), qd[m], bz);
+        }
+        gxv = dsc * bx; gyv = dsc * by; gzv = dsc * bz;
+      }
       const unsigned i = g.at(tp.x, tp.y, z);
       st32(div, i, (ax + ay) + az);
-      st32(src, i, (p0.dx * p0.ix + p0.dy * p0.iy) + p0.dz * p0.iz);
+      st32(src, i, (gxv * p0.ix + gyv * p0.iy) + gzv * p0.iz);
     }
     if (more) {
       T* nx_ = lx_ + (buf ^ 1) * NXS;
@@ -532,11 +580,26 @@ __global__ __launch_bounds__(kThreads) void k_div(DivArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < KY; k++)
         if (threadIdx.x + k * kThreads < NYS) ny_[threadIdx.x + k * kThreads] = hy[k];
+      if constexpr (GRAD) {
+        T* ngx = gx_ + (buf ^ 1) * NXS;
+        T* ngy = gy_ + (buf ^ 1) * NYS;
+#pragma unroll
+        for (int k = 0; k < KX; k++)
+          if (threadIdx.x + k * kThreads < NXS) ngx[threadIdx.x + k * kThreads] = hdx[k];
+#pragma unroll
+        for (int k = 0; k < KY; k++)
+          if (threadIdx.x + k * kThreads < NYS) ngy[threadIdx.x + k * kThreads] = hdy[k];
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 12; k++) q[k] = q[k + 1];
     q[12] = qn;
+    if constexpr (GRAD) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) qd[k] = qd[k + 1];
+      qd[12] = qdn;
+    }
     p0 = p1;
     buf ^= 1;
   }

@@ -84,3 +84,34 @@ def test_fp32_fields_match_oracle(n, h, natoms, zr, zth, packed, gpu_available, 
     print({k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
     for k, v in worst.items():
         assert v < 2e-6, (k, v)
+
+
+@pytest.mark.parametrize("n,h,natoms,zr", [CASES[0], CASES[2]])
+def test_div_gradient_matches_stored_gradient(n, h, natoms, zr, gpu_available):
+    """With FP32 fields and the plane transforms the relaxation's poten does
+    not store the density gradient: k_div forms it from the density with
+    k_grad's 13-point sums in k_grad's order.  Its mean field agrees with the
+    one poten_fp32 makes with the gradient stored by k_grad and read back by
+    k_div to FP32 rounding: the compiler contracts the sums of the two kernels
+    into FMAs differently (measured 8.3e-9 relative at 32^3).  Bar 1e-7
+    relative L2 and 1e-6 relative pointwise, far inside the 2e-6 the FP32
+    fields keep against the FP64 oracle."""
+    print(f"[fp32 div gradient] {n}^3", flush=True)
+    R = 2.22 * natoms ** (1 / 3)
+    imp = make_morse(rimp=(0.0, 0.0, zr * R))
+    p = make_params(n, xmax=n * h / 2, n4=natoms, core=CORE_OTC, h_core=2.377, precision=PREC_MIXED,
+                    psi_complex=0)
+    with ep.Context(p) as c:
+        assert c.plane_fused(True) != 0, "the case must run the plane transforms"
+        c.init_fermi(0.0218, R, 0.6, 1e-6, 616 + n)
+        c.set_impurity(imp)
+        c.poten_fp32()  # gradient stored and read back
+        stored = {k: c.field_fp32(k) for k in ("pot4", "ualphas", "sto1")}
+        c.kernel_time(2, reps=1)  # the relaxation's poten of the same density: gradient formed in k_div
+        formed = {k: c.field_fp32(k) for k in ("pot4", "ualphas", "sto1")}
+    for k in stored:
+        assert np.all(np.isfinite(formed[k])), k
+        r = rel(formed[k], stored[k])
+        m = np.max(np.abs(formed[k].astype(np.float64) - stored[k])) / np.max(np.abs(stored[k]))
+        print(k, f"{r:.2e} {m:.2e}", flush=True)
+        assert r < 1e-7 and m < 1e-6, (k, r, m)
