@@ -199,6 +199,23 @@ def run_cfg0(a):
                        E_tot_K=r.e.etot, E_imp_K=r.e.eimpu, step_ms=1000 * el / a.steps / 2000)
 
 
+def host_info():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, OpenMP
+    threads, and whether an FFTW3 library is installed (the oracle does not
+    link one: it runs its own FFT)."""
+    import ctypes.util
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "fftw3_installed": ctypes.util.find_library("fftw3") is not None, "oracle_fft": "own (no FFTW)"}
+
+
 def run_scan(a):
     from eff_pot_amd import scan
     prec = 1 if a.mode == "mixed" else 0
@@ -465,7 +482,8 @@ def run_oneshot(a):
         el_cpu = time.perf_counter() - t3
         o.close()
         cpu = {"value": 1.0 / el_cpu, "unit": "snapshots/s", "cores": int(os.environ.get("OMP_NUM_THREADS", "1")),
-               "kind": "port", "sample": "potenimp + poten + energy of one 128^3 snapshot (resident)"}
+               "kind": "port", "sample": "potenimp + poten + energy of one 128^3 snapshot (resident)",
+               "host": host_info()}
     for f_ in paths + txt:
         os.remove(f_)
     os.rmdir(d)
@@ -684,7 +702,8 @@ def main():
                                 "kind": "port",
                                 "sample": f"{ncpu} FP64 PC steps (+ renormalisation) of the same {a.n}^3 workload "
                                           f"after 3 RK steps, {tc:.3f} s/step, scaled by the GPU's "
-                                          f"{steps_per_solve:.0f} steps per converged solve"}
+                                          f"{steps_per_solve:.0f} steps per converged solve",
+                                "host": host_info()}
     print(json.dumps(line), flush=True)
 
 
